@@ -1,0 +1,22 @@
+"""MI355X (gfx950) exact squared EDT and local thickness of bone-morphometry.
+
+Python face of the C ABI in include/bm.h, mirroring the reference crate's
+interface for this path (bone-morphometry 0.7.0-rc.3):
+
+  BinaryVolume.new / phase_inside    src/volume.rs:12-41, 84-105
+  Phase                               src/edt.rs:8-32
+  euclidean_distance_transform_squared  src/edt.rs:57-156
+  local_thickness                     src/local_thickness.rs:16-44
+  analyze_thickness_spacing           src/metrics.rs:64-132 (thickness/spacing part)
+  FeatureMode                         src/metrics.rs:12-24
+
+Every call runs on the GPU through libbm.so; errors raise BmError carrying the
+reference's message.
+"""
+from .api import (BinaryVolume, BmError, DistributionStats, FeatureMode, Phase,  # noqa: F401
+                  ThicknessSpacing, analyze_thickness_spacing,
+                  euclidean_distance_transform_squared, local_thickness,
+                  local_thickness_stages)
+from . import synth  # noqa: F401
+
+__version__ = "0.1.0"

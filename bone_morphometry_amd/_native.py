@@ -1,0 +1,130 @@
+"""ctypes binding of include/bm.h (libbm.so) and of the synthetic generator.
+
+The library is loaded from this package directory.  It is the only compute path:
+there is no CPU fallback, and a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbm.so")
+SYNTH_PATH = os.path.join(HERE, "libbm_synth.so")
+
+BM_OK = 0
+BM_ERR_INVALID = 1
+BM_ERR_DEVICE = 2
+BM_ERR_UNSUPPORTED = 3
+
+# Every symbol include/bm.h declares.
+EXPORTS = (
+    "bm_last_error", "bm_version", "bm_device_count", "bm_binary_volume_check", "bm_phase_inside",
+    "bm_euclidean_distance_transform_squared", "bm_local_thickness",
+    "bm_analyze_thickness_spacing", "bm_euclidean_distance_transform_squared_device",
+    "bm_local_thickness_device", "bm_analyze_thickness_spacing_device",
+    "bm_local_thickness_stages", "bm_profile_enable", "bm_profile_read",
+)
+
+
+class BmVolume(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint64), ("width", ctypes.c_uint64),
+                ("height", ctypes.c_uint64), ("depth", ctypes.c_uint64)]
+
+
+class BmDistributionStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "mean", "standard_deviation", "maximum", "median", "percentile_10", "percentile_25",
+        "percentile_75", "percentile_90", "interquartile_range", "coefficient_of_variation")] + [
+        ("count", ctypes.c_uint64)]
+
+
+class BmThicknessSpacing(ctypes.Structure):
+    _fields_ = [("thickness", BmDistributionStats), ("spacing", BmDistributionStats),
+                ("bone_volume", ctypes.c_double), ("total_volume", ctypes.c_double),
+                ("bone_volume_fraction", ctypes.c_double)]
+
+
+class BmKernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
+                ("total_ms", ctypes.c_double)]
+
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """The loaded libbm.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libbm.so is not built; run __graft_entry__.build() or "
+                              "python bone_morphometry_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P, U64, I, VP = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p
+        VOL = ctypes.POINTER(BmVolume)
+        L.bm_last_error.restype = ctypes.c_char_p
+        L.bm_last_error.argtypes = []
+        L.bm_version.restype = I
+        L.bm_device_count.argtypes = [ctypes.POINTER(I)]
+        L.bm_binary_volume_check.argtypes = [VOL]
+        L.bm_phase_inside.argtypes = [VOL, VOL, I, P, U64]
+        L.bm_euclidean_distance_transform_squared.argtypes = [VOL, I, P, U64]
+        L.bm_local_thickness.argtypes = [VOL, I, P, U64]
+        L.bm_analyze_thickness_spacing.argtypes = [VOL, VOL, ctypes.POINTER(ctypes.c_double), I,
+                                                   ctypes.POINTER(BmThicknessSpacing)]
+        L.bm_euclidean_distance_transform_squared_device.argtypes = [P, U64, U64, U64, I, P, VP]
+        L.bm_local_thickness_device.argtypes = [P, U64, U64, U64, I, P, VP]
+        L.bm_analyze_thickness_spacing_device.argtypes = [
+            P, P, U64, U64, U64, ctypes.POINTER(ctypes.c_double), I,
+            ctypes.POINTER(BmThicknessSpacing), VP]
+        L.bm_local_thickness_stages.argtypes = [VOL, I, P, P, P, P, ctypes.POINTER(U64),
+                                                ctypes.POINTER(U64)]
+        L.bm_profile_enable.argtypes = [I]
+        L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("bm_last_error", "bm_version"):
+                f.restype = I
+        _lib = L
+    return _lib
+
+
+def synth():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise ImportError("libbm_synth.so is not built")
+        S = ctypes.CDLL(SYNTH_PATH)
+        P, Z, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double
+        S.bm_synth_volume.restype = D
+        S.bm_synth_volume.argtypes = [P, Z, Z, Z, D, D, D, D, ctypes.c_uint64, ctypes.c_int]
+        S.bm_synth_roi_cylinder.restype = None
+        S.bm_synth_roi_cylinder.argtypes = [P, Z, Z, Z, D, D, D]
+        S.bm_synth_roi_ellipsoid.restype = None
+        S.bm_synth_roi_ellipsoid.argtypes = [P, Z, Z, Z, D, D, D, D, D, D]
+        _synth = S
+    return _synth
+
+
+def device_count():
+    """HIP devices visible (0 when none or when libbm.so is missing)."""
+    try:
+        c = ctypes.c_int(0)
+        lib().bm_device_count(ctypes.byref(c))
+        return c.value
+    except (ImportError, OSError):
+        return 0
+
+
+def profile_enable(on=True):
+    lib().bm_profile_enable(1 if on else 0)
+
+
+def profile_read(max_kernels=64):
+    """{kernel name: (launches, total_ms)} since the last read."""
+    arr = (BmKernelTime * max_kernels)()
+    n = ctypes.c_int(0)
+    rc = lib().bm_profile_read(arr, max_kernels, ctypes.byref(n))
+    if rc != BM_OK:
+        raise RuntimeError(lib().bm_last_error().decode())
+    return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}

@@ -1,0 +1,162 @@
+"""Reference-shaped calls over libbm.so (see the package docstring)."""
+import ctypes
+import dataclasses
+import enum
+
+import numpy as np
+
+from . import _native
+
+
+class BmError(RuntimeError):
+    """A failed call; .code is the BM_ERR_* value, str() the reference's message."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def _check(rc):
+    if rc != _native.BM_OK:
+        raise BmError(rc, _native.lib().bm_last_error().decode())
+
+
+class Phase(enum.IntEnum):
+    """src/edt.rs:8-24."""
+    Foreground = 0
+    Background = 1
+
+    def is_selected(self, value):
+        return value != 0 if self is Phase.Foreground else value == 0
+
+
+class FeatureMode(enum.IntEnum):
+    """src/metrics.rs:12-24."""
+    Bonej = 0
+    Standard = 1
+    Classic = 2
+    Refined = 3
+    Experimental = 4
+
+
+class BinaryVolume:
+    """src/volume.rs:3-9: u8 voxels, index (z*height + y)*width + x."""
+
+    def __init__(self, data, width, height, depth):
+        self.data = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+        self.width, self.height, self.depth = int(width), int(height), int(depth)
+
+    @classmethod
+    def new(cls, data, width, height, depth):
+        """BinaryVolume::new, src/volume.rs:12-41 (validation by the library)."""
+        v = cls(data, width, height, depth)
+        vol = v._c()
+        _check(_native.lib().bm_binary_volume_check(ctypes.byref(vol)))
+        return v
+
+    def __len__(self):
+        return self.data.size
+
+    def index(self, x, y, z):
+        return (z * self.height + y) * self.width + x
+
+    def _c(self):
+        return _native.BmVolume(self.data.ctypes.data, self.data.size, self.width, self.height,
+                                self.depth)
+
+    def phase_inside(self, roi, select_bone):
+        """src/volume.rs:84-105, computed on the device."""
+        out = np.empty(self.data.size, np.uint8)
+        a, b = self._c(), roi._c()
+        _check(_native.lib().bm_phase_inside(ctypes.byref(a), ctypes.byref(b), int(select_bone),
+                                             out.ctypes.data, out.size))
+        return BinaryVolume(out, self.width, self.height, self.depth)
+
+
+def euclidean_distance_transform_squared(volume, phase):
+    """src/edt.rs:57-156 -> np.ndarray[uint32] of len(volume)."""
+    out = np.empty(volume.data.size, np.uint32)
+    v = volume._c()
+    _check(_native.lib().bm_euclidean_distance_transform_squared(
+        ctypes.byref(v), int(phase), out.ctypes.data, out.size))
+    return out
+
+
+def local_thickness(volume, phase):
+    """src/local_thickness.rs:16-44 -> np.ndarray[float32] of diameters."""
+    out = np.empty(volume.data.size, np.float32)
+    v = volume._c()
+    _check(_native.lib().bm_local_thickness(ctypes.byref(v), int(phase), out.ctypes.data,
+                                            out.size))
+    return out
+
+
+def local_thickness_stages(volume, phase):
+    """Squared EDT, ridge flags, propagated r^2 map and final map of local_thickness."""
+    n = volume.data.size
+    sq = np.empty(n, np.uint32)
+    flags = np.empty(n, np.uint8)
+    prop = np.empty(n, np.uint32)
+    th = np.empty(n, np.float32)
+    rc, fv = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    v = volume._c()
+    _check(_native.lib().bm_local_thickness_stages(
+        ctypes.byref(v), int(phase), sq.ctypes.data, flags.ctypes.data, prop.ctypes.data,
+        th.ctypes.data, ctypes.byref(rc), ctypes.byref(fv)))
+    return dict(squared_edt=sq, ridge_flags=flags, propagated=prop, thickness=th,
+                ridge_count=rc.value, fill_visits=fv.value)
+
+
+@dataclasses.dataclass
+class DistributionStats:
+    """src/metrics.rs:26-38 (+ count)."""
+    mean: float
+    standard_deviation: float
+    maximum: float
+    median: float
+    percentile_10: float
+    percentile_25: float
+    percentile_75: float
+    percentile_90: float
+    interquartile_range: float
+    coefficient_of_variation: float
+    count: int
+
+    @classmethod
+    def _from_c(cls, s):
+        return cls(**{f.name: getattr(s, f.name) for f in dataclasses.fields(cls)})
+
+
+@dataclasses.dataclass
+class ThicknessSpacing:
+    thickness: DistributionStats
+    spacing: DistributionStats
+    bone_volume: float
+    total_volume: float
+    bone_volume_fraction: float
+
+    @classmethod
+    def _from_c(cls, r):
+        return cls(DistributionStats._from_c(r.thickness), DistributionStats._from_c(r.spacing),
+                   r.bone_volume, r.total_volume, r.bone_volume_fraction)
+
+
+def analyze_thickness_spacing(bone, roi, voxel_spacing, feature_mode=FeatureMode.Standard):
+    """Thickness/spacing part of analyze, src/metrics.rs:64-132."""
+    sp = (ctypes.c_double * 3)(*[float(v) for v in voxel_spacing])
+    out = _native.BmThicknessSpacing()
+    a, b = bone._c(), roi._c()
+    _check(_native.lib().bm_analyze_thickness_spacing(ctypes.byref(a), ctypes.byref(b), sp,
+                                                      int(feature_mode), ctypes.byref(out)))
+    return ThicknessSpacing._from_c(out)
+
+
+def analyze_thickness_spacing_device(d_bone_ptr, d_roi_ptr, width, height, depth, voxel_spacing,
+                                     feature_mode=FeatureMode.Standard, stream=None):
+    """Device-resident form: bone/roi are device pointers (e.g. torch tensor.data_ptr())."""
+    sp = (ctypes.c_double * 3)(*[float(v) for v in voxel_spacing])
+    out = _native.BmThicknessSpacing()
+    _check(_native.lib().bm_analyze_thickness_spacing_device(
+        d_bone_ptr, d_roi_ptr, width, height, depth, sp, int(feature_mode), ctypes.byref(out),
+        stream))
+    return ThicknessSpacing._from_c(out)

@@ -1,0 +1,496 @@
+// bm_api.cpp -- the reference crate's interface for this path, in C++, over
+// the device engine, and the C ABI of include/bm.h.
+//
+// Names, argument meaning and error messages follow bone-morphometry 0.7.0-rc.3:
+//   BinaryVolume::new           src/volume.rs:12-41
+//   Phase                       src/edt.rs:8-32
+//   euclidean_distance_transform_squared   src/edt.rs:57-156
+//   local_thickness             src/local_thickness.rs:16-44
+//   analyze (thickness/spacing part)       src/metrics.rs:64-132
+//   nonzero_stats               src/metrics.rs:204-269
+// The arithmetic of the statistics runs here on the host from exact sums
+// computed on the device; this file must be built with -ffp-contract=off so
+// count*sum_of_squares - sum*sum is evaluated unfused, as Rust does.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/bm.h"
+#include "bm_engine.h"
+#include "bm_quantiles.h"
+
+namespace bm {
+
+enum class Phase { Foreground, Background };
+
+static const char* phase_name(Phase p) {
+  return p == Phase::Foreground ? "foreground" : "background";
+}
+
+static std::string fmt(const char* f, ...) __attribute__((format(printf, 1, 2)));
+static std::string fmt(const char* f, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+// BinaryVolume (src/volume.rs:3-9) as a borrowed view; create() applies the
+// checks of BinaryVolume::new (src/volume.rs:12-41).
+struct BinaryVolume {
+  const uint8_t* data = nullptr;
+  uint64_t width = 0, height = 0, depth = 0;
+
+  static BinaryVolume create(const uint8_t* data, uint64_t len, uint64_t w, uint64_t h,
+                             uint64_t d) {
+    unsigned __int128 expected = (unsigned __int128)w * h * d;
+    if (expected > (unsigned __int128)SIZE_MAX) throw Error("volume dimensions overflow usize");
+    if ((unsigned __int128)len != expected)
+      throw Error(fmt("volume has %llu voxels, but dimensions %llux%llux%llu require %llu",
+                      (unsigned long long)len, (unsigned long long)w, (unsigned long long)h,
+                      (unsigned long long)d, (unsigned long long)expected));
+    if (w < 2 || h < 2 || d < 2)
+      throw Error(fmt("all dimensions must be at least 2 for 3-D morphometry; got %llux%llux%llu",
+                      (unsigned long long)w, (unsigned long long)h, (unsigned long long)d));
+    BinaryVolume v;
+    v.data = data;
+    v.width = w;
+    v.height = h;
+    v.depth = d;
+    return v;
+  }
+  uint64_t len() const { return width * height * depth; }
+  bool same_shape(const BinaryVolume& o) const {
+    return width == o.width && height == o.height && depth == o.depth;
+  }
+};
+
+// src/volume.rs:86-94 and src/metrics.rs:79-89
+static std::string shape_mismatch(const BinaryVolume& a, const BinaryVolume& b) {
+  return fmt("binary image dimensions %llux%llux%llu do not match ROI dimensions %llux%llux%llu",
+             (unsigned long long)a.width, (unsigned long long)a.height,
+             (unsigned long long)a.depth, (unsigned long long)b.width,
+             (unsigned long long)b.height, (unsigned long long)b.depth);
+}
+
+// Limits of the device path (u16 row distances, u32 voxel indices).
+static Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
+  if (w < 2 || h < 2 || d < 2)
+    throw Error(fmt("all dimensions must be at least 2 for 3-D morphometry; got %llux%llux%llu",
+                    (unsigned long long)w, (unsigned long long)h, (unsigned long long)d));
+  if (w > 16384 || h > 16384 || d > 16384 || (unsigned __int128)w * h * d > 0xFFFFFFFFull)
+    throw std::range_error(
+        fmt("volume %llux%llux%llu exceeds the device path's limits (each axis <= 16384, "
+            "fewer than 2^32 voxels)",
+            (unsigned long long)w, (unsigned long long)h, (unsigned long long)d));
+  Dims dims;
+  dims.w = (uint32_t)w;
+  dims.h = (uint32_t)h;
+  dims.d = (uint32_t)d;
+  return dims;
+}
+
+static Tuning default_tuning() { return Tuning(); }
+
+// f64 in the shortest form that reads back exactly, like Rust's {:?}.
+static std::string rust_f64(double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+  char buf[64];
+  for (int prec = 1; prec <= 17; ++prec) {
+    snprintf(buf, sizeof buf, "%.*g", prec, v);
+    if (std::strtod(buf, nullptr) == v) break;
+  }
+  std::string s = buf;
+  size_t e = s.find('e');
+  if (e != std::string::npos) {
+    std::string mant = s.substr(0, e), ex = s.substr(e + 1);
+    if (!ex.empty() && ex[0] == '+') ex = ex.substr(1);
+    size_t nz = 0;
+    bool neg = !ex.empty() && ex[0] == '-';
+    std::string digits = neg ? ex.substr(1) : ex;
+    while (nz + 1 < digits.size() && digits[nz] == '0') ++nz;
+    return mant + "e" + (neg ? "-" : "") + digits.substr(nz);
+  }
+  if (s.find('.') == std::string::npos) s += ".0";
+  return s;
+}
+
+static std::string spacing_list(const double sp[3]) {
+  return "[" + rust_f64(sp[0]) + ", " + rust_f64(sp[1]) + ", " + rust_f64(sp[2]) + "]";
+}
+
+static void fail_all_selected(Phase p) {
+  throw Error(std::string("distance transform is undefined for an all-") + phase_name(p) +
+              " volume unless an exterior convention is specified");
+}
+
+// ---------------------------------------------------------------------------
+// euclidean_distance_transform_squared (src/edt.rs:57-156), device resident.
+static void edt_device(Engine& e, const uint8_t* d_data, const Dims& dims, Phase p,
+                       uint32_t* d_out, hipStream_t s) {
+  PhaseBuffers b = e.buffers_for(dims);
+  PhaseSource src;
+  src.data = d_data;
+  src.want_nonzero = p == Phase::Foreground;
+  e.run_edt(src, dims, b, d_out, s, default_tuning());
+  PhaseStatus st;
+  BM_HIP(hipMemcpyAsync(&st, b.status, sizeof st, hipMemcpyDeviceToHost, s));
+  BM_HIP(hipStreamSynchronize(s));
+  if (!st.has_site) fail_all_selected(p);
+}
+
+void euclidean_distance_transform_squared(const BinaryVolume& v, Phase p, uint32_t* out) {
+  const Dims dims = device_dims(v.width, v.height, v.depth);
+  Engine& e = engine_for_device(current_device());
+  std::lock_guard<std::mutex> g(e.mutex());
+  hipStream_t s = e.stream();
+  const uint64_t n = dims.n();
+  uint8_t* d_in = static_cast<uint8_t*>(e.scratch_bytes(n, 0));
+  BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
+  PhaseBuffers b = e.buffers_for(dims);
+  edt_device(e, d_in, dims, p, b.d2, s);
+  BM_HIP(hipMemcpyAsync(out, b.d2, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  BM_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// local_thickness (src/local_thickness.rs:16-44), device resident.
+struct StageOutputs {
+  uint8_t* ridge_flags = nullptr;
+  uint32_t* propagated = nullptr;
+  PhaseStatus status{};
+};
+
+static void local_thickness_device(Engine& e, const uint8_t* d_data, const Dims& dims, Phase p,
+                                   float* d_out, hipStream_t s, StageOutputs* stages = nullptr) {
+  PhaseBuffers b = e.buffers_for(dims);
+  PhaseSource src;
+  src.data = d_data;
+  src.want_nonzero = p == Phase::Foreground;
+  e.run_edt(src, dims, b, b.d2, s, default_tuning());
+  e.run_thickness(dims, b, d_out, 1.0f, stats_lsb_exp(1.0f), s,
+                  stages ? stages->ridge_flags : nullptr, stages ? stages->propagated : nullptr);
+  PhaseStatus st;
+  BM_HIP(hipMemcpyAsync(&st, b.status, sizeof st, hipMemcpyDeviceToHost, s));
+  BM_HIP(hipStreamSynchronize(s));
+  if (!st.has_site) fail_all_selected(p);
+  if (stages) stages->status = st;
+}
+
+void local_thickness(const BinaryVolume& v, Phase p, float* out) {
+  const Dims dims = device_dims(v.width, v.height, v.depth);
+  Engine& e = engine_for_device(current_device());
+  std::lock_guard<std::mutex> g(e.mutex());
+  hipStream_t s = e.stream();
+  const uint64_t n = dims.n();
+  char* stage = static_cast<char*>(e.scratch_bytes(n * 5 + 256, 0));
+  uint8_t* d_in = reinterpret_cast<uint8_t*>(stage);
+  float* d_out = reinterpret_cast<float*>(stage + ((n + 255) / 256) * 256);
+  BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
+  local_thickness_device(e, d_in, dims, p, d_out, s);
+  BM_HIP(hipMemcpyAsync(out, d_out, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  BM_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// nonzero_stats (src/metrics.rs:204-269) from the exact device sums.
+static bm_distribution_stats distribution_stats(const StatsResult& r, const QuantileResult* q) {
+  if (r.flags) throw DeviceError("statistics accumulator left its exact range");
+  if (r.count == 0) throw Error("cannot compute statistics because the selected phase is empty");
+  const double count_f64 = (double)r.count;
+  const double mean = r.sum / count_f64;
+  const double centered = (count_f64 * r.sum_of_squares - r.sum * r.sum) / count_f64;
+  const double sd =
+      (r.count > 1 && centered > 0.0) ? std::sqrt(centered / (count_f64 - 1.0)) : 0.0;
+  bm_distribution_stats st;
+  st.mean = mean;
+  st.standard_deviation = sd;
+  st.maximum = r.maximum;
+  st.median = q ? q->median : NAN;
+  st.percentile_10 = q ? q->p10 : NAN;
+  st.percentile_25 = q ? q->p25 : NAN;
+  st.percentile_75 = q ? q->p75 : NAN;
+  st.percentile_90 = q ? q->p90 : NAN;
+  st.interquartile_range = st.percentile_75 - st.percentile_25;
+  st.coefficient_of_variation = sd / mean;
+  st.count = r.count;
+  return st;
+}
+
+// Thickness/spacing part of analyze (src/metrics.rs:64-132) on device-resident
+// bone and ROI.  Both phases are derived on the fly from (bone, roi)
+// (src/volume.rs:101); the map of a phase is only materialised when quantiles
+// are requested.
+static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_roi,
+                           const Dims& dims, const double spacing[3], int mode,
+                           bm_thickness_spacing* out, hipStream_t s) {
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(spacing[i]) || spacing[i] <= 0.0)
+      throw Error("voxel spacings must be finite and positive; got " + spacing_list(spacing));
+  const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
+  const float scale_f32 = (float)spacing[0];
+  const int lsb = stats_lsb_exp(scale_f32);
+  const uint64_t n = dims.n();
+  PhaseBuffers b = e.buffers_for(dims);
+  float* d_map = quantiles ? static_cast<float*>(e.scratch_bytes(n * sizeof(float), 1)) : nullptr;
+  PhaseStatus st[2];
+  StatsPartial sp[2];
+  QuantileResult qr[2];
+  for (int ph = 0; ph < 2; ++ph) {
+    PhaseSource src;
+    src.data = d_bone;
+    src.roi = d_roi;
+    src.want_nonzero = ph == 0 ? 1u : 0u;
+    e.run_edt(src, dims, b, b.d2, s, default_tuning());
+    e.run_thickness(dims, b, d_map, scale_f32, lsb, s);
+    BM_HIP(hipMemcpyAsync(&st[ph], b.status, sizeof(PhaseStatus), hipMemcpyDeviceToHost, s));
+    BM_HIP(hipMemcpyAsync(&sp[ph], b.stats, sizeof(StatsPartial), hipMemcpyDeviceToHost, s));
+    if (quantiles) {
+      BM_HIP(hipStreamSynchronize(s));
+      if (sp[ph].count > 0) qr[ph] = device_quantiles(e, d_map, n, sp[ph].count, scale_f32, s);
+    }
+  }
+  BM_HIP(hipStreamSynchronize(s));
+  // Error precedence of src/metrics.rs:90-104.
+  const uint64_t bone_voxels = st[0].selected, marrow_voxels = st[1].selected;
+  const uint64_t roi_voxels = bone_voxels + marrow_voxels;
+  if (roi_voxels == 0) throw Error("ROI mask contains no included voxels");
+  if (mode == BM_MODE_BONEJ && roi_voxels == n)
+    throw Error("BoneJ compatibility mode requires an ROI mask with both inside and outside voxels");
+  if (bone_voxels == 0) throw Error("ROI contains no bone voxels");
+  if (marrow_voxels == 0) throw Error("ROI contains no marrow voxels");
+  out->thickness = distribution_stats(finish_stats(sp[0], lsb), quantiles ? &qr[0] : nullptr);
+  out->spacing = distribution_stats(finish_stats(sp[1], lsb), quantiles ? &qr[1] : nullptr);
+  const double voxel_volume = spacing[0] * spacing[1] * spacing[2];
+  out->bone_volume = (double)bone_voxels * voxel_volume;
+  out->total_volume = (double)roi_voxels * voxel_volume;
+  out->bone_volume_fraction = out->bone_volume / out->total_volume;
+}
+
+void analyze_thickness_spacing(const BinaryVolume& bone, const BinaryVolume& roi,
+                               const double spacing[3], int mode, bm_thickness_spacing* out) {
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(spacing[i]) || spacing[i] <= 0.0)
+      throw Error("voxel spacings must be finite and positive; got " + spacing_list(spacing));
+  if (!bone.same_shape(roi)) throw Error(shape_mismatch(bone, roi));
+  const Dims dims = device_dims(bone.width, bone.height, bone.depth);
+  Engine& e = engine_for_device(current_device());
+  std::lock_guard<std::mutex> g(e.mutex());
+  hipStream_t s = e.stream();
+  const uint64_t n = dims.n();
+  uint8_t* d_in = static_cast<uint8_t*>(e.scratch_bytes(2 * n + 256, 0));
+  uint8_t* d_roi = d_in + ((n + 255) / 256) * 256;
+  BM_HIP(hipMemcpyAsync(d_in, bone.data, n, hipMemcpyHostToDevice, s));
+  BM_HIP(hipMemcpyAsync(d_roi, roi.data, n, hipMemcpyHostToDevice, s));
+  analyze_device(e, d_in, d_roi, dims, spacing, mode, out, s);
+}
+
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return BM_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return BM_ERR_INVALID;
+  } catch (const std::range_error& e) {
+    g_last_error = e.what();
+    return BM_ERR_UNSUPPORTED;
+  } catch (const DeviceError& e) {
+    g_last_error = e.what();
+    return BM_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BM_ERR_DEVICE;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return BM_ERR_DEVICE;
+  }
+}
+
+static BinaryVolume from_c(const bm_volume* v) {
+  if (!v || (!v->data && v->len)) throw Error("null volume");
+  return BinaryVolume::create(v->data, v->len, v->width, v->height, v->depth);
+}
+
+static Phase phase_from_c(int phase) {
+  if (phase == BM_PHASE_FOREGROUND) return Phase::Foreground;
+  if (phase == BM_PHASE_BACKGROUND) return Phase::Background;
+  throw Error(fmt("unknown phase %d", phase));
+}
+
+static void check_out_len(uint64_t out_len, uint64_t n) {
+  if (out_len != n)
+    throw Error(fmt("output buffer holds %llu values, the volume has %llu voxels",
+                    (unsigned long long)out_len, (unsigned long long)n));
+}
+
+static hipStream_t stream_or_default(Engine& e, void* stream) {
+  return stream ? static_cast<hipStream_t>(stream) : e.stream();
+}
+
+}  // namespace bm
+
+using namespace bm;
+
+extern "C" {
+
+const char* bm_last_error(void) { return g_last_error.c_str(); }
+
+int bm_version(void) { return 100; }
+
+int bm_profile_enable(int on) {
+  return guarded([&] { profile_enable(on != 0); });
+}
+
+int bm_profile_read(bm_kernel_time* out, int max, int* count) {
+  return guarded([&] {
+    static_assert(sizeof(bm_kernel_time) == sizeof(KernelTime), "layout");
+    int n = profile_read(reinterpret_cast<KernelTime*>(out), max);
+    if (count) *count = n;
+  });
+}
+
+int bm_device_count(int* count) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  if (count) *count = c;
+  return BM_OK;
+}
+
+int bm_binary_volume_check(const bm_volume* volume) {
+  return guarded([&] { (void)from_c(volume); });
+}
+
+int bm_phase_inside(const bm_volume* bone, const bm_volume* roi, int select_bone, uint8_t* out,
+                    uint64_t out_len) {
+  return guarded([&] {
+    BinaryVolume b = from_c(bone);
+    BinaryVolume r = from_c(roi);
+    if (!b.same_shape(r)) throw Error(shape_mismatch(b, r));
+    check_out_len(out_len, b.len());
+    const uint64_t n = b.len();
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    uint8_t* d = static_cast<uint8_t*>(e.scratch_bytes(3 * n + 512, 0));
+    uint8_t* d_roi = d + ((n + 255) / 256) * 256;
+    uint8_t* d_out = d_roi + ((n + 255) / 256) * 256;
+    BM_HIP(hipMemcpyAsync(d, b.data, n, hipMemcpyHostToDevice, s));
+    BM_HIP(hipMemcpyAsync(d_roi, r.data, n, hipMemcpyHostToDevice, s));
+    e.run_phase_inside(d, d_roi, n, select_bone != 0, d_out, s);
+    BM_HIP(hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int bm_euclidean_distance_transform_squared(const bm_volume* volume, int phase, uint32_t* out,
+                                            uint64_t out_len) {
+  return guarded([&] {
+    BinaryVolume v = from_c(volume);
+    Phase p = phase_from_c(phase);
+    check_out_len(out_len, v.len());
+    euclidean_distance_transform_squared(v, p, out);
+  });
+}
+
+int bm_local_thickness(const bm_volume* volume, int phase, float* out, uint64_t out_len) {
+  return guarded([&] {
+    BinaryVolume v = from_c(volume);
+    Phase p = phase_from_c(phase);
+    check_out_len(out_len, v.len());
+    local_thickness(v, p, out);
+  });
+}
+
+int bm_analyze_thickness_spacing(const bm_volume* bone, const bm_volume* roi,
+                                 const double voxel_spacing[3], int feature_mode,
+                                 bm_thickness_spacing* out) {
+  return guarded([&] {
+    BinaryVolume b = from_c(bone);
+    BinaryVolume r = from_c(roi);
+    analyze_thickness_spacing(b, r, voxel_spacing, feature_mode, out);
+  });
+}
+
+int bm_euclidean_distance_transform_squared_device(const uint8_t* d_data, uint64_t width,
+                                                   uint64_t height, uint64_t depth, int phase,
+                                                   uint32_t* d_out, void* stream) {
+  return guarded([&] {
+    const Dims dims = device_dims(width, height, depth);
+    Phase p = phase_from_c(phase);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    edt_device(e, d_data, dims, p, d_out, stream_or_default(e, stream));
+  });
+}
+
+int bm_local_thickness_device(const uint8_t* d_data, uint64_t width, uint64_t height,
+                              uint64_t depth, int phase, float* d_out, void* stream) {
+  return guarded([&] {
+    const Dims dims = device_dims(width, height, depth);
+    Phase p = phase_from_c(phase);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    local_thickness_device(e, d_data, dims, p, d_out, stream_or_default(e, stream));
+  });
+}
+
+int bm_analyze_thickness_spacing_device(const uint8_t* d_bone, const uint8_t* d_roi,
+                                        uint64_t width, uint64_t height, uint64_t depth,
+                                        const double voxel_spacing[3], int feature_mode,
+                                        bm_thickness_spacing* out, void* stream) {
+  return guarded([&] {
+    const Dims dims = device_dims(width, height, depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    analyze_device(e, d_bone, d_roi, dims, voxel_spacing, feature_mode, out,
+                   stream_or_default(e, stream));
+  });
+}
+
+int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squared_edt,
+                              uint8_t* ridge_flags, uint32_t* propagated, float* thickness,
+                              uint64_t* ridge_count, uint64_t* fill_visits) {
+  return guarded([&] {
+    BinaryVolume v = from_c(volume);
+    Phase p = phase_from_c(phase);
+    const Dims dims = device_dims(v.width, v.height, v.depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    const uint64_t n = dims.n();
+    const uint64_t a = ((n + 255) / 256) * 256;
+    char* stage = static_cast<char*>(e.scratch_bytes(a * 10, 0));
+    uint8_t* d_in = reinterpret_cast<uint8_t*>(stage);
+    uint8_t* d_flags = reinterpret_cast<uint8_t*>(stage + a);
+    uint32_t* d_prop = reinterpret_cast<uint32_t*>(stage + 2 * a);
+    float* d_map = reinterpret_cast<float*>(stage + 6 * a);
+    BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
+    StageOutputs so;
+    so.ridge_flags = d_flags;
+    so.propagated = d_prop;
+    local_thickness_device(e, d_in, dims, p, d_map, s, &so);
+    PhaseBuffers b = e.buffers_for(dims);
+    if (squared_edt) BM_HIP(hipMemcpyAsync(squared_edt, b.d2, n * 4, hipMemcpyDeviceToHost, s));
+    if (ridge_flags) BM_HIP(hipMemcpyAsync(ridge_flags, d_flags, n, hipMemcpyDeviceToHost, s));
+    if (propagated) BM_HIP(hipMemcpyAsync(propagated, d_prop, n * 4, hipMemcpyDeviceToHost, s));
+    if (thickness) BM_HIP(hipMemcpyAsync(thickness, d_map, n * 4, hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+    if (ridge_count) *ridge_count = so.status.ridge_count;
+    if (fill_visits) *fill_visits = so.status.fill_visits;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// bm_engine.h -- device engine for the exact squared EDT and local thickness.
+//
+// Everything here is stream-ordered device work on one GPU.  The host-facing
+// mirror of the reference crate (BinaryVolume, Phase, euclidean_distance_
+// transform_squared, local_thickness, analyze) lives in bm_api.cpp and calls
+// this engine; the C-ABI in include/bm.h is built on top of both.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bm {
+
+// Error carrying the reference's message text (anyhow::bail! strings).
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+struct DeviceError : std::runtime_error {
+  explicit DeviceError(const std::string& m) : std::runtime_error(m) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define BM_HIP(call) ::bm::hip_check((call), #call)
+
+// Sentinels of the EDT passes.
+constexpr uint32_t kInf16 = 0xFFFFu;      // X pass: no site in the row
+constexpr uint32_t kInf32 = 0x40000000u;  // Y/Z passes: no site reachable
+
+// Which voxels are selected: roi[i] != 0 (or no ROI) and
+// (data[i] != 0) == want_nonzero.  Phase::Foreground is want_nonzero = 1,
+// Phase::Background want_nonzero = 0 (src/edt.rs:17-24); the bone and marrow
+// phases of analyze are (bone, roi, 1) and (bone, roi, 0) (src/volume.rs:101).
+struct PhaseSource {
+  const uint8_t* data = nullptr;
+  const uint8_t* roi = nullptr;
+  uint32_t want_nonzero = 1;
+};
+
+struct Dims {
+  uint32_t w = 0, h = 0, d = 0;
+  uint64_t n() const { return (uint64_t)w * h * d; }
+  uint64_t max_d2() const {
+    return (uint64_t)(w - 1) * (w - 1) + (uint64_t)(h - 1) * (h - 1) + (uint64_t)(d - 1) * (d - 1);
+  }
+};
+
+// Per-phase device status, written by the kernels of one phase.
+struct PhaseStatus {
+  unsigned long long selected;     // voxels of the phase (X pass)
+  unsigned long long ridge_count;  // distance ridges found
+  unsigned long long fill_visits;  // candidate tests / ball visits of the fill
+  uint32_t has_site;               // any non-selected voxel (X pass)
+  uint32_t max_d2;                 // maximum squared distance (Z pass)
+  uint32_t error;                  // device-detected inconsistency (bit flags)
+  uint32_t pad;
+};
+
+// Exact statistics accumulator: fixed point with LSB 2^lsb_exp.
+struct StatsPartial {
+  unsigned long long count;
+  unsigned long long sum_lo, sum_hi;
+  unsigned long long sq_lo, sq_hi;
+  uint32_t max_bits;  // f32 bits of the largest value (positive floats order as u32)
+  uint32_t flags;
+};
+
+struct StatsResult {
+  uint64_t count = 0;
+  double sum = 0.0, sum_of_squares = 0.0, maximum = 0.0;
+  uint32_t flags = 0;
+};
+
+// Tunables of the kernels (chosen by the autotuner for the batched path).
+// A Y-pass tile holds up to y_rows rows of 64 columns (u16) in LDS: the whole
+// column when H <= y_rows, otherwise a chunk of y_rows - 2*y_halo rows plus a
+// halo on each side (candidates beyond the halo are read from HBM).  The Z
+// pass is the same with z_rows lines of 32 columns (u32).
+struct Tuning {
+  uint32_t y_rows = 512;
+  uint32_t y_halo = 64;
+  uint32_t z_rows = 512;
+  uint32_t z_halo = 64;
+};
+
+// Device buffers for one phase of one volume.
+struct PhaseBuffers {
+  uint16_t* dx = nullptr;       // X pass output (N u16)
+  uint32_t* h = nullptr;        // Y pass output, then the propagated map (N u32)
+  uint32_t* d2 = nullptr;       // squared EDT (N u32)
+  uint2* ridges = nullptr;      // (index, r^2) list, capacity N
+  uint32_t* tmpl = nullptr;     // 3 x (bound+1) templates
+  float* diam = nullptr;        // (bound+1) diameters 2*sqrt(m)
+  PhaseStatus* status = nullptr;
+  StatsPartial* partials = nullptr;  // kStatsBlocks
+  StatsPartial* stats = nullptr;     // final record
+  uint64_t bound = 0;
+};
+
+constexpr int kStatsBlocks = 1024;
+
+// A growable device arena, freed by the engine.
+class Arena {
+ public:
+  ~Arena();
+  void reserve(size_t bytes);
+  void* base() const { return base_; }
+  size_t capacity() const { return cap_; }
+
+ private:
+  void* base_ = nullptr;
+  size_t cap_ = 0;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  std::mutex& mutex() { return mu_; }
+
+  // Workspace for a volume of these dims (both phases share it sequentially).
+  PhaseBuffers buffers_for(const Dims& dims);
+  void* scratch_bytes(size_t bytes, int slot);  // extra staging buffers (H2D/D2H)
+
+  // Squared EDT into d2 (device pointer, may be buffers.d2).
+  void run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffers& b, uint32_t* d2,
+               hipStream_t s, const Tuning& t = Tuning());
+  // phase_inside (src/volume.rs:84-105) as a device map.
+  void run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t n, bool select_bone,
+                        uint8_t* out, hipStream_t s);
+  // Full local thickness from an EDT already in b.d2: ridges, fill, clean-up,
+  // phase mask, statistics.  map_out may be null (stats only).
+  void run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out, float scale_f32,
+                     int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out = nullptr,
+                     uint32_t* propagated_out = nullptr);
+
+ private:
+  int device_;
+  hipStream_t stream_ = nullptr;
+  std::mutex mu_;
+  Arena work_;
+  Arena stage_[3];
+};
+
+// Per-kernel timing with HIP events recorded on the launching stream (off by
+// default).  Used by the benchmark to measure kernel durations live.
+struct KernelTime {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+};
+void profile_enable(bool on);
+bool profile_enabled();
+hipEvent_t profile_begin();
+void profile_end(const char* name, hipEvent_t start, hipStream_t s);
+int profile_read(KernelTime* out, int max);  // synchronises, aggregates, resets
+
+// Launch a kernel, bracketed by profiling events when profiling is on.
+#define BM_LAUNCH(name, kernel, grid, block, lds, stream, ...)           \
+  do {                                                                   \
+    hipEvent_t bm_ev0_ = ::bm::profile_enabled() ? ::bm::profile_begin() \
+                                                 : nullptr;              \
+    if (bm_ev0_) BM_HIP(hipEventRecord(bm_ev0_, stream));                \
+    kernel<<<grid, block, lds, stream>>>(__VA_ARGS__);                   \
+    if (bm_ev0_) ::bm::profile_end(name, bm_ev0_, stream);               \
+  } while (0)
+
+// Engine of a device (created on first use; process lifetime).
+Engine& engine_for_device(int device);
+int current_device();
+
+// Host-side conversion of the exact accumulator into f64 sums.
+StatsResult finish_stats(const StatsPartial& p, int lsb_exp);
+int stats_lsb_exp(float scale_f32);
+
+}  // namespace bm

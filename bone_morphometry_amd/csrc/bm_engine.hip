@@ -1,0 +1,310 @@
+// bm_engine.hip -- host side of the device engine: arenas, launch sequences.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bm_engine.h"
+
+namespace bm {
+
+// Kernels (bm_kernels.hip).
+__global__ void k_edt_x(const uint8_t*, const uint8_t*, uint32_t, uint16_t*, uint32_t, uint32_t,
+                        PhaseStatus*);
+__global__ void k_edt_y(const uint16_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t);
+__global__ void k_edt_z(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t,
+                        uint32_t, PhaseStatus*);
+__global__ void k_luts(const PhaseStatus*, uint32_t, uint32_t*, float*);
+__global__ void k_ridges(const uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
+                         PhaseStatus*, uint2*);
+__global__ void k_fill_scatter(const uint2*, const PhaseStatus*, uint32_t, uint32_t, uint32_t,
+                               uint32_t*);
+__global__ void k_cleanup_stats(const uint32_t*, const uint32_t*, const float*,
+                                const PhaseStatus*, uint32_t, uint32_t, uint32_t, float, int,
+                                float*, StatsPartial*);
+__global__ void k_stats_final(const StatsPartial*, uint32_t, StatsPartial*);
+__global__ void k_fill_u32(uint32_t*, uint64_t, uint32_t);
+__global__ void k_fill_u8(uint8_t*, uint64_t, uint8_t);
+__global__ void k_ridge_flags(const uint2*, const PhaseStatus*, uint8_t*);
+__global__ void k_phase_inside(const uint8_t*, const uint8_t*, uint64_t, uint32_t, uint8_t*);
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw DeviceError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+  }
+}
+
+Arena::~Arena() {
+  if (base_) (void)hipFree(base_);
+}
+
+void Arena::reserve(size_t bytes) {
+  if (bytes <= cap_) return;
+  if (base_) {
+    BM_HIP(hipDeviceSynchronize());
+    BM_HIP(hipFree(base_));
+    base_ = nullptr;
+    cap_ = 0;
+  }
+  size_t want = bytes + bytes / 8;  // headroom against small growth
+  BM_HIP(hipMalloc(&base_, want));
+  cap_ = want;
+}
+
+Engine::Engine(int device) : device_(device) {
+  BM_HIP(hipSetDevice(device));
+  BM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+PhaseBuffers Engine::buffers_for(const Dims& dims) {
+  const uint64_t n = dims.n();
+  const uint64_t bound = dims.max_d2();
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  const size_t o_dx = take(n * sizeof(uint16_t));
+  const size_t o_h = take(n * sizeof(uint32_t));
+  const size_t o_d2 = take(n * sizeof(uint32_t));
+  const size_t o_r = take(n * sizeof(uint2));
+  const size_t o_t = take(3 * (bound + 1) * sizeof(uint32_t));
+  const size_t o_dm = take((bound + 1) * sizeof(float));
+  const size_t o_st = take(sizeof(PhaseStatus));
+  const size_t o_p = take(kStatsBlocks * sizeof(StatsPartial));
+  const size_t o_s = take(sizeof(StatsPartial));
+  work_.reserve(off);
+  char* b = static_cast<char*>(work_.base());
+  PhaseBuffers pb;
+  pb.dx = reinterpret_cast<uint16_t*>(b + o_dx);
+  pb.h = reinterpret_cast<uint32_t*>(b + o_h);
+  pb.d2 = reinterpret_cast<uint32_t*>(b + o_d2);
+  pb.ridges = reinterpret_cast<uint2*>(b + o_r);
+  pb.tmpl = reinterpret_cast<uint32_t*>(b + o_t);
+  pb.diam = reinterpret_cast<float*>(b + o_dm);
+  pb.status = reinterpret_cast<PhaseStatus*>(b + o_st);
+  pb.partials = reinterpret_cast<StatsPartial*>(b + o_p);
+  pb.stats = reinterpret_cast<StatsPartial*>(b + o_s);
+  pb.bound = bound;
+  return pb;
+}
+
+void* Engine::scratch_bytes(size_t bytes, int slot) {
+  stage_[slot].reserve(bytes);
+  return stage_[slot].base();
+}
+
+static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+void Engine::run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffers& b,
+                     uint32_t* d2, hipStream_t s, const Tuning& t) {
+  const uint32_t W = dims.w, H = dims.h, D = dims.d;
+  // status <- 0 (a small fill kernel, never a memset node: graph-safe)
+  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(b.status),
+                              sizeof(PhaseStatus) / sizeof(uint32_t), 0u);
+  const uint32_t rows = H * D;
+  BM_LAUNCH("edt_x", k_edt_x, grid_for((uint64_t)rows * 64, 256, 8192), 256, 0, s, src.data,
+            src.roi, src.want_nonzero, b.dx, W, rows, b.status);
+  {
+    const uint32_t chunk = H <= t.y_rows ? H : t.y_rows - 2 * t.y_halo;
+    const uint32_t halo = H <= t.y_rows ? 0 : t.y_halo;
+    const uint32_t rows_lds = std::min<uint32_t>(H, chunk + 2 * halo);
+    dim3 grid((W + 63) / 64, D, (H + chunk - 1) / chunk);
+    BM_LAUNCH("edt_y", k_edt_y, grid, 256, rows_lds * 64 * sizeof(uint16_t), s, b.dx, b.h, W, H,
+              chunk, halo);
+  }
+  {
+    const uint32_t chunk = D <= t.z_rows ? D : t.z_rows - 2 * t.z_halo;
+    const uint32_t halo = D <= t.z_rows ? 0 : t.z_halo;
+    const uint32_t rows_lds = std::min<uint32_t>(D, chunk + 2 * halo);
+    dim3 grid((W + 31) / 32, H, (D + chunk - 1) / chunk);
+    BM_LAUNCH("edt_z", k_edt_z, grid, 256, rows_lds * 32 * sizeof(uint32_t), s, b.h, d2, W, H, D,
+              chunk, halo, b.status);
+  }
+  BM_HIP(hipGetLastError());
+}
+
+void Engine::run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t n,
+                              bool select_bone, uint8_t* out, hipStream_t s) {
+  k_phase_inside<<<grid_for(n, 256 * 8, 65536), 256, 0, s>>>(bone, roi, n, select_bone ? 1u : 0u,
+                                                             out);
+  BM_HIP(hipGetLastError());
+}
+
+void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out,
+                           float scale_f32, int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out,
+                           uint32_t* propagated_out) {
+  const uint32_t W = dims.w, H = dims.h, D = dims.d;
+  const uint64_t n = dims.n();
+  const uint32_t bound = (uint32_t)b.bound;
+  BM_LAUNCH("luts", k_luts, grid_for((uint64_t)bound + 1, 256, 1u << 30), 256, 0, s, b.status,
+            bound, b.tmpl, b.diam);
+  BM_LAUNCH("ridges", k_ridges, grid_for(n, 256, 65536), 256, 0, s, b.d2, W, H, D, b.tmpl, bound,
+            b.status, b.ridges);
+  uint32_t* M = b.h;  // the Y-pass buffer is free once the Z pass has run
+  BM_LAUNCH("zero_map", k_fill_u32, grid_for(n, 256, 65536), 256, 0, s, M, n, 0u);
+  BM_LAUNCH("fill_scatter", k_fill_scatter, 8192, 256, 0, s, b.ridges, b.status, W, H, D, M);
+  if (ridge_flags_out) {
+    k_fill_u8<<<grid_for(n, 256, 65536), 256, 0, s>>>(ridge_flags_out, n, 0);
+    k_ridge_flags<<<4096, 256, 0, s>>>(b.ridges, b.status, ridge_flags_out);
+  }
+  if (propagated_out) BM_HIP(hipMemcpyAsync(propagated_out, M, n * 4, hipMemcpyDeviceToDevice, s));
+  BM_LAUNCH("cleanup_stats", k_cleanup_stats, kStatsBlocks, 256, 0, s, M, b.d2, b.diam, b.status,
+            W, H, D, scale_f32, lsb_exp, map_out, b.partials);
+  BM_LAUNCH("stats_final", k_stats_final, 1, 256, 0, s, b.partials, kStatsBlocks, b.stats);
+  BM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Engines live until the process ends and are never destroyed: freeing device
+// memory from a static destructor could run after the HIP runtime is gone.
+static std::mutex g_engines_mu;
+static std::map<int, Engine*>& engines() {
+  static std::map<int, Engine*>* m = new std::map<int, Engine*>();
+  return *m;
+}
+
+int current_device() {
+  int dev = 0;
+  BM_HIP(hipGetDevice(&dev));
+  return dev;
+}
+
+Engine& engine_for_device(int device) {
+  std::lock_guard<std::mutex> g(g_engines_mu);
+  auto& m = engines();
+  auto it = m.find(device);
+  if (it != m.end()) return *it->second;
+  Engine* e = new Engine(device);
+  m.emplace(device, e);
+  return *e;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel profiling.
+namespace {
+struct ProfRecord {
+  std::string name;
+  hipEvent_t start, end;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRecord>* g_prof_records = new std::vector<ProfRecord>();
+std::vector<hipEvent_t>* g_prof_pool = new std::vector<hipEvent_t>();
+
+hipEvent_t take_event() {
+  if (!g_prof_pool->empty()) {
+    hipEvent_t e = g_prof_pool->back();
+    g_prof_pool->pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  BM_HIP(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+void profile_enable(bool on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_on = on;
+}
+
+bool profile_enabled() { return g_prof_on; }
+
+hipEvent_t profile_begin() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  return take_event();
+}
+
+void profile_end(const char* name, hipEvent_t start, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  hipEvent_t end = take_event();
+  BM_HIP(hipEventRecord(end, s));
+  g_prof_records->push_back({name, start, end});
+}
+
+int profile_read(KernelTime* out, int max) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  std::vector<std::pair<std::string, std::pair<uint64_t, double>>> agg;
+  for (auto& r : *g_prof_records) {
+    BM_HIP(hipEventSynchronize(r.end));
+    float ms = 0.0f;
+    BM_HIP(hipEventElapsedTime(&ms, r.start, r.end));
+    bool found = false;
+    for (auto& a : agg)
+      if (a.first == r.name) {
+        a.second.first += 1;
+        a.second.second += ms;
+        found = true;
+        break;
+      }
+    if (!found) agg.push_back({r.name, {1, (double)ms}});
+    g_prof_pool->push_back(r.start);
+    g_prof_pool->push_back(r.end);
+  }
+  g_prof_records->clear();
+  int n = 0;
+  for (auto& a : agg) {
+    if (n >= max) break;
+    std::memset(out[n].name, 0, sizeof out[n].name);
+    std::strncpy(out[n].name, a.first.c_str(), sizeof out[n].name - 1);
+    out[n].launches = a.second.first;
+    out[n].total_ms = a.second.second;
+    ++n;
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Exact statistics on the host.
+int stats_lsb_exp(float scale_f32) {
+  int e = 0;
+  (void)std::frexp(scale_f32, &e);  // scale = f * 2^e, f in [0.5, 1)
+  return (e - 1) - 26;
+}
+
+// Correctly rounded (ties-to-even) conversion of a 128-bit integer to f64.
+static double u128_to_double(unsigned __int128 v) {
+  if (v == 0) return 0.0;
+  int nbits = 128 - (v >> 64 ? __builtin_clzll((unsigned long long)(v >> 64))
+                             : 64 + __builtin_clzll((unsigned long long)v));
+  if (nbits <= 53) return (double)(unsigned long long)v;
+  const int drop = nbits - 53;
+  unsigned __int128 top = v >> drop;
+  const unsigned __int128 rem = v - (top << drop);
+  const unsigned __int128 half = (unsigned __int128)1 << (drop - 1);
+  if (rem > half || (rem == half && (top & 1))) top += 1;
+  return std::ldexp((double)(unsigned long long)top, drop);  // top <= 2^53: exact
+}
+
+StatsResult finish_stats(const StatsPartial& p, int lsb_exp) {
+  StatsResult r;
+  r.count = p.count;
+  const unsigned __int128 s = ((unsigned __int128)p.sum_hi << 64) | p.sum_lo;
+  const unsigned __int128 s2 = ((unsigned __int128)p.sq_hi << 64) | p.sq_lo;
+  r.sum = std::ldexp(u128_to_double(s), lsb_exp);
+  r.sum_of_squares = std::ldexp(u128_to_double(s2), 2 * lsb_exp);
+  float mx;
+  std::memcpy(&mx, &p.max_bits, 4);
+  r.maximum = (double)mx;
+  r.flags = p.flags;
+  return r;
+}
+
+}  // namespace bm

@@ -1,0 +1,78 @@
+"""The benchmark and test workloads: seeded synthetic volumes with their ROIs.
+
+plain_small   128^3, sigma 3, BV/TV 0.25, full-cube ROI, 10 um, seed 1
+plain_512     512^3, sigma 4, BV/TV 0.15, z-cylinder ROI of diameter 460
+              centred at (255.5, 255.5), 10.5 um, seed 2
+large         1024 x 1024 x 1536, sigma (5, 5, 9), BV/TV 0.12, ellipsoid ROI
+              with semi-axes (500, 480, 740) about the centre, 8 um, seed 3
+batched       96 volumes, axes uniform in 64..384, sigma U[2, 6], BV/TV
+              U[0.08, 0.35], cylinder ROI of diameter U[0.70, 0.95]*min(W, H),
+              spacing U[6, 20] um, seed 4 (per-volume parameters from a numpy
+              Generator seeded with 4)
+"""
+import dataclasses
+
+import numpy as np
+
+from . import synth
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    width: int
+    height: int
+    depth: int
+    bone: np.ndarray  # u8 0/255, z-major
+    roi: np.ndarray   # u8 0/1
+    spacing: tuple
+
+    @property
+    def n(self):
+        return self.width * self.height * self.depth
+
+
+def plain_small(threads=0):
+    w = h = d = 128
+    bone = synth.volume(w, h, d, 3.0, 0.25, 1, threads)
+    return Workload("plain_small_128", w, h, d, bone, synth.roi_full(w, h, d), (10.0,) * 3)
+
+
+def plain_512(threads=0):
+    w = h = d = 512
+    bone = synth.volume(w, h, d, 4.0, 0.15, 2, threads)
+    roi = synth.roi_cylinder(w, h, d, 460.0, 255.5, 255.5)
+    return Workload("plain_512", w, h, d, bone, roi, (10.5,) * 3)
+
+
+def large(threads=0):
+    w, h, d = 1024, 1024, 1536
+    bone = synth.volume(w, h, d, (5.0, 5.0, 9.0), 0.12, 3, threads)
+    roi = synth.roi_ellipsoid(w, h, d, (500.0, 480.0, 740.0))
+    return Workload("large_1024x1024x1536", w, h, d, bone, roi, (8.0,) * 3)
+
+
+def batched_params(count=96, seed=4, lo=64, hi=384):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        w, h, d = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        sigma = float(rng.uniform(2.0, 6.0))
+        bvtv = float(rng.uniform(0.08, 0.35))
+        frac = float(rng.uniform(0.70, 0.95))
+        sp = float(rng.uniform(6.0, 20.0))
+        sub_seed = int(rng.integers(1, 2**62))
+        out.append(dict(width=w, height=h, depth=d, sigma=sigma, bv_tv=bvtv,
+                        diameter=frac * min(w, h), spacing=sp, seed=sub_seed))
+    return out
+
+
+def make_volume(p, threads=0):
+    w, h, d = p["width"], p["height"], p["depth"]
+    bone = synth.volume(w, h, d, p["sigma"], p["bv_tv"], p["seed"], threads)
+    roi = synth.roi_cylinder(w, h, d, p["diameter"])
+    return Workload(f"vol_{w}x{h}x{d}", w, h, d, bone, roi, (p["spacing"],) * 3)
+
+
+def batched(count=96, seed=4, threads=0, lo=64, hi=384):
+    return [make_volume(p, threads) for p in batched_params(count, seed, lo, hi)]

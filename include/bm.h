@@ -1,0 +1,157 @@
+/*
+ * bm.h -- C ABI of the MI355X (gfx950) exact squared EDT and local-thickness
+ * path of bone-morphometry 0.7.0-rc.3.
+ *
+ * Each entry point replaces one function of the reference crate (paths relative
+ * to the crate root) and keeps its argument meaning, its output layout (z-major,
+ * index (z*height + y)*width + x, src/volume.rs:44-46) and its error behaviour:
+ * every function returns BM_OK or an error code, and bm_last_error() returns the
+ * reference's message text (the anyhow::bail! strings) for the calling thread.
+ * Nothing unwinds across this boundary (the reference's release profile aborts
+ * on panic, Cargo.toml:23).
+ *
+ * Host-pointer entry points copy the input to the current HIP device, compute
+ * there, and copy the result back; outputs are caller-allocated.  The *_device
+ * variants take device pointers and an optional hipStream_t (NULL = the
+ * library's stream of that device) and leave the data resident.
+ *
+ * Thread safety: calls may come from any thread; work on one device is
+ * serialised by the library.
+ */
+#ifndef BM_H
+#define BM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_OK 0
+#define BM_ERR_INVALID 1     /* input rejected (the reference's bail! cases) */
+#define BM_ERR_DEVICE 2      /* HIP runtime error or device unavailable */
+#define BM_ERR_UNSUPPORTED 3 /* outside this implementation's size limits */
+
+/* Phase, src/edt.rs:8-24: FOREGROUND selects value != 0, BACKGROUND value == 0. */
+#define BM_PHASE_FOREGROUND 0
+#define BM_PHASE_BACKGROUND 1
+
+/* FeatureMode, src/metrics.rs:12-24 (declaration order). */
+#define BM_MODE_BONEJ 0
+#define BM_MODE_STANDARD 1
+#define BM_MODE_CLASSIC 2
+#define BM_MODE_REFINED 3
+#define BM_MODE_EXPERIMENTAL 4
+
+/* BinaryVolume, src/volume.rs:3-9: data[len] with len == width*height*depth,
+ * every dimension >= 2 (src/volume.rs:12-41). */
+typedef struct bm_volume {
+  const uint8_t *data;
+  uint64_t len;
+  uint64_t width, height, depth;
+} bm_volume;
+
+/* DistributionStats, src/metrics.rs:26-38 (plus the number of values). */
+typedef struct bm_distribution_stats {
+  double mean;
+  double standard_deviation;
+  double maximum;
+  double median;
+  double percentile_10;
+  double percentile_25;
+  double percentile_75;
+  double percentile_90;
+  double interquartile_range;
+  double coefficient_of_variation;
+  uint64_t count;
+} bm_distribution_stats;
+
+/* The thickness/spacing part of Metrics, src/metrics.rs:48-62. */
+typedef struct bm_thickness_spacing {
+  bm_distribution_stats thickness; /* Tb.Th, src/metrics.rs:111-117 */
+  bm_distribution_stats spacing;   /* Tb.Sp, src/metrics.rs:119-121 */
+  double bone_volume;              /* src/metrics.rs:123-132 */
+  double total_volume;
+  double bone_volume_fraction;
+} bm_thickness_spacing;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *bm_last_error(void);
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int bm_version(void);
+
+/* Number of HIP devices visible to the process (0 without a GPU). */
+int bm_device_count(int *count);
+
+/* Replaces the checks of BinaryVolume::new, src/volume.rs:12-41: returns
+ * BM_OK, or BM_ERR_INVALID with the reference's message. */
+int bm_binary_volume_check(const bm_volume *volume);
+
+/* Replaces BinaryVolume::phase_inside(&self, roi, select_bone),
+ * src/volume.rs:84-105: out[i] = (roi[i] != 0 && (bone[i] != 0) == select_bone),
+ * computed on the device.  Fails like the reference on a shape mismatch. */
+int bm_phase_inside(const bm_volume *bone, const bm_volume *roi, int select_bone, uint8_t *out,
+                    uint64_t out_len);
+
+/* Replaces euclidean_distance_transform_squared(&BinaryVolume, Phase)
+ * -> Result<Vec<u32>>, src/edt.rs:57-156.  out[out_len] with out_len == len.
+ * Fails like the reference when every voxel is selected (src/edt.rs:61-66). */
+int bm_euclidean_distance_transform_squared(const bm_volume *volume, int phase, uint32_t *out,
+                                            uint64_t out_len);
+
+/* Replaces local_thickness(&BinaryVolume, Phase) -> Result<Vec<f32>>,
+ * src/local_thickness.rs:16-44.  out[out_len] receives diameters in voxel
+ * units, 0 outside the selected phase. */
+int bm_local_thickness(const bm_volume *volume, int phase, float *out, uint64_t out_len);
+
+/* Replaces the thickness/spacing part of analyze(bone, roi, voxel_spacing,
+ * feature_mode), src/metrics.rs:64-132: the validation of :70-104, the two
+ * ROI-clipped phases of :97-98 (src/volume.rs:84-105), local thickness of
+ * each and nonzero_stats (src/metrics.rs:204-269) scaled by voxel_spacing[0],
+ * and BV, TV, BV/TV.  Quantiles are filled for STANDARD and EXPERIMENTAL,
+ * NaN otherwise, as in the reference. */
+int bm_analyze_thickness_spacing(const bm_volume *bone, const bm_volume *roi,
+                                 const double voxel_spacing[3], int feature_mode,
+                                 bm_thickness_spacing *out);
+
+/* Device-resident forms of the three calls above: d_* are device pointers on
+ * the current HIP device, stream is a hipStream_t or NULL.  Each returns once
+ * its result is complete (the statistics are copied to *out). */
+int bm_euclidean_distance_transform_squared_device(const uint8_t *d_data, uint64_t width,
+                                                   uint64_t height, uint64_t depth, int phase,
+                                                   uint32_t *d_out, void *stream);
+int bm_local_thickness_device(const uint8_t *d_data, uint64_t width, uint64_t height,
+                              uint64_t depth, int phase, float *d_out, void *stream);
+int bm_analyze_thickness_spacing_device(const uint8_t *d_bone, const uint8_t *d_roi,
+                                        uint64_t width, uint64_t height, uint64_t depth,
+                                        const double voxel_spacing[3], int feature_mode,
+                                        bm_thickness_spacing *out, void *stream);
+
+/* Intermediate products of local_thickness for parity tests and diagnostics
+ * (no counterpart in the reference's public API; these are the values of
+ * src/local_thickness.rs:17,20-26,28 before the clean-up).  Any output pointer
+ * may be NULL.  ridge_count/fill_visits receive the ridge count and the work
+ * counter of the sphere fill. */
+int bm_local_thickness_stages(const bm_volume *volume, int phase, uint32_t *squared_edt,
+                              uint8_t *ridge_flags, uint32_t *propagated, float *thickness,
+                              uint64_t *ridge_count, uint64_t *fill_visits);
+
+/* Kernel timing for benchmarks and diagnostics (no counterpart in the
+ * reference).  bm_profile_enable(1) brackets every kernel launch with HIP
+ * events on its stream; bm_profile_read() waits for them and returns, per
+ * kernel name, the launch count and total milliseconds since the last read. */
+typedef struct bm_kernel_time {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+} bm_kernel_time;
+int bm_profile_enable(int on);
+int bm_profile_read(bm_kernel_time *out, int max, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BM_H */

@@ -1,0 +1,583 @@
+/*
+ * oracle.c -- CPU restatement of bone-morphometry's exact squared EDT and
+ * Fiji-style local-thickness path (Tb.Th / Tb.Sp).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker and the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; the product library (bone_morphometry_amd/libbm.so) never
+ * links, loads or calls it.
+ *
+ * Every function follows the reference step for step and cites the Rust it
+ * restates (paths relative to the reference crate root).  The arithmetic is
+ * kept identical where the result depends on it:
+ *   - the EDT is the same f64 Felzenszwalb-Huttenlocher pass structure with
+ *     INF = 1e30 (src/edt.rs:6,57-223);
+ *   - the clean-up pass 2 sums in f32 in NEIGHBORS_26 order and divides by
+ *     (float)count (src/local_thickness.rs:243-277,285-312);
+ *   - nonzero_stats accumulates sequentially in f64 in linear order
+ *     (src/metrics.rs:204-241).
+ * Build with -ffp-contract=off so no product is fused into an FMA.
+ *
+ * Parallelism mirrors rayon's decomposition with OpenMP: EDT lines, the ridge
+ * filter and the ridge propagation (atomic max) run in parallel; distance_index,
+ * the clean-up and the statistics are serial, as in the reference.
+ *
+ * Pinning status of each function (how the tests check it):
+ *   oracle_phase_inside      known answer volume.rs:123-136 + closed form
+ *   oracle_edt_squared       known answers edt.rs:240-274, brute force,
+ *                            scipy-derived fixtures, single-site closed form
+ *   oracle_templates         known answer local_thickness.rs:395-401 and a
+ *                            brute-force lattice maximum
+ *   oracle_local_thickness   known answers local_thickness.rs:364-393, the
+ *                            propagate-from-all-voxels invariant, clean-up
+ *                            order independence (reverse traversal), bounds
+ *   oracle_nonzero_stats     known answer metrics.rs:315-326 + closed forms
+ *   oracle_quantile          known answer metrics.rs:304-313 + numpy linear
+ *                            quantiles
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORACLE_INF 1.0e30 /* src/edt.rs:6 */
+
+static __thread char g_err[512];
+
+const char *oracle_last_error(void) { return g_err; }
+
+static int fail(const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return 1;
+}
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* Phase::is_selected, src/edt.rs:17-24.  phase 0 = Foreground, 1 = Background. */
+static inline int is_selected(int phase, uint8_t v) { return phase == 0 ? v != 0 : v == 0; }
+
+/* BinaryVolume::new dimension rule, src/volume.rs:27-34. */
+static int check_dims(size_t w, size_t h, size_t d) {
+  if (w < 2 || h < 2 || d < 2) {
+    snprintf(g_err, sizeof g_err,
+             "all dimensions must be at least 2 for 3-D morphometry; got %zux%zux%zu", w, h, d);
+    return 1;
+  }
+  return 0;
+}
+
+/* BinaryVolume::phase_inside, src/volume.rs:84-105 (shape check done by caller). */
+void oracle_phase_inside(const uint8_t *bone, const uint8_t *roi, size_t n, int select_bone,
+                         uint8_t *out) {
+  for (size_t i = 0; i < n; ++i)
+    out[i] = (uint8_t)(roi[i] != 0 && ((bone[i] != 0) == (select_bone != 0)));
+}
+
+/* intersection, src/edt.rs:219-223 */
+static inline double intersection(const double *f, size_t p, size_t q, double qf) {
+  double pf = (double)p;
+  return ((f[q] + qf * qf) - (f[p] + pf * pf)) / (2.0 * (qf - pf));
+}
+
+/* transform_1d (Felzenszwalb-Huttenlocher), src/edt.rs:167-217 */
+static void transform_1d(const double *f, double *d, size_t *v, double *z, size_t n) {
+  size_t k = 0;
+  v[0] = 0;
+  z[0] = -INFINITY;
+  z[1] = INFINITY;
+  for (size_t q = 1; q < n; ++q) {
+    double qf = (double)q;
+    size_t p = v[k];
+    double s = intersection(f, p, q, qf);
+    while (s <= z[k]) {
+      if (k == 0) break;
+      k -= 1;
+      p = v[k];
+      s = intersection(f, p, q, qf);
+    }
+    if (k == 0 && s <= z[0]) {
+      v[0] = q;
+      z[0] = -INFINITY;
+      z[1] = INFINITY;
+    } else {
+      k += 1;
+      v[k] = q;
+      z[k] = s;
+      z[k + 1] = INFINITY;
+    }
+  }
+  size_t env = 0;
+  for (size_t q = 0; q < n; ++q) {
+    double qf = (double)q;
+    while (z[env + 1] < qf) env += 1;
+    size_t p = v[env];
+    double delta = qf - (double)p;
+    d[q] = delta * delta + f[p];
+  }
+}
+
+/* euclidean_distance_transform_squared, src/edt.rs:57-156 */
+int oracle_edt_squared(const uint8_t *vol, size_t w, size_t h, size_t dd, int phase,
+                       uint32_t *out) {
+  if (check_dims(w, h, dd)) return 1;
+  size_t n = w * h * dd, slice = w * h;
+  int all = 1;
+  for (size_t i = 0; i < n && all; ++i) all = is_selected(phase, vol[i]);
+  if (all) { /* src/edt.rs:61-66 */
+    snprintf(g_err, sizeof g_err,
+             "distance transform is undefined for an all-%s volume unless an exterior "
+             "convention is specified",
+             phase == 0 ? "foreground" : "background");
+    return 1;
+  }
+  double *data = (double *)malloc(n * sizeof(double));
+  double *zl = (double *)malloc(n * sizeof(double));
+  if (!data || !zl) { free(data); free(zl); return fail("oracle: out of host memory"); }
+
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) data[i] = is_selected(phase, vol[i]) ? ORACLE_INF : 0.0;
+
+  size_t mx = w > h ? w : h;
+  if (dd > mx) mx = dd;
+#pragma omp parallel
+  {
+    double *in = (double *)malloc(mx * sizeof(double));
+    double *ob = (double *)malloc(mx * sizeof(double));
+    size_t *v = (size_t *)malloc(mx * sizeof(size_t));
+    double *z = (double *)malloc((mx + 1) * sizeof(double));
+    /* X pass, src/edt.rs:79-90 */
+#pragma omp for schedule(static)
+    for (size_t r = 0; r < h * dd; ++r) {
+      double *row = data + r * w;
+      memcpy(in, row, w * sizeof(double));
+      transform_1d(in, row, v, z, w);
+    }
+    /* Y pass, src/edt.rs:92-113 */
+#pragma omp for schedule(static)
+    for (size_t zi = 0; zi < dd; ++zi) {
+      double *sl = data + zi * slice;
+      for (size_t x = 0; x < w; ++x) {
+        for (size_t y = 0; y < h; ++y) in[y] = sl[y * w + x];
+        transform_1d(in, ob, v, z, h);
+        for (size_t y = 0; y < h; ++y) sl[y * w + x] = ob[y];
+      }
+    }
+    /* Z pass into z-line-major buffer, src/edt.rs:115-133 */
+#pragma omp for schedule(static)
+    for (size_t line = 0; line < slice; ++line) {
+      for (size_t zi = 0; zi < dd; ++zi) in[zi] = data[zi * slice + line];
+      transform_1d(in, zl + line * dd, v, z, dd);
+    }
+    free(in); free(ob); free(v); free(z);
+  }
+  /* back-transpose, round, phase mask, src/edt.rs:137-153 */
+#pragma omp parallel for schedule(static)
+  for (size_t zi = 0; zi < dd; ++zi) {
+    for (size_t line = 0; line < slice; ++line) {
+      size_t idx = zi * slice + line;
+      out[idx] = is_selected(phase, vol[idx]) ? (uint32_t)round(zl[line * dd + zi]) : 0u;
+    }
+  }
+  free(data);
+  free(zl);
+  return 0;
+}
+
+/* integer_sqrt, src/local_thickness.rs:338-348 */
+static inline uint32_t integer_sqrt(uint32_t value) {
+  uint32_t root = (uint32_t)sqrt((double)value);
+  while ((uint64_t)(root + 1) * (uint64_t)(root + 1) <= (uint64_t)value) root += 1;
+  while ((uint64_t)root * (uint64_t)root > (uint64_t)value) root -= 1;
+  return root;
+}
+
+/* ceiling_sqrt, src/local_thickness.rs:350-358 */
+static inline uint32_t ceiling_sqrt(uint32_t value) {
+  uint32_t fl = integer_sqrt(value);
+  return fl * fl == value ? fl : fl + 1;
+}
+
+/* scan_cube, src/local_thickness.rs:75-102 (one radius) */
+static uint32_t scan_cube_one(uint32_t dx, uint32_t dy, uint32_t dz, uint32_t r2) {
+  uint32_t radius = integer_sqrt(r2) + 1, maximum = 0;
+  for (uint32_t k = 0; k <= radius; ++k) {
+    uint32_t sk = k * k, dk = (k + dz) * (k + dz);
+    for (uint32_t j = 0; j <= radius; ++j) {
+      uint32_t skj = sk + j * j;
+      if (skj > r2) continue;
+      uint32_t ip = integer_sqrt(r2 - skj) + dx;
+      uint32_t cand = dk + (j + dy) * (j + dy) + ip * ip;
+      if (cand > maximum) maximum = cand;
+    }
+  }
+  return maximum;
+}
+
+/* create_templates, src/local_thickness.rs:67-73: out[c*count + i] for c = 0..2 */
+void oracle_templates(const uint32_t *values, size_t count, uint32_t *out) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (size_t i = 0; i < count; ++i) {
+    out[i] = scan_cube_one(1, 0, 0, values[i]);
+    out[count + i] = scan_cube_one(1, 1, 0, values[i]);
+    out[2 * count + i] = scan_cube_one(1, 1, 1, values[i]);
+  }
+}
+
+/* NEIGHBORS_26 in Fiji's visiting order, src/local_thickness.rs:283-312 */
+static const int NB26[26][3] = {
+    {0, 0, -1},  {0, 0, 1},   {0, -1, 0},  {0, 1, 0},  {-1, 0, 0},  {1, 0, 0},  {0, 1, -1},
+    {0, 1, 1},   {1, -1, 0},  {1, 1, 0},   {-1, 0, 1}, {1, 0, 1},   {0, -1, -1}, {0, -1, 1},
+    {-1, -1, 0}, {-1, 1, 0},  {-1, 0, -1}, {1, 0, -1}, {1, 1, 1},   {1, -1, 1},  {-1, 1, 1},
+    {-1, -1, 1}, {1, 1, -1},  {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+
+/* look, src/local_thickness.rs:314-331 */
+static inline float look(const float *v, long x, long y, long z, long w, long h, long d) {
+  if (x < 0 || y < 0 || z < 0 || x >= w || y >= h || z >= d) return -1.0f;
+  return v[((size_t)z * (size_t)h + (size_t)y) * (size_t)w + (size_t)x];
+}
+
+/* clean_up_surface_jaggies, src/local_thickness.rs:210-281.  reverse != 0 runs
+ * pass 2 in reverse z,y,x order (used only to test order independence). */
+static void clean_up(const float *input, float *output, size_t w, size_t h, size_t d,
+                     int reverse) {
+  size_t n = w * h * d;
+  memset(output, 0, n * sizeof(float));
+  for (size_t z = 0; z < d; ++z)
+    for (size_t y = 0; y < h; ++y)
+      for (size_t x = 0; x < w; ++x) {
+        size_t idx = (z * h + y) * w + x;
+        float value = input[idx];
+        if (value == 0.0f) continue;
+        int borders = 0;
+        for (int k = 0; k < 26 && !borders; ++k)
+          borders = look(input, (long)x + NB26[k][0], (long)y + NB26[k][1],
+                         (long)z + NB26[k][2], (long)w, (long)h, (long)d) == 0.0f;
+        output[idx] = borders ? -1.0f : value;
+      }
+  for (size_t t = 0; t < n; ++t) {
+    size_t idx = reverse ? n - 1 - t : t;
+    if (output[idx] != -1.0f) continue;
+    size_t z = idx / (w * h), rem = idx - z * w * h, y = rem / w, x = rem - y * w;
+    size_t count = 0;
+    float sum = 0.0f;
+    for (int k = 0; k < 26; ++k) {
+      float v = look(output, (long)x + NB26[k][0], (long)y + NB26[k][1], (long)z + NB26[k][2],
+                     (long)w, (long)h, (long)d);
+      if (v > 0.0f) {
+        count += 1;
+        sum += v;
+      }
+    }
+    float corrected = count > 0 ? sum / (float)count : input[idx];
+    output[idx] = -corrected;
+  }
+  for (size_t i = 0; i < n; ++i) output[i] = fabsf(output[i]);
+}
+
+static inline void atomic_max_u32(uint32_t *p, uint32_t v) {
+  uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (cur < v &&
+         !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+}
+
+/* Optional intermediate outputs of oracle_local_thickness_ex. */
+typedef struct {
+  uint8_t *ridge_flags;     /* N bytes: 1 where distance_ridges selected the voxel */
+  uint32_t *propagated;     /* N: max r^2 over covering ridge balls (before sqrt) */
+  uint32_t *squared_edt;    /* N: the squared EDT */
+  uint64_t ridge_count;     /* out */
+  uint64_t fill_visits;     /* out: sum over ridges of |ball(r^2) within the array| */
+  int propagate_from_all;   /* in: propagate from every r^2>0 voxel, not only ridges */
+  int reverse_cleanup;      /* in: pass 2 traversal reversed */
+} oracle_lt_debug;
+
+/* local_thickness, src/local_thickness.rs:16-44, with the helpers it calls. */
+int oracle_local_thickness_ex(const uint8_t *vol, size_t w, size_t h, size_t d, int phase,
+                              float *out, oracle_lt_debug *dbg) {
+  if (check_dims(w, h, d)) return 1;
+  size_t n = w * h * d, slice = w * h;
+  uint32_t *sq = (dbg && dbg->squared_edt) ? dbg->squared_edt
+                                           : (uint32_t *)malloc(n * sizeof(uint32_t));
+  if (!sq) return fail("oracle: out of host memory");
+  if (oracle_edt_squared(vol, w, h, d, phase, sq)) {
+    if (!(dbg && dbg->squared_edt)) free(sq);
+    return 1;
+  }
+  /* distance_index, src/local_thickness.rs:46-65 (serial) */
+  uint32_t maximum = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (sq[i] > maximum) maximum = sq[i];
+  uint8_t *occurs = (uint8_t *)calloc((size_t)maximum + 1, 1);
+  for (size_t i = 0; i < n; ++i) occurs[sq[i]] = 1;
+  size_t nvals = 0;
+  for (size_t v = 0; v <= maximum; ++v) nvals += occurs[v];
+  uint32_t *values = (uint32_t *)malloc(nvals * sizeof(uint32_t));
+  size_t *rank = (size_t *)malloc(((size_t)maximum + 1) * sizeof(size_t));
+  for (size_t v = 0, k = 0; v <= maximum; ++v) {
+    rank[v] = (size_t)-1;
+    if (occurs[v]) {
+      values[k] = (uint32_t)v;
+      rank[v] = k++;
+    }
+  }
+  /* create_templates, src/local_thickness.rs:67-73 */
+  uint32_t *tmpl = (uint32_t *)malloc(3 * nvals * sizeof(uint32_t));
+  oracle_templates(values, nvals, tmpl);
+
+  /* distance_ridges, src/local_thickness.rs:104-167.  The ridge list is kept in
+   * ascending index order (rayon's indexed collect) via a flag array. */
+  uint8_t *ridge = (uint8_t *)calloc(n, 1);
+#pragma omp parallel for schedule(static)
+  for (size_t idx = 0; idx < n; ++idx) {
+    if (!is_selected(phase, vol[idx])) continue;
+    uint32_t r2 = sq[idx];
+    if (r2 == 0) continue;
+    if (dbg && dbg->propagate_from_all) {
+      ridge[idx] = 1;
+      continue;
+    }
+    size_t z = idx / slice, rem = idx - z * slice, y = rem / w, x = rem - y * w;
+    size_t ri = rank[r2];
+    int is_ridge = 1;
+    for (long dz = -1; dz <= 1 && is_ridge; ++dz) {
+      long nz = (long)z + dz;
+      if (nz < 0 || nz >= (long)d) continue;
+      for (long dy = -1; dy <= 1 && is_ridge; ++dy) {
+        long ny = (long)y + dy;
+        if (ny < 0 || ny >= (long)h) continue;
+        for (long dx = -1; dx <= 1; ++dx) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          long nx = (long)x + dx;
+          if (nx < 0 || nx >= (long)w) continue;
+          int comps = (dx != 0) + (dy != 0) + (dz != 0);
+          size_t nb = ((size_t)nz * h + (size_t)ny) * w + (size_t)nx;
+          if (sq[nb] >= tmpl[(size_t)(comps - 1) * nvals + ri]) {
+            is_ridge = 0;
+            break;
+          }
+        }
+      }
+    }
+    ridge[idx] = (uint8_t)is_ridge;
+  }
+
+  /* propagate_ridges, src/local_thickness.rs:169-208 */
+  uint32_t *prop = (dbg && dbg->propagated) ? dbg->propagated
+                                            : (uint32_t *)malloc(n * sizeof(uint32_t));
+  memset(prop, 0, n * sizeof(uint32_t));
+  uint64_t ridge_count = 0, visits = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : ridge_count, visits)
+  for (size_t idx = 0; idx < n; ++idx) {
+    if (!ridge[idx]) continue;
+    ridge_count += 1;
+    uint32_t r2 = sq[idx];
+    size_t z = idx / slice, rem = idx - z * slice, y = rem / w, x = rem - y * w;
+    size_t radius = ceiling_sqrt(r2);
+    size_t x0 = x > radius ? x - radius : 0, y0 = y > radius ? y - radius : 0,
+           z0 = z > radius ? z - radius : 0;
+    size_t x1 = x + radius < w - 1 ? x + radius : w - 1;
+    size_t y1 = y + radius < h - 1 ? y + radius : h - 1;
+    size_t z1 = z + radius < d - 1 ? z + radius : d - 1;
+    uint64_t r2u = r2;
+    for (size_t nz = z0; nz <= z1; ++nz) {
+      uint64_t dzz = nz > z ? nz - z : z - nz;
+      uint64_t dz2 = dzz * dzz;
+      for (size_t ny = y0; ny <= y1; ++ny) {
+        uint64_t dyy = ny > y ? ny - y : y - ny;
+        uint64_t dyz = dz2 + dyy * dyy;
+        if (dyz > r2u) continue;
+        for (size_t nx = x0; nx <= x1; ++nx) {
+          uint64_t dxx = nx > x ? nx - x : x - nx;
+          if (dyz + dxx * dxx <= r2u) {
+            atomic_max_u32(&prop[(nz * h + ny) * w + nx], r2);
+            visits += 1;
+          }
+        }
+      }
+    }
+  }
+  float *diam = (float *)malloc(n * sizeof(float));
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) diam[i] = (float)(2.0 * sqrt((double)prop[i]));
+
+  float *cleaned = (float *)malloc(n * sizeof(float));
+  clean_up(diam, cleaned, w, h, d, dbg ? dbg->reverse_cleanup : 0);
+
+  /* phase mask, src/local_thickness.rs:31-43 */
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i) out[i] = is_selected(phase, vol[i]) ? cleaned[i] : 0.0f;
+
+  if (dbg) {
+    dbg->ridge_count = ridge_count;
+    dbg->fill_visits = visits;
+    if (dbg->ridge_flags) memcpy(dbg->ridge_flags, ridge, n);
+  }
+  free(cleaned);
+  free(diam);
+  if (!(dbg && dbg->propagated)) free(prop);
+  free(ridge);
+  free(tmpl);
+  free(rank);
+  free(values);
+  free(occurs);
+  if (!(dbg && dbg->squared_edt)) free(sq);
+  return 0;
+}
+
+int oracle_local_thickness(const uint8_t *vol, size_t w, size_t h, size_t d, int phase,
+                           float *out) {
+  return oracle_local_thickness_ex(vol, w, h, d, phase, out, NULL);
+}
+
+/* DistributionStats, src/metrics.rs:26-38 */
+typedef struct {
+  double mean, standard_deviation, maximum, median, percentile_10, percentile_25,
+      percentile_75, percentile_90, interquartile_range, coefficient_of_variation;
+  uint64_t count;
+} oracle_stats;
+
+static int cmp_f32(const void *a, const void *b) {
+  float x = *(const float *)a, y = *(const float *)b;
+  return (x > y) - (x < y);
+}
+
+/* linear_quantile_unstable, src/metrics.rs:271-291, over values sorted
+ * ascending (selection by sort gives the same order statistics). */
+static double linear_quantile_sorted(const float *sorted, size_t len, double q, float scale) {
+  double position = q * (double)(len - 1);
+  size_t lower = (size_t)floor(position), upper = (size_t)ceil(position);
+  double fraction = position - (double)lower;
+  double lo = (double)sorted[lower];
+  double hi = upper == lower ? lo : (double)sorted[upper];
+  double qv = lo + fraction * (hi - lo);
+  return (double)((float)qv * scale);
+}
+
+double oracle_quantile(const float *values, size_t len, double q, float scale) {
+  float *s = (float *)malloc(len * sizeof(float));
+  memcpy(s, values, len * sizeof(float));
+  qsort(s, len, sizeof(float), cmp_f32);
+  double r = linear_quantile_sorted(s, len, q, scale);
+  free(s);
+  return r;
+}
+
+/* nonzero_stats, src/metrics.rs:204-269 */
+int oracle_nonzero_stats(const float *values, size_t n, double scale, int include_quantiles,
+                         oracle_stats *st) {
+  uint64_t count = 0;
+  double sum = 0.0, sumsq = 0.0, maximum = 0.0;
+  float scale_f32 = (float)scale;
+  for (size_t i = 0; i < n; ++i) {
+    float raw = values[i];
+    if (!isfinite(raw) || raw <= 0.0f) continue;
+    double value = (double)(raw * scale_f32);
+    count += 1;
+    sum += value;
+    sumsq += value * value;
+    maximum = maximum > value ? maximum : value;
+  }
+  if (count == 0) return fail("cannot compute statistics because the selected phase is empty");
+  double cf = (double)count;
+  double mean = sum / cf;
+  double centered = (cf * sumsq - sum * sum) / cf;
+  double sd = (count > 1 && centered > 0.0) ? sqrt(centered / (cf - 1.0)) : 0.0;
+  double med = NAN, p10 = NAN, p25 = NAN, p75 = NAN, p90 = NAN;
+  if (include_quantiles) {
+    float *kept = (float *)malloc(count * sizeof(float));
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (isfinite(values[i]) && values[i] > 0.0f) kept[k++] = values[i];
+    qsort(kept, k, sizeof(float), cmp_f32);
+    med = linear_quantile_sorted(kept, k, 0.5, scale_f32);
+    p10 = linear_quantile_sorted(kept, k, 0.1, scale_f32);
+    p25 = linear_quantile_sorted(kept, k, 0.25, scale_f32);
+    p75 = linear_quantile_sorted(kept, k, 0.75, scale_f32);
+    p90 = linear_quantile_sorted(kept, k, 0.9, scale_f32);
+    free(kept);
+  }
+  st->mean = mean;
+  st->standard_deviation = sd;
+  st->maximum = maximum;
+  st->median = med;
+  st->percentile_10 = p10;
+  st->percentile_25 = p25;
+  st->percentile_75 = p75;
+  st->percentile_90 = p90;
+  st->interquartile_range = p75 - p25;
+  st->coefficient_of_variation = sd / mean;
+  st->count = count;
+  return 0;
+}
+
+/* Thickness/spacing part of analyze, src/metrics.rs:64-132. */
+typedef struct {
+  oracle_stats thickness, spacing;
+  double bone_volume, total_volume, bone_volume_fraction;
+} oracle_metrics;
+
+int oracle_analyze_thickness_spacing(const uint8_t *bone, const uint8_t *roi, size_t w, size_t h,
+                                     size_t d, const double spacing[3], int bonej_mode,
+                                     int include_quantiles, oracle_metrics *m) {
+  if (check_dims(w, h, d)) return 1;
+  for (int i = 0; i < 3; ++i)
+    if (!isfinite(spacing[i]) || spacing[i] <= 0.0) {
+      snprintf(g_err, sizeof g_err, "voxel spacings must be finite and positive; got [%g, %g, %g]",
+               spacing[0], spacing[1], spacing[2]);
+      return 1;
+    }
+  size_t n = w * h * d;
+  size_t roi_voxels = 0;
+  for (size_t i = 0; i < n; ++i) roi_voxels += roi[i] != 0;
+  if (roi_voxels == 0) return fail("ROI mask contains no included voxels");
+  if (bonej_mode && roi_voxels == n)
+    return fail(
+        "BoneJ compatibility mode requires an ROI mask with both inside and outside voxels");
+  uint8_t *ab = (uint8_t *)malloc(n), *am = (uint8_t *)malloc(n);
+  oracle_phase_inside(bone, roi, n, 1, ab);
+  oracle_phase_inside(bone, roi, n, 0, am);
+  size_t bone_voxels = 0, marrow_voxels = 0;
+  for (size_t i = 0; i < n; ++i) {
+    bone_voxels += ab[i] != 0;
+    marrow_voxels += am[i] != 0;
+  }
+  int rc = 0;
+  float *map = NULL;
+  if (bone_voxels == 0) { rc = fail("ROI contains no bone voxels"); goto done; }
+  if (marrow_voxels == 0) { rc = fail("ROI contains no marrow voxels"); goto done; }
+  map = (float *)malloc(n * sizeof(float));
+  if ((rc = oracle_local_thickness(ab, w, h, d, 0, map))) goto done;
+  if ((rc = oracle_nonzero_stats(map, n, spacing[0], include_quantiles, &m->thickness))) goto done;
+  if ((rc = oracle_local_thickness(am, w, h, d, 0, map))) goto done;
+  if ((rc = oracle_nonzero_stats(map, n, spacing[0], include_quantiles, &m->spacing))) goto done;
+  {
+    double vv = spacing[0] * spacing[1] * spacing[2];
+    m->bone_volume = (double)bone_voxels * vv;
+    m->total_volume = (double)roi_voxels * vv;
+    m->bone_volume_fraction = m->bone_volume / m->total_volume;
+  }
+done:
+  free(map);
+  free(ab);
+  free(am);
+  return rc;
+}

@@ -1,0 +1,142 @@
+"""ctypes binding of oracle/liboracle.so (the CPU restatement; tests only)."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+Z, P, I, D = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+class LtDebug(ctypes.Structure):
+    _fields_ = [("ridge_flags", P), ("propagated", P), ("squared_edt", P),
+                ("ridge_count", ctypes.c_uint64), ("fill_visits", ctypes.c_uint64),
+                ("propagate_from_all", I), ("reverse_cleanup", I)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, D) for n in ("mean", "standard_deviation", "maximum", "median",
+                                 "percentile_10", "percentile_25", "percentile_75",
+                                 "percentile_90", "interquartile_range",
+                                 "coefficient_of_variation")] + [("count", ctypes.c_uint64)]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [("thickness", Stats), ("spacing", Stats), ("bone_volume", D),
+                ("total_volume", D), ("bone_volume_fraction", D)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(PATH):
+            raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(PATH)
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_phase_inside.argtypes = [P, P, Z, I, P]
+        L.oracle_phase_inside.restype = None
+        L.oracle_edt_squared.argtypes = [P, Z, Z, Z, I, P]
+        L.oracle_templates.argtypes = [P, Z, P]
+        L.oracle_templates.restype = None
+        L.oracle_local_thickness_ex.argtypes = [P, Z, Z, Z, I, P, ctypes.POINTER(LtDebug)]
+        L.oracle_local_thickness.argtypes = [P, Z, Z, Z, I, P]
+        L.oracle_nonzero_stats.argtypes = [P, Z, D, I, ctypes.POINTER(Stats)]
+        L.oracle_quantile.argtypes = [P, Z, D, ctypes.c_float]
+        L.oracle_quantile.restype = D
+        L.oracle_analyze_thickness_spacing.argtypes = [P, P, Z, Z, Z, ctypes.POINTER(D), I, I,
+                                                       ctypes.POINTER(Metrics)]
+        L.oracle_set_num_threads.argtypes = [I]
+        L.oracle_set_num_threads.restype = None
+        L.oracle_num_threads.restype = I
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc != 0:
+        raise OracleError(lib().oracle_last_error().decode())
+
+
+def phase_inside(bone, roi, select_bone):
+    bone = np.ascontiguousarray(bone, np.uint8)
+    roi = np.ascontiguousarray(roi, np.uint8)
+    out = np.empty(bone.size, np.uint8)
+    lib().oracle_phase_inside(bone.ctypes.data, roi.ctypes.data, bone.size, int(select_bone),
+                              out.ctypes.data)
+    return out
+
+
+def edt_squared(vol, w, h, d, phase=0):
+    vol = np.ascontiguousarray(vol, np.uint8).reshape(-1)
+    out = np.empty(vol.size, np.uint32)
+    _chk(lib().oracle_edt_squared(vol.ctypes.data, w, h, d, int(phase), out.ctypes.data))
+    return out
+
+
+def templates(values):
+    v = np.ascontiguousarray(values, np.uint32)
+    out = np.empty(3 * v.size, np.uint32)
+    lib().oracle_templates(v.ctypes.data, v.size, out.ctypes.data)
+    return out.reshape(3, v.size)
+
+
+def local_thickness_stages(vol, w, h, d, phase=0, propagate_from_all=False,
+                           reverse_cleanup=False):
+    vol = np.ascontiguousarray(vol, np.uint8).reshape(-1)
+    n = vol.size
+    out = np.empty(n, np.float32)
+    flags = np.empty(n, np.uint8)
+    prop = np.empty(n, np.uint32)
+    sq = np.empty(n, np.uint32)
+    dbg = LtDebug(flags.ctypes.data, prop.ctypes.data, sq.ctypes.data, 0, 0,
+                  int(propagate_from_all), int(reverse_cleanup))
+    _chk(lib().oracle_local_thickness_ex(vol.ctypes.data, w, h, d, int(phase), out.ctypes.data,
+                                         ctypes.byref(dbg)))
+    return dict(thickness=out, ridge_flags=flags, propagated=prop, squared_edt=sq,
+                ridge_count=dbg.ridge_count, fill_visits=dbg.fill_visits)
+
+
+def local_thickness(vol, w, h, d, phase=0):
+    return local_thickness_stages(vol, w, h, d, phase)["thickness"]
+
+
+def nonzero_stats(values, scale, include_quantiles=True):
+    v = np.ascontiguousarray(values, np.float32)
+    st = Stats()
+    _chk(lib().oracle_nonzero_stats(v.ctypes.data, v.size, float(scale), int(include_quantiles),
+                                    ctypes.byref(st)))
+    return st
+
+
+def quantile(values, q, scale):
+    v = np.ascontiguousarray(values, np.float32)
+    return lib().oracle_quantile(v.ctypes.data, v.size, float(q), float(scale))
+
+
+def analyze_thickness_spacing(bone, roi, w, h, d, spacing, bonej_mode=False,
+                              include_quantiles=True):
+    bone = np.ascontiguousarray(bone, np.uint8).reshape(-1)
+    roi = np.ascontiguousarray(roi, np.uint8).reshape(-1)
+    sp = (D * 3)(*spacing)
+    m = Metrics()
+    _chk(lib().oracle_analyze_thickness_spacing(bone.ctypes.data, roi.ctypes.data, w, h, d, sp,
+                                                int(bonej_mode), int(include_quantiles),
+                                                ctypes.byref(m)))
+    return m
+
+
+def set_threads(n):
+    lib().oracle_set_num_threads(int(n))
+
+
+def num_threads():
+    return lib().oracle_num_threads()

@@ -1,0 +1,74 @@
+"""The C ABI library loads, exports every symbol include/bm.h declares, and
+rejects invalid input with the reference's messages before touching a GPU."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import bone_morphometry_amd as bm
+from bone_morphometry_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "bm.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(bm_\w+)\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_empty_error():
+    assert _native.lib().bm_version() == 100
+    assert _native.lib().bm_last_error() == b""
+
+
+# BinaryVolume::new, src/volume.rs:12-41
+def test_binary_volume_new_rejects_bad_input():
+    with pytest.raises(bm.BmError, match=r"volume has 7 voxels, but dimensions 2x2x2 require 8"):
+        bm.BinaryVolume.new(np.zeros(7, np.uint8), 2, 2, 2)
+    with pytest.raises(bm.BmError, match=r"at least 2 for 3-D morphometry; got 1x4x4"):
+        bm.BinaryVolume.new(np.zeros(16, np.uint8), 1, 4, 4)
+    bm.BinaryVolume.new(np.zeros(8, np.uint8), 2, 2, 2)
+
+
+def test_calls_validate_before_device_work():
+    v = bm.BinaryVolume(np.zeros(4, np.uint8), 1, 2, 2)
+    with pytest.raises(bm.BmError) as e:
+        bm.euclidean_distance_transform_squared(v, bm.Phase.Foreground)
+    assert e.value.code == _native.BM_ERR_INVALID
+    with pytest.raises(bm.BmError, match="at least 2"):
+        bm.local_thickness(v, bm.Phase.Foreground)
+
+
+def test_analyze_validates_spacing_and_shape():
+    a = bm.BinaryVolume(np.zeros(8, np.uint8), 2, 2, 2)
+    b = bm.BinaryVolume(np.zeros(12, np.uint8), 2, 2, 3)
+    with pytest.raises(bm.BmError, match=r"voxel spacings must be finite and positive; got "
+                                          r"\[1\.0, NaN, 2\.5\]"):
+        bm.analyze_thickness_spacing(a, a, (1.0, math.nan, 2.5))
+    with pytest.raises(bm.BmError, match="binary image dimensions 2x2x2 do not match ROI "
+                                          "dimensions 2x2x3"):
+        bm.analyze_thickness_spacing(a, b, (1.0, 1.0, 1.0))
+
+
+def test_unsupported_sizes_are_reported():
+    # checked before any allocation: 20000 > the per-axis limit
+    big = _native.BmVolume(None, 0, 20000, 2, 2)
+    rc = _native.lib().bm_euclidean_distance_transform_squared(ctypes.byref(big), 0, None, 0)
+    assert rc == _native.BM_ERR_INVALID  # len 0 != 80000 fails BinaryVolume::new first
+    rc = _native.lib().bm_euclidean_distance_transform_squared_device(None, 20000, 2, 2, 0, None,
+                                                                      None)
+    assert rc == _native.BM_ERR_UNSUPPORTED
+    assert b"exceeds the device path's limits" in _native.lib().bm_last_error()

@@ -1,0 +1,206 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle, the
+golden fixtures and the reference's own known answers.  Integer products (the
+squared EDT, ridge flags, propagated r^2) and the f32 thickness maps must be
+bit-identical; the f64 summaries within 1e-12 relative (reduction order)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as orc
+import bone_morphometry_amd as bm
+from bone_morphometry_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "edt_scipy.npz")
+REL = 1e-12
+
+
+def vol(a, w, h, d):
+    return bm.BinaryVolume(np.asarray(a, np.uint8).reshape(-1), w, h, d)
+
+
+def _structure(seed, shape, frac):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(shape)
+    for ax in range(3):
+        f = (f + np.roll(f, 1, ax) + np.roll(f, -1, ax)) / 3
+    return (f > np.quantile(f, 1.0 - frac)).astype(np.uint8)
+
+
+# --- the reference's unit tests through the C ABI -----------------------------
+def test_phase_inside_known_answer():
+    bone = vol([0, 1, 0, 1, 0, 1, 0, 1], 2, 2, 2)
+    roi = vol([0, 0, 1, 1, 1, 1, 0, 0], 2, 2, 2)
+    assert bone.phase_inside(roi, True).data.tolist() == [0, 0, 0, 1, 0, 1, 0, 0]
+    assert bone.phase_inside(roi, False).data.tolist() == [0, 0, 1, 0, 1, 0, 0, 0]
+
+
+def test_edt_known_answers():
+    a = np.ones(27, np.uint8)
+    a[13] = 0
+    e = bm.euclidean_distance_transform_squared(vol(a, 3, 3, 3), bm.Phase.Foreground)
+    assert (e[13], e[12], e[0]) == (0, 1, 3)
+    b = np.zeros(27, np.uint8)
+    b[13] = 1
+    e = bm.euclidean_distance_transform_squared(vol(b, 3, 3, 3), bm.Phase.Background)
+    assert (e[13], e[12], e[0]) == (0, 1, 3)
+
+
+def test_edt_all_selected_error():
+    with pytest.raises(bm.BmError, match="distance transform is undefined for an "
+                                          "all-foreground volume unless an exterior "
+                                          "convention is specified"):
+        bm.euclidean_distance_transform_squared(vol(np.ones(8), 2, 2, 2), bm.Phase.Foreground)
+    with pytest.raises(bm.BmError, match="all-background"):
+        bm.local_thickness(vol(np.zeros(8), 2, 2, 2), bm.Phase.Background)
+
+
+def test_isolated_voxel_has_diameter_two():
+    a = np.zeros(27, np.uint8)
+    a[13] = 1
+    t = bm.local_thickness(vol(a, 3, 3, 3), bm.Phase.Foreground)
+    assert abs(t[13] - 2.0) < 1e-6 and (t > 0).sum() == 1
+
+
+def test_background_matches_complemented_foreground():
+    a = np.zeros((5, 5, 5), np.uint8)
+    a[1:4, 1:4, 1:4] = 1
+    d = bm.local_thickness(vol(a, 5, 5, 5), bm.Phase.Background)
+    c = bm.local_thickness(vol(a == 0, 5, 5, 5), bm.Phase.Foreground)
+    np.testing.assert_array_equal(d, c)
+
+
+# --- EDT ------------------------------------------------------------------------
+def test_edt_matches_golden_fixtures():
+    g = np.load(GOLDEN)
+    k = 0
+    while f"vol_{k}" in g:
+        w, h, d = (int(v) for v in g[f"shape_{k}"])
+        got = bm.euclidean_distance_transform_squared(vol(g[f"vol_{k}"], w, h, d),
+                                                      bm.Phase.Foreground)
+        np.testing.assert_array_equal(got, g[f"d2_{k}"], err_msg=f"case {k}: {w}x{h}x{d}")
+        k += 1
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 5, 7), (64, 64, 64), (65, 63, 67),
+                                   (600, 5, 4), (5, 700, 3), (4, 3, 1300), (97, 33, 530)])
+@pytest.mark.parametrize("phase", [bm.Phase.Foreground, bm.Phase.Background])
+def test_edt_matches_oracle(shape, phase):
+    w, h, d = shape
+    a = _structure(sum(shape) + int(phase), (d, h, w), 0.6)
+    if a.all() or not a.any():
+        a.flat[0] ^= 1
+    got = bm.euclidean_distance_transform_squared(vol(a, w, h, d), phase)
+    np.testing.assert_array_equal(got, orc.edt_squared(a, w, h, d, int(phase)))
+
+
+def test_edt_sparse_sites_long_searches():
+    w, h, d = 70, 80, 90
+    a = np.ones((d, h, w), np.uint8)
+    a[3, 77, 5] = 0
+    a[60, 1, 66] = 0
+    got = bm.euclidean_distance_transform_squared(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got, orc.edt_squared(a, w, h, d, 0))
+
+
+# --- local thickness --------------------------------------------------------------
+@pytest.mark.parametrize("seed,shape,frac", [(0, (20, 21, 22), 0.3), (1, (33, 17, 25), 0.7),
+                                             (2, (40, 40, 40), 0.85), (3, (7, 60, 9), 0.5)])
+def test_local_thickness_stages_match_oracle(seed, shape, frac):
+    w, h, d = shape
+    a = _structure(seed, (d, h, w), frac)
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    np.testing.assert_array_equal(got["squared_edt"], ref["squared_edt"])
+    np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
+    assert got["ridge_count"] == ref["ridge_count"]
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
+def test_local_thickness_plain_small_matches_oracle():
+    wl = workloads.plain_small()
+    for phase in (bm.Phase.Foreground, bm.Phase.Background):
+        got = bm.local_thickness(vol(wl.bone, wl.width, wl.height, wl.depth), phase)
+        ref = orc.local_thickness(wl.bone, wl.width, wl.height, wl.depth, int(phase))
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_local_thickness_is_deterministic():
+    wl = workloads.plain_small()
+    v = vol(wl.bone, wl.width, wl.height, wl.depth)
+    a = bm.local_thickness(v, bm.Phase.Background)
+    b = bm.local_thickness(v, bm.Phase.Background)
+    np.testing.assert_array_equal(a, b)
+
+
+# --- analyze (thickness / spacing part) ------------------------------------------
+def _close(a, b):
+    if isinstance(b, float) and math.isnan(b):
+        return math.isnan(a)
+    return abs(a - b) <= REL * abs(b)
+
+
+def sd_tolerance(n, cv):
+    """Relative SD difference the reference's sequential f64 sum of squares can
+    cause: |error of sum x^2| <= (n-1) 2^-53 sum x^2, amplified by the
+    cancellation in n*sum x^2 - (sum x)^2 by (1 + 1/cv^2), halved by the sqrt.
+    The device sums are exact (fixed point), so this is the whole gap."""
+    return max(REL, (n - 1) * 2.0 ** -53 * (1.0 + 1.0 / cv ** 2) / 2.0)
+
+
+def _check_stats(g, r):
+    assert g.count == r.count
+    assert g.maximum == r.maximum
+    # the sum of values stays exactly representable at these sizes: same mean
+    assert _close(g.mean, r.mean), ("mean", g.mean, r.mean)
+    tol = sd_tolerance(r.count, r.coefficient_of_variation)
+    for f in ("standard_deviation", "coefficient_of_variation"):
+        gv, rv = getattr(g, f), getattr(r, f)
+        assert abs(gv - rv) <= tol * abs(rv), (f, gv, rv, tol)
+    for f in ("median", "percentile_10", "percentile_25", "percentile_75", "percentile_90",
+              "interquartile_range"):
+        gv, rv = getattr(g, f), getattr(r, f)
+        assert (math.isnan(gv) and math.isnan(rv)) or gv == rv, (f, gv, rv)
+
+
+@pytest.mark.parametrize("mode", [bm.FeatureMode.Standard, bm.FeatureMode.Classic])
+def test_analyze_plain_small_matches_oracle(mode):
+    wl = workloads.plain_small()
+    b = vol(wl.bone, wl.width, wl.height, wl.depth)
+    r = vol(wl.roi, wl.width, wl.height, wl.depth)
+    got = bm.analyze_thickness_spacing(b, r, wl.spacing, mode)
+    q = mode in (bm.FeatureMode.Standard, bm.FeatureMode.Experimental)
+    ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, wl.depth,
+                                        wl.spacing, include_quantiles=q)
+    _check_stats(got.thickness, ref.thickness)
+    _check_stats(got.spacing, ref.spacing)
+    assert got.bone_volume == ref.bone_volume and got.total_volume == ref.total_volume
+    assert got.bone_volume_fraction == ref.bone_volume_fraction
+
+
+def test_analyze_cylinder_roi_matches_oracle():
+    w, h, d = 96, 80, 72
+    from bone_morphometry_amd import synth
+    bone = synth.volume(w, h, d, 2.5, 0.2, 11)
+    roi = synth.roi_cylinder(w, h, d, 70.0)
+    got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d), (7.25, 7.25, 7.25))
+    ref = orc.analyze_thickness_spacing(bone, roi, w, h, d, (7.25, 7.25, 7.25))
+    _check_stats(got.thickness, ref.thickness)
+    _check_stats(got.spacing, ref.spacing)
+
+
+@pytest.mark.parametrize("bone,roi,mode,msg", [
+    (np.zeros(8), np.zeros(8), bm.FeatureMode.Standard, "ROI mask contains no included voxels"),
+    (np.zeros(8), np.ones(8), bm.FeatureMode.Standard, "ROI contains no bone voxels"),
+    (np.ones(8), np.ones(8), bm.FeatureMode.Standard, "ROI contains no marrow voxels"),
+    (np.eye(2).reshape(-1).repeat(2), np.ones(8), bm.FeatureMode.Bonej,
+     "BoneJ compatibility mode requires an ROI mask with both inside and outside voxels"),
+])
+def test_analyze_error_precedence(bone, roi, mode, msg):
+    with pytest.raises(bm.BmError, match=msg):
+        bm.analyze_thickness_spacing(vol(bone, 2, 2, 2), vol(roi, 2, 2, 2), (1, 1, 1), mode)
