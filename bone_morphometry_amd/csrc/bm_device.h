@@ -1,0 +1,143 @@
+// bm_device.h -- device helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bm_engine.h"
+
+namespace bm {
+
+typedef unsigned __int128 u128;
+
+__device__ __forceinline__ uint32_t sq16(uint32_t d) { return d == kInf16 ? kInf32 : d * d; }
+
+__device__ __forceinline__ bool selected_voxel(uint8_t v, uint8_t r, uint32_t want) {
+  return r != 0 && ((v != 0) == (want != 0));
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// integer_sqrt (src/local_thickness.rs:338-348): exact floor(sqrt(v)).
+__device__ __forceinline__ uint32_t isqrt_u32(uint32_t v) {
+  uint32_t r = (uint32_t)sqrtf((float)v);
+  while ((uint64_t)(r + 1) * (r + 1) <= v) ++r;
+  while ((uint64_t)r * r > v) --r;
+  return r;
+}
+__device__ __forceinline__ uint32_t ceil_sqrt_u32(uint32_t v) {
+  const uint32_t f = isqrt_u32(v);
+  return f * f == v ? f : f + 1;
+}
+
+// scan_cube (src/local_thickness.rs:75-102) for one r^2.
+__device__ inline uint32_t scan_cube(uint32_t dx, uint32_t dy, uint32_t dz, uint32_t r2) {
+  const uint32_t radius = isqrt_u32(r2) + 1;
+  uint32_t maximum = 0;
+  for (uint32_t k = 0; k <= radius; ++k) {
+    const uint32_t sk = k * k, dk = (k + dz) * (k + dz);
+    for (uint32_t j = 0; j <= radius; ++j) {
+      const uint32_t skj = sk + j * j;
+      if (skj > r2) break;  // j*j grows with j: the rest of the row is outside too
+      const uint32_t ip = isqrt_u32(r2 - skj) + dx;
+      maximum = max(maximum, dk + (j + dy) * (j + dy) + ip * ip);
+    }
+  }
+  return maximum;
+}
+
+// Diameter (2 * sqrt_f64(m)) as f32 (src/local_thickness.rs:206).  For
+// m < 2^24, (float)m is exact and the correctly rounded f32 sqrt scaled by 2
+// equals RN32(2 sqrt(m)); the reference's RN32(RN64(2 sqrt(m))) is the same
+// value because 2 sqrt(m) never lies within 2^-53 (relative) of an f32
+// rounding midpoint.  Larger m take the f64 route.
+__device__ __forceinline__ float diameter_of(uint32_t m) {
+  if (m < (1u << 24)) return 2.0f * sqrtf((float)m);
+  return (float)(2.0 * sqrt((double)m));
+}
+
+// ---------------------------------------------------------------------------
+// NEIGHBORS_26 in Fiji's visiting order (src/local_thickness.rs:283-312); the
+// f32 sum of the clean-up runs in this order.
+static __constant__ int8_t c_nb26[26][3] = {
+    {0, 0, -1},  {0, 0, 1},   {0, -1, 0},  {0, 1, 0},   {-1, 0, 0},  {1, 0, 0},  {0, 1, -1},
+    {0, 1, 1},   {1, -1, 0},  {1, 1, 0},   {-1, 0, 1},  {1, 0, 1},   {0, -1, -1}, {0, -1, 1},
+    {-1, -1, 0}, {-1, 1, 0},  {-1, 0, -1}, {1, 0, -1},  {1, 1, 1},   {1, -1, 1},  {-1, 1, 1},
+    {-1, -1, 1}, {1, 1, -1},  {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
+
+// Fixed-point accumulation of one value (positive finite f32).
+__device__ __forceinline__ void acc_value(float v, int lsb_exp, unsigned long long& cnt, u128& s,
+                                          u128& s2, uint32_t& mxb, uint32_t& flags) {
+  const uint32_t bits = __float_as_uint(v);
+  const int e = (int)((bits >> 23) & 0xFF);
+  const uint32_t mant = (bits & 0x7FFFFFu) | (e ? 0x800000u : 0u);
+  // Values lie in [2s, 2^17 s] for scale s, so shift is in [4, 21]; with
+  // shift <= 24 each k^2 < 2^96 and 2^32 of them cannot overflow 128 bits.
+  const int shift = (e ? e : 1) - 150 - lsb_exp;
+  if (shift < 0 || shift > 24) {
+    flags |= 1u;  // outside the exact fixed-point range
+    return;
+  }
+  const u128 k = (u128)mant << shift;
+  cnt += 1;
+  s += k;
+  s2 += (u128)((unsigned long long)mant * mant) << (2 * shift);
+  mxb = max(mxb, bits);
+}
+
+__device__ __forceinline__ u128 shfl_xor_u128(u128 v, int off) {
+  unsigned long long lo = (unsigned long long)v, hi = (unsigned long long)(v >> 64);
+  lo = __shfl_xor(lo, off);
+  hi = __shfl_xor(hi, off);
+  return ((u128)hi << 64) | lo;
+}
+
+__device__ inline void block_reduce_store(unsigned long long cnt, u128 s, u128 s2, uint32_t mxb,
+                                   uint32_t flags, StatsPartial* __restrict__ dst) {
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt += __shfl_xor(cnt, off);
+    s += shfl_xor_u128(s, off);
+    s2 += shfl_xor_u128(s2, off);
+    mxb = max(mxb, (uint32_t)__shfl_xor((int)mxb, off));
+    flags |= (uint32_t)__shfl_xor((int)flags, off);
+  }
+  __shared__ unsigned long long r_cnt[16];
+  __shared__ u128 r_s[16], r_s2[16];
+  __shared__ uint32_t r_mx[16], r_fl[16];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    r_cnt[wv] = cnt;
+    r_s[wv] = s;
+    r_s2[wv] = s2;
+    r_mx[wv] = mxb;
+    r_fl[wv] = flags;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nwv = blockDim.x >> 6;
+    for (uint32_t i = 1; i < nwv; ++i) {
+      cnt += r_cnt[i];
+      s += r_s[i];
+      s2 += r_s2[i];
+      mxb = max(mxb, r_mx[i]);
+      flags |= r_fl[i];
+    }
+    dst->count = cnt;
+    dst->sum_lo = (unsigned long long)s;
+    dst->sum_hi = (unsigned long long)(s >> 64);
+    dst->sq_lo = (unsigned long long)s2;
+    dst->sq_hi = (unsigned long long)(s2 >> 64);
+    dst->max_bits = mxb;
+    dst->flags = flags;
+  }
+}
+
+}  // namespace bm

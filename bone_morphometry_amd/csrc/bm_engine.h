@@ -54,10 +54,18 @@ struct Dims {
 struct PhaseStatus {
   unsigned long long selected;     // voxels of the phase (X pass)
   unsigned long long ridge_count;  // distance ridges found
-  unsigned long long fill_visits;  // candidate tests / ball visits of the fill
+  unsigned long long fill_visits;  // voxel-candidate tests of the fill
   uint32_t has_site;               // any non-selected voxel (X pass)
   uint32_t max_d2;                 // maximum squared distance (Z pass)
-  uint32_t error;                  // device-detected inconsistency (bit flags)
+  uint32_t error;                  // bit 0: r^2 too large for the tile lists
+  uint32_t overflow_tiles;         // tiles whose fill candidates exceeded LDS
+  unsigned long long fill_diag16[16];  // fill work counters and clocks (diagnostics)
+};
+
+// Per-block partials of the X pass.
+struct BlockPartial {
+  unsigned long long selected;
+  uint32_t flag;
   uint32_t pad;
 };
 
@@ -81,28 +89,40 @@ struct StatsResult {
 // column when H <= y_rows, otherwise a chunk of y_rows - 2*y_halo rows plus a
 // halo on each side (candidates beyond the halo are read from HBM).  The Z
 // pass is the same with z_rows lines of 32 columns (u32).
+// fill_cap bounds the candidates the fill holds in LDS per round (at most
+// kFillCap); a tile with more candidates runs several rounds.
 struct Tuning {
   uint32_t y_rows = 512;
   uint32_t y_halo = 64;
   uint32_t z_rows = 512;
   uint32_t z_halo = 64;
+  uint32_t fill_cap = 0;  // 0: the kernel's capacity
 };
 
 // Device buffers for one phase of one volume.
+struct TileMeta;
 struct PhaseBuffers {
   uint16_t* dx = nullptr;       // X pass output (N u16)
   uint32_t* h = nullptr;        // Y pass output, then the propagated map (N u32)
   uint32_t* d2 = nullptr;       // squared EDT (N u32)
-  uint2* ridges = nullptr;      // (index, r^2) list, capacity N
+  uint32_t* lists = nullptr;    // per-tile sorted ridge lists (512 per tile)
+  TileMeta* meta = nullptr;     // per-tile list header
+  unsigned long long* masks = nullptr;  // per tile: 8 selected + 8 target words
+  uint32_t* overflow = nullptr; // [count, tile ids...]
+  uint32_t tiles[3] = {0, 0, 0};
   uint32_t* tmpl = nullptr;     // 3 x (bound+1) templates
   float* diam = nullptr;        // (bound+1) diameters 2*sqrt(m)
   PhaseStatus* status = nullptr;
+  BlockPartial* xpart = nullptr;     // kEdtXBlocks
+  uint32_t* zmax = nullptr;          // one per Z-pass block
   StatsPartial* partials = nullptr;  // kStatsBlocks
   StatsPartial* stats = nullptr;     // final record
   uint64_t bound = 0;
 };
 
 constexpr int kStatsBlocks = 1024;
+constexpr int kEdtXBlocks = 4096;
+constexpr uint32_t kMinZChunk = 64;  // smallest z extent of a Z-pass tile
 
 // A growable device arena, freed by the engine.
 class Arena {
@@ -139,7 +159,7 @@ class Engine {
   // phase mask, statistics.  map_out may be null (stats only).
   void run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out, float scale_f32,
                      int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out = nullptr,
-                     uint32_t* propagated_out = nullptr);
+                     uint32_t* propagated_out = nullptr, const Tuning& t = Tuning());
 
  private:
   int device_;

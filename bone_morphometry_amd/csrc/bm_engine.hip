@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -9,27 +10,44 @@
 #include <vector>
 
 #include "bm_engine.h"
+#include "bm_tiles.h"
 
 namespace bm {
 
 // Kernels (bm_kernels.hip).
 __global__ void k_edt_x(const uint8_t*, const uint8_t*, uint32_t, uint16_t*, uint32_t, uint32_t,
-                        PhaseStatus*);
+                        BlockPartial*);
+__global__ void k_reduce_status(const BlockPartial*, uint32_t, const uint32_t*, uint32_t,
+                                PhaseStatus*);
 __global__ void k_edt_y(const uint16_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t);
 __global__ void k_edt_z(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t,
-                        uint32_t, PhaseStatus*);
+                        uint32_t, uint32_t*);
 __global__ void k_luts(const PhaseStatus*, uint32_t, uint32_t*, float*);
-__global__ void k_ridges(const uint32_t*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
-                         PhaseStatus*, uint2*);
-__global__ void k_fill_scatter(const uint2*, const PhaseStatus*, uint32_t, uint32_t, uint32_t,
-                               uint32_t*);
-__global__ void k_cleanup_stats(const uint32_t*, const uint32_t*, const float*,
-                                const PhaseStatus*, uint32_t, uint32_t, uint32_t, float, int,
-                                float*, StatsPartial*);
+// Tile kernels (bm_tiles.hip).
+__global__ void k_ridges_tiled(const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                               uint32_t, const uint32_t*, uint32_t, PhaseStatus*, uint32_t*,
+                               TileMeta*, unsigned long long*);
+__global__ void k_fill_gather(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                              const uint32_t*, const TileMeta*, const unsigned long long*,
+                              PhaseStatus*, uint32_t*, uint32_t*, uint32_t, uint32_t,
+                              unsigned long long*);
+__global__ void k_reduce_diag(const unsigned long long*, uint64_t, PhaseStatus*);
+__global__ void k_count_ridges(const TileMeta*, uint32_t, PhaseStatus*);
+__global__ void k_fill_overflow(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                const uint32_t*, const TileMeta*, const PhaseStatus*, uint32_t*,
+                                const uint32_t*);
+__global__ void k_fill_scatter_lists(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                     const uint32_t*, const TileMeta*, const PhaseStatus*,
+                                     uint32_t*);
+__global__ void k_zero_if_scatter(uint32_t*, uint64_t, const PhaseStatus*);
+__global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                 const uint32_t*, const TileMeta*, uint8_t*);
+__global__ void k_cleanup_tiled(const uint32_t*, const unsigned long long*, const float*,
+                                const PhaseStatus*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                uint32_t, uint32_t, float, int, float*, StatsPartial*);
 __global__ void k_stats_final(const StatsPartial*, uint32_t, StatsPartial*);
 __global__ void k_fill_u32(uint32_t*, uint64_t, uint32_t);
 __global__ void k_fill_u8(uint8_t*, uint64_t, uint8_t);
-__global__ void k_ridge_flags(const uint2*, const PhaseStatus*, uint8_t*);
 __global__ void k_phase_inside(const uint8_t*, const uint8_t*, uint64_t, uint32_t, uint8_t*);
 
 void hip_check(hipError_t e, const char* what) {
@@ -78,10 +96,19 @@ PhaseBuffers Engine::buffers_for(const Dims& dims) {
   const size_t o_dx = take(n * sizeof(uint16_t));
   const size_t o_h = take(n * sizeof(uint32_t));
   const size_t o_d2 = take(n * sizeof(uint32_t));
-  const size_t o_r = take(n * sizeof(uint2));
+  const uint32_t TX = tiles_of(dims.w), TY = tiles_of(dims.h), TZ = tiles_of(dims.d);
+  const uint64_t ntiles = (uint64_t)TX * TY * TZ;
+  const size_t o_l = take(ntiles * 512 * sizeof(uint32_t));
+  const size_t o_meta = take(ntiles * sizeof(TileMeta));
+  const size_t o_mask = take(ntiles * 16 * sizeof(unsigned long long));
+  const size_t o_of = take((ntiles + 1) * sizeof(uint32_t));
   const size_t o_t = take(3 * (bound + 1) * sizeof(uint32_t));
   const size_t o_dm = take((bound + 1) * sizeof(float));
   const size_t o_st = take(sizeof(PhaseStatus));
+  const size_t o_xp = take(kEdtXBlocks * sizeof(BlockPartial));
+  // Z-pass blocks for any z chunk of at least kMinZChunk lines
+  const uint64_t zblocks = (uint64_t)((dims.w + 31) / 32) * dims.h * ((dims.d + kMinZChunk - 1) / kMinZChunk);
+  const size_t o_zm = take(zblocks * sizeof(uint32_t));
   const size_t o_p = take(kStatsBlocks * sizeof(StatsPartial));
   const size_t o_s = take(sizeof(StatsPartial));
   work_.reserve(off);
@@ -90,10 +117,18 @@ PhaseBuffers Engine::buffers_for(const Dims& dims) {
   pb.dx = reinterpret_cast<uint16_t*>(b + o_dx);
   pb.h = reinterpret_cast<uint32_t*>(b + o_h);
   pb.d2 = reinterpret_cast<uint32_t*>(b + o_d2);
-  pb.ridges = reinterpret_cast<uint2*>(b + o_r);
+  pb.lists = reinterpret_cast<uint32_t*>(b + o_l);
+  pb.meta = reinterpret_cast<TileMeta*>(b + o_meta);
+  pb.masks = reinterpret_cast<unsigned long long*>(b + o_mask);
+  pb.overflow = reinterpret_cast<uint32_t*>(b + o_of);
+  pb.tiles[0] = TX;
+  pb.tiles[1] = TY;
+  pb.tiles[2] = TZ;
   pb.tmpl = reinterpret_cast<uint32_t*>(b + o_t);
   pb.diam = reinterpret_cast<float*>(b + o_dm);
   pb.status = reinterpret_cast<PhaseStatus*>(b + o_st);
+  pb.xpart = reinterpret_cast<BlockPartial*>(b + o_xp);
+  pb.zmax = reinterpret_cast<uint32_t*>(b + o_zm);
   pb.partials = reinterpret_cast<StatsPartial*>(b + o_p);
   pb.stats = reinterpret_cast<StatsPartial*>(b + o_s);
   pb.bound = bound;
@@ -119,8 +154,9 @@ void Engine::run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffer
   k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(b.status),
                               sizeof(PhaseStatus) / sizeof(uint32_t), 0u);
   const uint32_t rows = H * D;
-  BM_LAUNCH("edt_x", k_edt_x, grid_for((uint64_t)rows * 64, 256, 8192), 256, 0, s, src.data,
-            src.roi, src.want_nonzero, b.dx, W, rows, b.status);
+  const unsigned xblocks = grid_for((uint64_t)rows * 64, 256, kEdtXBlocks);
+  BM_LAUNCH("edt_x", k_edt_x, xblocks, 256, 0, s, src.data, src.roi, src.want_nonzero, b.dx, W,
+            rows, b.xpart);
   {
     const uint32_t chunk = H <= t.y_rows ? H : t.y_rows - 2 * t.y_halo;
     const uint32_t halo = H <= t.y_rows ? 0 : t.y_halo;
@@ -134,8 +170,11 @@ void Engine::run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffer
     const uint32_t halo = D <= t.z_rows ? 0 : t.z_halo;
     const uint32_t rows_lds = std::min<uint32_t>(D, chunk + 2 * halo);
     dim3 grid((W + 31) / 32, H, (D + chunk - 1) / chunk);
+    if (chunk < kMinZChunk && chunk < D) throw std::invalid_argument("z tile chunk below kMinZChunk");
     BM_LAUNCH("edt_z", k_edt_z, grid, 256, rows_lds * 32 * sizeof(uint32_t), s, b.h, d2, W, H, D,
-              chunk, halo, b.status);
+              chunk, halo, b.zmax);
+    BM_LAUNCH("reduce_status", k_reduce_status, 1, 1024, 0, s, b.xpart, xblocks, b.zmax,
+              grid.x * grid.y * grid.z, b.status);
   }
   BM_HIP(hipGetLastError());
 }
@@ -149,24 +188,50 @@ void Engine::run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t 
 
 void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out,
                            float scale_f32, int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out,
-                           uint32_t* propagated_out) {
+                           uint32_t* propagated_out, const Tuning& tune) {
   const uint32_t W = dims.w, H = dims.h, D = dims.d;
   const uint64_t n = dims.n();
   const uint32_t bound = (uint32_t)b.bound;
   BM_LAUNCH("luts", k_luts, grid_for((uint64_t)bound + 1, 256, 1u << 30), 256, 0, s, b.status,
             bound, b.tmpl, b.diam);
-  BM_LAUNCH("ridges", k_ridges, grid_for(n, 256, 65536), 256, 0, s, b.d2, W, H, D, b.tmpl, bound,
-            b.status, b.ridges);
+  const uint32_t TX = b.tiles[0], TY = b.tiles[1], TZ = b.tiles[2];
+  const uint32_t ntiles = TX * TY * TZ;
+  BM_LAUNCH("ridges", k_ridges_tiled, ntiles, 512, 0, s, b.d2, W, H, D, TX, TY, TZ, b.tmpl, bound,
+            b.status, b.lists, b.meta, b.masks);
   uint32_t* M = b.h;  // the Y-pass buffer is free once the Z pass has run
-  BM_LAUNCH("zero_map", k_fill_u32, grid_for(n, 256, 65536), 256, 0, s, M, n, 0u);
-  BM_LAUNCH("fill_scatter", k_fill_scatter, 8192, 256, 0, s, b.ridges, b.status, W, H, D, M);
+  k_fill_u32<<<1, 64, 0, s>>>(b.overflow, 1, 0u);
+  // The gather fills every tile when the largest radius is at most
+  // kGatherMaxRadius; otherwise it returns at once and the scatter runs.
+  uint32_t cap = tune.fill_cap ? std::min<uint32_t>(tune.fill_cap, kFillCap) : kFillCap;
+  if (const char* env = std::getenv("BM_FILL_CAP")) {  // tuning / test override
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 1 && v <= kFillCap) cap = (uint32_t)v;
+  }
+  const uint32_t diag = std::getenv("BM_FILL_DIAG") ? 1u : 0u;  // work counters
+  unsigned long long* diag_buf = nullptr;
+  if (diag) {
+    diag_buf = static_cast<unsigned long long*>(scratch_bytes((size_t)ntiles * 8 * 16 * 8, 2));
+    k_fill_u32<<<grid_for((uint64_t)ntiles * 8 * 32, 256, 65536), 256, 0, s>>>(
+        reinterpret_cast<uint32_t*>(diag_buf), (uint64_t)ntiles * 8 * 32, 0u);
+  }
+  BM_LAUNCH("fill_gather", k_fill_gather, ntiles, 512, 0, s, W, H, D, TX, TY, TZ, b.lists, b.meta,
+            b.masks, b.status, M, b.overflow, cap, diag, diag_buf);
+  if (diag) k_reduce_diag<<<1, 256, 0, s>>>(diag_buf, (uint64_t)ntiles * 8, b.status);
+  BM_LAUNCH("fill_overflow", k_fill_overflow, 256, 512, 0, s, W, H, D, TX, TY, TZ, b.lists,
+            b.meta, b.status, M, b.overflow);
+  BM_LAUNCH("fill_scatter_zero", k_zero_if_scatter, grid_for(n, 256, 65536), 256, 0, s, M, n,
+            b.status);
+  BM_LAUNCH("fill_scatter", k_fill_scatter_lists, 2048, 256, 0, s, W, H, D, TX, TY, ntiles, b.lists,
+            b.meta, b.status, M);
   if (ridge_flags_out) {
+    k_count_ridges<<<256, 256, 0, s>>>(b.meta, ntiles, b.status);
     k_fill_u8<<<grid_for(n, 256, 65536), 256, 0, s>>>(ridge_flags_out, n, 0);
-    k_ridge_flags<<<4096, 256, 0, s>>>(b.ridges, b.status, ridge_flags_out);
+    k_lists_to_flags<<<4096, 256, 0, s>>>(W, H, D, TX, TY, ntiles, b.lists, b.meta,
+                                          ridge_flags_out);
   }
   if (propagated_out) BM_HIP(hipMemcpyAsync(propagated_out, M, n * 4, hipMemcpyDeviceToDevice, s));
-  BM_LAUNCH("cleanup_stats", k_cleanup_stats, kStatsBlocks, 256, 0, s, M, b.d2, b.diam, b.status,
-            W, H, D, scale_f32, lsb_exp, map_out, b.partials);
+  BM_LAUNCH("cleanup_stats", k_cleanup_tiled, kStatsBlocks, 512, 0, s, M, b.masks, b.diam,
+            b.status, W, H, D, TX, TY, ntiles, scale_f32, lsb_exp, map_out, b.partials);
   BM_LAUNCH("stats_final", k_stats_final, 1, 256, 0, s, b.partials, kStatsBlocks, b.stats);
   BM_HIP(hipGetLastError());
 }

@@ -16,27 +16,10 @@
 // fill, the surface clean-up with the phase mask, and exact statistics.
 #include <hip/hip_runtime.h>
 
+#include "bm_device.h"
 #include "bm_engine.h"
 
 namespace bm {
-
-typedef unsigned __int128 u128;
-
-__device__ __forceinline__ uint32_t sq16(uint32_t d) { return d == kInf16 ? kInf32 : d * d; }
-
-__device__ __forceinline__ bool selected_voxel(uint8_t v, uint8_t r, uint32_t want) {
-  return r != 0 && ((v != 0) == (want != 0));
-}
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-  return v;
-}
 
 // ---------------------------------------------------------------------------
 // X pass.  Sites bits of 8 consecutive voxels x..x+7 of a row; bit j set when
@@ -77,7 +60,7 @@ constexpr int kNoneR = (1 << 21);
 
 __global__ void __launch_bounds__(256)
 k_edt_x(const uint8_t* __restrict__ data, const uint8_t* __restrict__ roi, uint32_t want,
-        uint16_t* __restrict__ out, uint32_t W, uint32_t rows, PhaseStatus* __restrict__ st) {
+        uint16_t* __restrict__ out, uint32_t W, uint32_t rows, BlockPartial* __restrict__ part) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
@@ -152,11 +135,25 @@ k_edt_x(const uint8_t* __restrict__ data, const uint8_t* __restrict__ roi, uint3
       }
     }
   }
+  // Per-block partials (reduced by k_reduce_status): no contended atomics.
   const unsigned long long tot = wave_sum_u64(selcnt);
   const unsigned long long anyb = __ballot(anysite != 0);
+  __shared__ unsigned long long s_sel[4];
+  __shared__ uint32_t s_any[4];
   if (lane == 0) {
-    if (tot) atomicAdd(&st->selected, tot);
-    if (anyb) atomicOr(&st->has_site, 1u);
+    s_sel[threadIdx.x >> 6] = tot;
+    s_any[threadIdx.x >> 6] = anyb != 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0;
+    uint32_t b = 0;
+    for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) {
+      a += s_sel[i];
+      b |= s_any[i];
+    }
+    part[blockIdx.x].selected = a;
+    part[blockIdx.x].flag = b;
   }
 }
 
@@ -217,7 +214,7 @@ k_edt_y(const uint16_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W,
 // Z pass: tile of 32 x-columns x z-range [za, zb) at one y in LDS (u32).
 __global__ void __launch_bounds__(256)
 k_edt_z(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W, uint32_t H,
-        uint32_t D, uint32_t chunk, uint32_t halo, PhaseStatus* __restrict__ st) {
+        uint32_t D, uint32_t chunk, uint32_t halo, uint32_t* __restrict__ zmax) {
   extern __shared__ uint32_t ztile[];
   const uint32_t x0 = blockIdx.x * 32, y = blockIdx.y;
   const uint32_t z0 = blockIdx.z * chunk, z1 = min(D, z0 + chunk);
@@ -270,46 +267,48 @@ k_edt_z(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W,
     }
   }
   mx = wave_max_u32(mx);
-  if (lane == 0 && mx) atomicMax(&st->max_d2, mx);
-}
-
-// ---------------------------------------------------------------------------
-// integer_sqrt (src/local_thickness.rs:338-348): exact floor(sqrt(v)).
-__device__ __forceinline__ uint32_t isqrt_u32(uint32_t v) {
-  uint32_t r = (uint32_t)sqrtf((float)v);
-  while ((uint64_t)(r + 1) * (r + 1) <= v) ++r;
-  while ((uint64_t)r * r > v) --r;
-  return r;
-}
-__device__ __forceinline__ uint32_t ceil_sqrt_u32(uint32_t v) {
-  const uint32_t f = isqrt_u32(v);
-  return f * f == v ? f : f + 1;
-}
-
-// scan_cube (src/local_thickness.rs:75-102) for one r^2.
-__device__ uint32_t scan_cube(uint32_t dx, uint32_t dy, uint32_t dz, uint32_t r2) {
-  const uint32_t radius = isqrt_u32(r2) + 1;
-  uint32_t maximum = 0;
-  for (uint32_t k = 0; k <= radius; ++k) {
-    const uint32_t sk = k * k, dk = (k + dz) * (k + dz);
-    for (uint32_t j = 0; j <= radius; ++j) {
-      const uint32_t skj = sk + j * j;
-      if (skj > r2) break;  // j*j grows with j: the rest of the row is outside too
-      const uint32_t ip = isqrt_u32(r2 - skj) + dx;
-      maximum = max(maximum, dk + (j + dy) * (j + dy) + ip * ip);
-    }
+  __shared__ uint32_t s_mx[4];
+  if (lane == 0) s_mx[wv] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < nwv; ++i) r = max(r, s_mx[i]);
+    zmax[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
   }
-  return maximum;
 }
 
-// Diameter (2 * sqrt_f64(m)) as f32 (src/local_thickness.rs:206).  For
-// m < 2^24, (float)m is exact and the correctly rounded f32 sqrt scaled by 2
-// equals RN32(2 sqrt(m)); the reference's RN32(RN64(2 sqrt(m))) is the same
-// value because 2 sqrt(m) never lies within 2^-53 (relative) of an f32
-// rounding midpoint.  Larger m take the f64 route.
-__device__ __forceinline__ float diameter_of(uint32_t m) {
-  if (m < (1u << 24)) return 2.0f * sqrtf((float)m);
-  return (float)(2.0 * sqrt((double)m));
+// Status of a phase from the EDT passes' per-block partials (one block).
+__global__ void __launch_bounds__(1024)
+k_reduce_status(const BlockPartial* __restrict__ xpart, uint32_t nx, const uint32_t* __restrict__ zmax,
+                uint32_t nz, PhaseStatus* __restrict__ st) {
+  unsigned long long sel = 0;
+  uint32_t any = 0, mx = 0;
+  for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
+    sel += xpart[i].selected;
+    any |= xpart[i].flag;
+  }
+  for (uint32_t i = threadIdx.x; i < nz; i += blockDim.x) mx = max(mx, zmax[i]);
+  sel = wave_sum_u64(sel);
+  mx = wave_max_u32(mx);
+  any = __ballot(any != 0) != 0;
+  __shared__ unsigned long long s_sel[16];
+  __shared__ uint32_t s_mx[16], s_any[16];
+  if ((threadIdx.x & 63) == 0) {
+    s_sel[threadIdx.x >> 6] = sel;
+    s_mx[threadIdx.x >> 6] = mx;
+    s_any[threadIdx.x >> 6] = any;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t i = 1; i < (blockDim.x >> 6); ++i) {
+      sel += s_sel[i];
+      mx = max(mx, s_mx[i]);
+      any |= s_any[i];
+    }
+    st->selected = sel;
+    st->has_site = any;
+    st->max_d2 = mx;
+  }
 }
 
 // Templates T_c[r^2] and diameters for every r^2 in [0, max_d2].
@@ -325,243 +324,6 @@ k_luts(const PhaseStatus* __restrict__ st, uint32_t bound, uint32_t* __restrict_
   tmpl[m] = scan_cube(1, 0, 0, m);
   tmpl[(bound + 1) + m] = scan_cube(1, 1, 0, m);
   tmpl[2 * (bound + 1) + m] = scan_cube(1, 1, 1, m);
-}
-
-// ---------------------------------------------------------------------------
-// distance_ridges (src/local_thickness.rs:104-167): a selected voxel (r^2 > 0)
-// is a ridge unless an in-bounds 26-neighbour has d2 >= T_c(r^2), c = number of
-// nonzero offset components.
-__global__ void __launch_bounds__(256)
-k_ridges(const uint32_t* __restrict__ d2, uint32_t W, uint32_t H, uint32_t D,
-         const uint32_t* __restrict__ tmpl, uint32_t bound, PhaseStatus* __restrict__ st,
-         uint2* __restrict__ list) {
-  if (!st->has_site) return;
-  const uint64_t n = (uint64_t)W * H * D;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-    const uint64_t idx = base + threadIdx.x;
-    bool ridge = false;
-    uint32_t r2 = 0;
-    if (idx < n) {
-      r2 = d2[idx];
-      if (r2 != 0) {
-        const uint32_t x = (uint32_t)(idx % W);
-        const uint64_t t = idx / W;
-        const uint32_t y = (uint32_t)(t % H), z = (uint32_t)(t / H);
-        const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
-        ridge = true;
-        for (int dz = -1; dz <= 1 && ridge; ++dz) {
-          const int nz = (int)z + dz;
-          if (nz < 0 || nz >= (int)D) continue;
-          for (int dy = -1; dy <= 1 && ridge; ++dy) {
-            const int ny = (int)y + dy;
-            if (ny < 0 || ny >= (int)H) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-              if (dx == 0 && dy == 0 && dz == 0) continue;
-              const int nx = (int)x + dx;
-              if (nx < 0 || nx >= (int)W) continue;
-              const int c = (dx != 0) + (dy != 0) + (dz != 0);
-              const uint32_t tv = c == 1 ? t1 : (c == 2 ? t2 : t3);
-              if (d2[((size_t)nz * H + ny) * W + nx] >= tv) {
-                ridge = false;
-                break;
-              }
-            }
-          }
-        }
-      }
-    }
-    const unsigned long long m = __ballot(ridge);
-    if (m) {
-      unsigned long long slot0 = 0;
-      if (lane == 0) slot0 = atomicAdd(&st->ridge_count, (unsigned long long)__popcll(m));
-      slot0 = __shfl(slot0, 0);
-      if (ridge) {
-        const unsigned long long below = m & ((1ull << lane) - 1ull);
-        list[slot0 + __popcll(below)] = make_uint2((uint32_t)idx, r2);
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// propagate_ridges (src/local_thickness.rs:169-203) as an atomic-max scatter:
-// one wavefront per ridge, lanes over the (2R+1)^2 (y, x) plane of each z.
-__global__ void __launch_bounds__(256)
-k_fill_scatter(const uint2* __restrict__ list, const PhaseStatus* __restrict__ st, uint32_t W,
-               uint32_t H, uint32_t D, uint32_t* __restrict__ M) {
-  if (!st->has_site) return;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint64_t count = st->ridge_count;
-  for (uint64_t r = gw; r < count; r += nw) {
-    const uint2 e = list[r];
-    const uint32_t r2 = e.y;
-    const uint32_t x = e.x % W;
-    const uint32_t t = e.x / W;
-    const uint32_t y = t % H, z = t / H;
-    const int R = (int)ceil_sqrt_u32(r2);
-    const int side = 2 * R + 1;
-    const int plane = side * side;
-    for (int dz = -R; dz <= R; ++dz) {
-      const int nz = (int)z + dz;
-      const uint32_t dz2 = (uint32_t)(dz * dz);
-      if (nz < 0 || nz >= (int)D || dz2 > r2) continue;
-      for (int k = lane; k < plane; k += 64) {
-        const int dy = k / side - R, dx = k % side - R;
-        const int ny = (int)y + dy, nx = (int)x + dx;
-        if (ny < 0 || ny >= (int)H || nx < 0 || nx >= (int)W) continue;
-        if (dz2 + (uint32_t)(dy * dy) + (uint32_t)(dx * dx) > r2) continue;
-        atomicMax(&M[((size_t)nz * H + ny) * W + nx], r2);
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// NEIGHBORS_26 in Fiji's visiting order (src/local_thickness.rs:283-312); the
-// f32 sum of the clean-up runs in this order.
-__constant__ int8_t c_nb26[26][3] = {
-    {0, 0, -1},  {0, 0, 1},   {0, -1, 0},  {0, 1, 0},   {-1, 0, 0},  {1, 0, 0},  {0, 1, -1},
-    {0, 1, 1},   {1, -1, 0},  {1, 1, 0},   {-1, 0, 1},  {1, 0, 1},   {0, -1, -1}, {0, -1, 1},
-    {-1, -1, 0}, {-1, 1, 0},  {-1, 0, -1}, {1, 0, -1},  {1, 1, 1},   {1, -1, 1},  {-1, 1, 1},
-    {-1, -1, 1}, {1, 1, -1},  {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}};
-
-// Pass 1 of clean_up_surface_jaggies (src/local_thickness.rs:214-238): a
-// nonzero voxel with an in-bounds 26-neighbour equal to zero is a surface voxel.
-// Out-of-bounds reads are -1 (src/local_thickness.rs:325-327), never zero.
-__device__ __forceinline__ bool surface_voxel(const uint32_t* __restrict__ M, int x, int y, int z,
-                                              int W, int H, int D) {
-  if (M[((size_t)z * H + y) * W + x] == 0) return false;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int nz = z + dz;
-    if (nz < 0 || nz >= D) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int ny = y + dy;
-      if (ny < 0 || ny >= H) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int nx = x + dx;
-        if ((dx | dy | dz) == 0 || nx < 0 || nx >= W) continue;
-        if (M[((size_t)nz * H + ny) * W + nx] == 0) return true;
-      }
-    }
-  }
-  return false;
-}
-
-// Fixed-point accumulation of one value (positive finite f32).
-__device__ __forceinline__ void acc_value(float v, int lsb_exp, unsigned long long& cnt, u128& s,
-                                          u128& s2, uint32_t& mxb, uint32_t& flags) {
-  const uint32_t bits = __float_as_uint(v);
-  const int e = (int)((bits >> 23) & 0xFF);
-  const uint32_t mant = (bits & 0x7FFFFFu) | (e ? 0x800000u : 0u);
-  // Values lie in [2s, 2^17 s] for scale s, so shift is in [4, 21]; with
-  // shift <= 24 each k^2 < 2^96 and 2^32 of them cannot overflow 128 bits.
-  const int shift = (e ? e : 1) - 150 - lsb_exp;
-  if (shift < 0 || shift > 24) {
-    flags |= 1u;  // outside the exact fixed-point range
-    return;
-  }
-  const u128 k = (u128)mant << shift;
-  cnt += 1;
-  s += k;
-  s2 += (u128)((unsigned long long)mant * mant) << (2 * shift);
-  mxb = max(mxb, bits);
-}
-
-__device__ __forceinline__ u128 shfl_xor_u128(u128 v, int off) {
-  unsigned long long lo = (unsigned long long)v, hi = (unsigned long long)(v >> 64);
-  lo = __shfl_xor(lo, off);
-  hi = __shfl_xor(hi, off);
-  return ((u128)hi << 64) | lo;
-}
-
-__device__ void block_reduce_store(unsigned long long cnt, u128 s, u128 s2, uint32_t mxb,
-                                   uint32_t flags, StatsPartial* __restrict__ dst) {
-  for (int off = 32; off > 0; off >>= 1) {
-    cnt += __shfl_xor(cnt, off);
-    s += shfl_xor_u128(s, off);
-    s2 += shfl_xor_u128(s2, off);
-    mxb = max(mxb, (uint32_t)__shfl_xor((int)mxb, off));
-    flags |= (uint32_t)__shfl_xor((int)flags, off);
-  }
-  __shared__ unsigned long long r_cnt[16];
-  __shared__ u128 r_s[16], r_s2[16];
-  __shared__ uint32_t r_mx[16], r_fl[16];
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) {
-    r_cnt[wv] = cnt;
-    r_s[wv] = s;
-    r_s2[wv] = s2;
-    r_mx[wv] = mxb;
-    r_fl[wv] = flags;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t nwv = blockDim.x >> 6;
-    for (uint32_t i = 1; i < nwv; ++i) {
-      cnt += r_cnt[i];
-      s += r_s[i];
-      s2 += r_s2[i];
-      mxb = max(mxb, r_mx[i]);
-      flags |= r_fl[i];
-    }
-    dst->count = cnt;
-    dst->sum_lo = (unsigned long long)s;
-    dst->sum_hi = (unsigned long long)(s >> 64);
-    dst->sq_lo = (unsigned long long)s2;
-    dst->sq_hi = (unsigned long long)(s2 >> 64);
-    dst->max_bits = mxb;
-    dst->flags = flags;
-  }
-}
-
-// Clean-up passes 1-3 (src/local_thickness.rs:210-281), phase mask (:31-43)
-// and the value part of nonzero_stats (src/metrics.rs:215-226), fused.  Only
-// selected voxels (d2 > 0) carry a value.  A surface voxel's value is the f32
-// mean, in NEIGHBORS_26 order, of its positive non-surface neighbours, or its
-// own diameter when there is none.  Pass 2 only reads pass-1 results (surface
-// values become negative and are never summed), so it is order-independent.
-__global__ void __launch_bounds__(256)
-k_cleanup_stats(const uint32_t* __restrict__ M, const uint32_t* __restrict__ d2,
-                const float* __restrict__ diam, const PhaseStatus* __restrict__ st, uint32_t W,
-                uint32_t H, uint32_t D, float scale_f32, int lsb_exp, float* __restrict__ map_out,
-                StatsPartial* __restrict__ partials) {
-  const uint64_t n = (uint64_t)W * H * D;
-  const bool valid = st->has_site != 0;
-  unsigned long long cnt = 0;
-  u128 s = 0, s2 = 0;
-  uint32_t mxb = 0, flags = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
-    float t = 0.0f;
-    if (valid && d2[idx] != 0) {
-      const int x = (int)(idx % W);
-      const uint64_t q = idx / W;
-      const int y = (int)(q % H), z = (int)(q / H);
-      const uint32_t m = M[idx];
-      if (!surface_voxel(M, x, y, z, (int)W, (int)H, (int)D)) {
-        t = diam[m];
-      } else {
-        float sum = 0.0f;
-        uint32_t count = 0;
-        for (int k = 0; k < 26; ++k) {
-          const int nx = x + c_nb26[k][0], ny = y + c_nb26[k][1], nz = z + c_nb26[k][2];
-          if (nx < 0 || ny < 0 || nz < 0 || nx >= (int)W || ny >= (int)H || nz >= (int)D) continue;
-          const uint32_t mn = M[((size_t)nz * H + ny) * W + nx];
-          if (mn == 0 || surface_voxel(M, nx, ny, nz, (int)W, (int)H, (int)D)) continue;
-          sum += diam[mn];
-          count += 1;
-        }
-        t = count > 0 ? sum / (float)count : diam[m];
-      }
-      if (t > 0.0f && t < INFINITY) acc_value(t * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
-    }
-    if (map_out) map_out[idx] = t;
-  }
-  block_reduce_store(cnt, s, s2, mxb, flags, partials + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256)
@@ -589,15 +351,6 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint
 __global__ void __launch_bounds__(256) k_fill_u8(uint8_t* __restrict__ p, uint64_t n, uint8_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
-}
-
-__global__ void __launch_bounds__(256)
-k_ridge_flags(const uint2* __restrict__ list, const PhaseStatus* __restrict__ st,
-              uint8_t* __restrict__ flags) {
-  const uint64_t count = st->ridge_count;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
-    flags[list[i].x] = 1;
 }
 
 // phase_inside (src/volume.rs:97-102): u8(roi != 0 && (bone != 0) == select_bone).

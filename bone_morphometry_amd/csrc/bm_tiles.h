@@ -1,0 +1,32 @@
+// bm_tiles.h -- layout of the 8x8x8 tile products of local thickness.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bm {
+
+// Per tile: number of ridges, their largest r^2, and prefix counts: pref[j] is
+// an upper bound on the number of list entries with r^2 >= kPrefR2(j), i.e. the
+// number of entries whose bucket key r^2 >> shift is at least kPrefR2(j) >> shift.
+// Every list is sorted by decreasing bucket key; shift is the same for all
+// tiles of a volume (from the volume's largest squared distance).
+struct TileMeta {
+  uint32_t count;
+  uint32_t maxr2;
+  uint16_t pref[16];
+};
+
+constexpr int kBucketBits = 9;
+constexpr int kBuckets = 1 << kBucketBits;  // radius buckets of the counting sorts
+constexpr int kFillCap = 1024;              // LDS candidates per round of the fill
+constexpr int kMaxNeighbours = 512;         // tiles in reach kept per tile
+constexpr uint32_t kGatherMaxRadius = 128;  // beyond this the scatter fill is used
+
+// Radius thresholds of the prefix counts: (2j + 1)^2.
+inline __host__ __device__ uint32_t kPrefR2(int j) { return (uint32_t)(2 * j + 1) * (2 * j + 1); }
+
+inline uint32_t tiles_of(uint32_t n) { return (n + 7) / 8; }
+
+}  // namespace bm

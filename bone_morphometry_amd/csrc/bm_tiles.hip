@@ -1,0 +1,769 @@
+// bm_tiles.hip -- tile kernels of local thickness: ridges, sphere fill, clean-up.
+//
+// The volume is cut into 8x8x8 tiles (one 512-thread workgroup per tile, one
+// wavefront per 8x8 z-plane or per 4x4x4 sub-brick).
+//
+// Ridges (src/local_thickness.rs:104-167).  A tile stages its squared EDT with
+// a 1-voxel halo in LDS and applies the 26-neighbour template rule.  Each
+// tile's ridges are written as a list sorted by decreasing r^2 (counting sort
+// on r^2 >> shift), entry = r^2 << 9 | local index, at a fixed slot of 512
+// entries per tile.  Two bit masks per tile are written too: the selected
+// voxels (d2 > 0) and the fill targets (selected voxels and non-selected voxels
+// with a selected face neighbour).
+//
+// Sphere fill (src/local_thickness.rs:169-208).  M[v] = max r^2(c) over ridges
+// c with |v - c|^2 <= r^2(c).  Instead of scattering every ball, each tile
+// gathers the ridges whose ball meets it from the sorted lists of the tiles in
+// reach, orders them by decreasing r^2 in LDS, and every 4x4x4 sub-brick scans
+// them until all of its targets are covered: in decreasing order the first
+// (bucket of) covering ball(s) gives the maximum.  Exactness of the exit test:
+//   * a selected voxel v is covered by its own ball, hence by a ridge ball
+//     (the ridge filter only drops a ball contained in a neighbour's larger
+//     ball), so it is covered by some candidate;
+//   * a non-selected voxel s is touched by a ball of centre c only if s is a
+//     nearest site of c; if s had no selected face neighbour, every in-bounds
+//     face neighbour s+e would be a site and |s+e-c|^2 < |s-c|^2 whenever
+//     (c-s).e > 0, so c - s would have no positive component in any direction,
+//     i.e. c = s -- impossible.  If s has a selected face neighbour u, then
+//     d2(u) = 1 and the unit ball of u contains s, so M[s] >= 1.
+// Hence targets are exactly the voxels with M > 0, every target is covered by
+// some candidate, and all other voxels have M = 0.
+//
+// Clean-up (src/local_thickness.rs:210-281) + phase mask (:31-43) + the value
+// part of nonzero_stats (src/metrics.rs:215-226): a tile of M with a 2-voxel
+// halo in LDS; surface marks for the 1-halo region, then the f32 neighbour
+// means in NEIGHBORS_26 order.
+#include <hip/hip_runtime.h>
+
+#include "bm_device.h"
+#include "bm_engine.h"
+#include "bm_tiles.h"
+
+namespace bm {
+
+__device__ __forceinline__ uint32_t bitlen(uint32_t v) { return v ? 32 - __clz(v) : 0; }
+
+// Shift so that (max_r2 >> shift) < kBuckets.
+__device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
+  const uint32_t b = bitlen(max_r2);
+  return b > kBucketBits ? b - kBucketBits : 0;
+}
+
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* s_tmp) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_tmp[wv] = v;
+  __syncthreads();
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) r = max(r, s_tmp[i]);
+  __syncthreads();
+  return r;
+}
+
+// Exclusive scan of s_cnt[0..kBuckets) in place by a 512-thread block; returns
+// the total.  Each thread owns kBuckets/512 consecutive counters.
+__device__ uint32_t block_exclusive_scan_buckets(uint32_t* s_cnt, uint32_t* s_tmp) {
+  constexpr int per = kBuckets / 512;
+  const uint32_t t = threadIdx.x;
+  uint32_t local[per], sum = 0;
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    local[i] = s_cnt[t * per + i];
+    sum += local[i];
+  }
+  // inclusive scan of sum across the block
+  const uint32_t lane = t & 63, wv = t >> 6;
+  uint32_t x = sum;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) s_tmp[wv] = x;
+  __syncthreads();
+  uint32_t wave_off = 0;
+  for (uint32_t i = 0; i < wv; ++i) wave_off += s_tmp[i];
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) total += s_tmp[i];
+  uint32_t run = wave_off + x - sum;  // exclusive prefix of this thread
+#pragma unroll
+  for (int i = 0; i < per; ++i) {
+    s_cnt[t * per + i] = run;
+    run += local[i];
+  }
+  __syncthreads();
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512)
+k_ridges_tiled(const uint32_t* __restrict__ d2, uint32_t W, uint32_t H, uint32_t D, uint32_t TX,
+               uint32_t TY, uint32_t TZ, const uint32_t* __restrict__ tmpl, uint32_t bound,
+               PhaseStatus* __restrict__ st, uint32_t* __restrict__ lists,
+               TileMeta* __restrict__ meta, unsigned long long* __restrict__ masks) {
+  __shared__ uint32_t s_d2[10 * 10 * 10];
+  __shared__ uint32_t s_cnt[kBuckets];
+  __shared__ uint32_t s_tmp[16];
+  if (!st->has_site) return;
+  const uint32_t t = blockIdx.x;
+  const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+  const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
+  for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
+    const int hx = (int)(i % 10), hy = (int)((i / 10) % 10), hz = (int)(i / 100);
+    const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+    uint32_t v = 0;  // outside the array: never >= a template, never selected
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
+      v = d2[((size_t)gz * H + gy) * W + gx];
+    s_d2[i] = v;
+  }
+  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  const uint32_t li = threadIdx.x;
+  const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
+  const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
+  const int c = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
+  const uint32_t r2 = valid ? s_d2[c] : 0;
+  const bool sel = r2 != 0;
+  bool target = sel;
+  if (valid && !sel) {
+    target = s_d2[c - 1] | s_d2[c + 1] | s_d2[c - 10] | s_d2[c + 10] | s_d2[c - 100] |
+             s_d2[c + 100];
+  }
+  bool ridge = sel;
+  if (sel) {
+    const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
+    for (int dz = -1; dz <= 1 && ridge; ++dz)
+      for (int dy = -1; dy <= 1 && ridge; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int k = (dx != 0) + (dy != 0) + (dz != 0);
+          if (k == 0) continue;
+          const uint32_t tv = k == 1 ? t1 : (k == 2 ? t2 : t3);
+          if (s_d2[c + dx + 10 * dy + 100 * dz] >= tv) {
+            ridge = false;
+            break;
+          }
+        }
+  }
+  const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
+  if ((li & 63) == 0) {
+    masks[(size_t)t * 16 + (li >> 6)] = msel;
+    masks[(size_t)t * 16 + 8 + (li >> 6)] = mtgt;
+  }
+  const uint32_t maxr2 = block_max_u32(ridge ? r2 : 0u, s_tmp);
+  const uint32_t shift = bucket_shift(st->max_d2);  // one key for every tile
+  const uint32_t top = st->max_d2 >> shift;
+  uint32_t bucket = 0;
+  if (ridge) {
+    bucket = top - (r2 >> shift);  // 0 = largest radii
+    atomicAdd(&s_cnt[bucket], 1u);
+  }
+  __syncthreads();
+  const uint32_t count = block_exclusive_scan_buckets(s_cnt, s_tmp);
+  // prefix counts: entries with key >= kPrefR2(j) >> shift are the buckets
+  // 0 .. top - (kPrefR2(j) >> shift), i.e. the start of the next bucket
+  if (threadIdx.x < 16) {
+    const uint32_t kj = kPrefR2((int)threadIdx.x) >> shift;
+    uint32_t p = 0;
+    if (kj <= top) {
+      const uint32_t nb = top - kj + 1;  // first bucket below the threshold
+      p = nb < (uint32_t)kBuckets ? s_cnt[nb] : count;
+    }
+    s_tmp[threadIdx.x] = p;
+  }
+  __syncthreads();
+  if (ridge) {
+    const uint32_t pos = atomicAdd(&s_cnt[bucket], 1u);
+    lists[(size_t)t * 512 + pos] = (r2 << 9) | li;
+  }
+  if (threadIdx.x == 0) {
+    TileMeta m;
+    m.count = count;
+    m.maxr2 = maxr2;
+    for (int j = 0; j < 16; ++j) m.pref[j] = (uint16_t)s_tmp[j];
+    meta[t] = m;
+    if (maxr2 >= (1u << 23)) atomicOr(&st->error, 1u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate position relative to the tile origin, biased by 512 per axis.
+__device__ __forceinline__ uint32_t pack_pos(int dx, int dy, int dz) {
+  return (uint32_t)(dx + 512) | ((uint32_t)(dy + 512) << 10) | ((uint32_t)(dz + 512) << 20);
+}
+
+__device__ __forceinline__ uint32_t box_gap(int lo, int hi, int v) {
+  return v < lo ? (uint32_t)(lo - v) : (v > hi ? (uint32_t)(v - hi) : 0u);
+}
+
+// Number of entries of a tile list that can reach a box at squared distance
+// bd2: those whose bucket key is at least that of kPrefR2(j), j the largest
+// with kPrefR2(j) <= bd2 (an upper bound read from pref[]).
+__device__ __forceinline__ uint32_t reach_len(const TileMeta* __restrict__ m, uint32_t count,
+                                              uint32_t bd2) {
+  if (bd2 < 1) return count;
+  // (2j+1)^2 <= bd2  <=>  j <= (isqrt(bd2) - 1) / 2
+  const uint32_t j = min((isqrt_u32(bd2) - 1) >> 1, 15u);
+  return m->pref[j];
+}
+
+// Runs of equal bucket among the kept lanes of a wavefront whose buckets do
+// not decrease with the lane index (a chunk of a sorted list).  Returns, for a
+// kept lane, its rank within its run and (for the run's first kept lane,
+// `head`) the run length; one LDS atomic per run then replaces one per lane.
+__device__ __forceinline__ void run_of_bucket(bool keep, uint32_t bucket, uint32_t lane,
+                                              uint32_t& rank, uint32_t& len, bool& head) {
+  const unsigned long long km = __ballot(keep);
+  // a kept lane starts a run when no earlier kept lane has the same bucket;
+  // buckets are monotone, so it suffices to find the previous kept lane
+  const unsigned long long below = km & ((1ull << lane) - 1ull);
+  const int pk = below ? 63 - __clzll(below) : -1;
+  const uint32_t pkb = (uint32_t)__shfl((int)bucket, pk < 0 ? 0 : pk);
+  head = keep && (pk < 0 || pkb != bucket);
+  const unsigned long long hm = __ballot(head);
+  // run of this lane: from the last head at or below it to the next head
+  const unsigned long long hb = hm & ((2ull << lane) - 1ull);  // heads at or below (lane<63 ok)
+  const int h0 = lane == 63 ? (hm ? 63 - __clzll(hm) : 0) : (hb ? 63 - __clzll(hb) : 0);
+  const unsigned long long above = lane == 63 ? 0ull : (hm & (~0ull << (lane + 1)));
+  const int h1 = above ? __ffsll(above) - 1 : 64;
+  const unsigned long long seg = (h1 == 64 ? ~0ull : ((1ull << h1) - 1ull)) & ~((1ull << h0) - 1ull);
+  rank = (uint32_t)__popcll(km & seg & ((1ull << lane) - 1ull));
+  len = (uint32_t)__popcll(km & seg);
+}
+
+
+// Candidate of a list entry of tile (ux, uy, uz) (tile units) for the tile box
+// [ox, hx] x [oy, hy] x [oz, hz]: position relative to (ox, oy, oz) and r^2;
+// returns false when the ball misses the box.
+__device__ __forceinline__ bool entry_candidate(uint32_t ent, int ux, int uy, int uz, int ox,
+                                                int oy, int oz, int hx, int hy, int hz,
+                                                uint32_t& pos, uint32_t& r2) {
+  r2 = ent >> 9;
+  const uint32_t l = ent & 511;
+  const int cx = ux * 8 + (int)(l & 7), cy = uy * 8 + (int)((l >> 3) & 7),
+            cz = uz * 8 + (int)(l >> 6);
+  const uint32_t ex = box_gap(ox, hx, cx), ey = box_gap(oy, hy, cy), ez = box_gap(oz, hz, cz);
+  pos = pack_pos(cx - ox, cy - oy, cz - oz);
+  return ex * ex + ey * ey + ez * ez <= r2;
+}
+
+__global__ void __launch_bounds__(512)
+k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t TZ,
+              const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
+              const unsigned long long* __restrict__ masks, PhaseStatus* __restrict__ st,
+              uint32_t* __restrict__ M, uint32_t* __restrict__ overflow, uint32_t cap,
+              uint32_t diag, unsigned long long* __restrict__ diag_buf) {
+  __shared__ uint32_t s_pos[kFillCap], s_r2[kFillCap];  // one round, sorted by bucket
+  __shared__ uint32_t s_cnt[kBuckets];                  // histogram, then cursors
+  __shared__ uint32_t s_start[kBuckets + 1];            // bucket starts over all rounds
+  __shared__ uint32_t s_nbr[kMaxNeighbours];            // neighbour tile in reach
+  __shared__ uint32_t s_len[kMaxNeighbours];            // its useful list prefix
+  __shared__ uint32_t s_org[kMaxNeighbours];            // its tile coordinates, packed
+  __shared__ uint32_t s_off[kBuckets];                  // chunk (or entry) offsets
+  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_nn, s_over, s_pending;
+  __shared__ unsigned long long s_tgt[8];
+  if (!st->has_site) return;
+  const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
+  unsigned long long clk_p3 = 0, clk_p4 = 0;
+  const uint32_t t = blockIdx.x;
+  const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+  const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
+  const int hx = min(ox + 7, (int)W - 1), hy = min(oy + 7, (int)H - 1), hz = min(oz + 7, (int)D - 1);
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t max_r2 = st->max_d2;
+  const int R = (int)ceil_sqrt_u32(max_r2);
+  if (R > (int)kGatherMaxRadius) return;  // the scatter fill handles this volume
+  if (threadIdx.x < 8) s_tgt[threadIdx.x] = masks[(size_t)t * 16 + 8 + threadIdx.x];
+  if (threadIdx.x == 0) {
+    s_nn = 0;
+    s_over = 0;
+  }
+  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) {
+    s_cnt[i] = 0;
+    s_off[i] = 0;
+  }
+  __syncthreads();
+  const unsigned long long ta = diag ? wall_clock64() : 0ull;
+  bool any_target = false;
+  for (int i = 0; i < 8; ++i) any_target |= s_tgt[i] != 0;
+  const int sx = (int)(wv & 1) * 4, sy = (int)((wv >> 1) & 1) * 4, sz = (int)(wv >> 2) * 4;
+  const int vx = sx + (int)(lane & 3), vy = sy + (int)((lane >> 2) & 3), vz = sz + (int)(lane >> 4);
+  const uint32_t li = (uint32_t)(vx + 8 * vy + 64 * vz);
+  const bool valid = ox + vx < (int)W && oy + vy < (int)H && oz + vz < (int)D;
+  const size_t gidx = ((size_t)(oz + vz) * H + (oy + vy)) * W + (ox + vx);
+  if (!any_target) {
+    if (valid) M[gidx] = 0;
+    return;
+  }
+  // P1: tiles in reach and the length of their useful list prefix.
+  const int K = (R + 7) / 8;
+  const uint32_t side = 2 * K + 1;
+  // side <= 8 (K <= 3, the common case): 3-bit fields of i instead of divisions
+  const uint32_t nnb = side <= 8 ? 512u : side * side * side;
+  for (uint32_t i = threadIdx.x; i < nnb; i += blockDim.x) {
+    int ix, iy, iz;
+    if (side <= 8) {
+      ix = (int)(i & 7);
+      iy = (int)((i >> 3) & 7);
+      iz = (int)(i >> 6);
+      if (ix >= (int)side || iy >= (int)side || iz >= (int)side) continue;
+    } else {
+      ix = (int)(i % side);
+      iy = (int)((i / side) % side);
+      iz = (int)(i / (side * side));
+    }
+    const int ux = (int)tx + ix - K, uy = (int)ty + iy - K, uz = (int)tz + iz - K;
+    if (ux < 0 || uy < 0 || uz < 0 || ux >= (int)TX || uy >= (int)TY || uz >= (int)TZ) continue;
+    const uint32_t u = ((uint32_t)uz * TY + (uint32_t)uy) * TX + (uint32_t)ux;
+    const uint32_t mcount = meta[u].count, mmax = meta[u].maxr2;
+    if (mcount == 0) continue;
+    const int ulx = ux * 8, uly = uy * 8, ulz = uz * 8;
+    const int uhx = min(ulx + 7, (int)W - 1), uhy = min(uly + 7, (int)H - 1),
+              uhz = min(ulz + 7, (int)D - 1);
+    const uint32_t gx = ulx > hx ? (uint32_t)(ulx - hx) : (ox > uhx ? (uint32_t)(ox - uhx) : 0u);
+    const uint32_t gy = uly > hy ? (uint32_t)(uly - hy) : (oy > uhy ? (uint32_t)(oy - uhy) : 0u);
+    const uint32_t gz = ulz > hz ? (uint32_t)(ulz - hz) : (oz > uhz ? (uint32_t)(oz - uhz) : 0u);
+    const uint32_t bd2 = gx * gx + gy * gy + gz * gz;
+    if (mmax < bd2) continue;
+    const uint32_t len = reach_len(meta + u, mcount, bd2);
+    if (len == 0) continue;
+    const uint32_t slot = atomicAdd(&s_nn, 1u);
+    if (slot < kMaxNeighbours) {
+      s_nbr[slot] = u;
+      s_len[slot] = len;
+      s_org[slot] = (uint32_t)ux | ((uint32_t)uy << 10) | ((uint32_t)uz << 20);
+    } else {
+      s_over = 1;
+    }
+  }
+  __syncthreads();
+  const unsigned long long tb = diag ? wall_clock64() : 0ull;
+  if (s_over) {  // too many tiles in reach: exact slow path
+    if (threadIdx.x == 0) {
+      overflow[1 + atomicAdd(&overflow[0], 1u)] = t;
+      atomicAdd(&st->overflow_tiles, 1u);
+    }
+    return;
+  }
+  const uint32_t nn = s_nn;
+  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = (s_len[j] + 63) >> 6;
+  __syncthreads();
+  const uint32_t nchunks = block_exclusive_scan_buckets(s_off, s_tmp);  // s_off: chunk offsets
+  const unsigned long long tc = diag ? wall_clock64() : 0ull;
+  unsigned long long td = tc;
+  const uint32_t gshift = bucket_shift(max_r2);
+  const uint32_t gtop = max_r2 >> gshift;
+  // Chunks q = wv, wv + 8, ... of the flattened (neighbour, 64 entries) range;
+  // one chunk is a sorted run of one list.  Read four chunks per trip.
+  auto chunk_pass = [&](auto&& use) {
+    int cur = 0;  // neighbour slot of chunk q (q grows)
+    for (uint32_t q0 = wv; q0 < nchunks; q0 += 32) {
+      uint32_t ent[4], org[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t q = q0 + 8u * k;
+        ent[k] = 0;
+        org[k] = 0;
+        if (q < nchunks) {
+          while (cur + 1 < (int)nn && s_off[cur + 1] <= q) ++cur;
+          const uint32_t e = (q - s_off[cur]) * 64 + lane;
+          org[k] = s_org[cur];
+          if (e < s_len[cur]) ent[k] = lists[(size_t)s_nbr[cur] * 512 + e];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (q0 + 8u * k >= nchunks) break;  // uniform over the wave
+        uint32_t pos = 0, r2 = 0;
+        const bool keep = ent[k] != 0 &&
+                          entry_candidate(ent[k], (int)(org[k] & 1023), (int)((org[k] >> 10) & 1023),
+                                          (int)(org[k] >> 20), ox, oy, oz, hx, hy, hz, pos, r2);
+        use(keep, pos, r2);
+      }
+    }
+  };
+  // P2: histogram by bucket, one LDS atomic per run of equal buckets.
+  chunk_pass([&](bool keep, uint32_t pos, uint32_t r2) {
+    (void)pos;
+    const uint32_t b = keep ? gtop - (r2 >> gshift) : 0u;
+    uint32_t rank, len;
+    bool head;
+    run_of_bucket(keep, b, lane, rank, len, head);
+    if (head) atomicAdd(&s_cnt[b], len);
+  });
+  __syncthreads();
+  const uint32_t n = block_exclusive_scan_buckets(s_cnt, s_tmp);
+  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_start[i] = s_cnt[i];
+  if (threadIdx.x == 0) s_start[kBuckets] = n;
+  __syncthreads();
+  const unsigned long long clk1 = diag ? wall_clock64() : 0ull;
+  // Rounds over bucket ranges that fit in LDS, largest radii first.
+  const bool target = (s_tgt[li >> 6] >> (li & 63)) & 1ull;
+  uint32_t m = 0;
+  bool done = __ballot(target) == 0;
+  uint32_t held = 0xFFFFFFFFu;  // bucket key that must be finished before leaving
+  unsigned long long tests = 0, scanned = 0, rounds = 0;
+  uint32_t b0 = 0;
+  while (b0 <= gtop) {
+    ++rounds;
+    uint32_t b1 = b0 + 1;
+    if (s_start[b1] - s_start[b0] > cap) {
+      if (threadIdx.x == 0) {  // one bucket alone exceeds LDS: exact slow path
+        overflow[1 + atomicAdd(&overflow[0], 1u)] = t;
+        atomicAdd(&st->overflow_tiles, 1u);
+      }
+      return;
+    }
+    {  // largest b1 <= gtop + 1 with s_start[b1] - s_start[b0] <= cap (starts are monotone)
+      uint32_t lo = b1, hi = gtop + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] - s_start[b0] <= cap) lo = mid;
+        else hi = mid - 1;
+      }
+      b1 = lo;
+    }
+    const uint32_t base = s_start[b0], cnt_round = s_start[b1] - base;
+    const unsigned long long c3 = diag ? wall_clock64() : 0ull;
+    // P3: place this round's candidates at their sorted positions.
+    chunk_pass([&](bool keep0, uint32_t pos, uint32_t r2) {
+      const uint32_t b = keep0 ? gtop - (r2 >> gshift) : 0u;
+      const bool keep = keep0 && b >= b0 && b < b1;
+      uint32_t rank, len;
+      bool head;
+      run_of_bucket(keep, b, lane, rank, len, head);
+      uint32_t k0 = 0;
+      if (head) k0 = atomicAdd(&s_cnt[b], len);
+      const unsigned long long hm = __ballot(head);
+      const unsigned long long hb = hm & ((2ull << lane) - 1ull);
+      const int hl = hb ? 63 - __clzll(hb) : 0;
+      k0 = (uint32_t)__shfl((int)k0, hl);
+      if (keep) {
+        const uint32_t k = k0 + rank - base;
+        s_pos[k] = pos;
+        s_r2[k] = r2;
+      }
+    });
+    __syncthreads();
+    const unsigned long long c4 = diag ? wall_clock64() : 0ull;
+    clk_p3 += c4 - c3;
+    // P4: this wave's 4x4x4 sub-brick scans the round in order.
+    if (!done) {
+      // Box of the voxels still waiting (lane = x + 4y + 16z of the
+      // sub-brick), from the wave's mask with scalar bit operations; a
+      // candidate that misses it cannot change any result.
+      unsigned long long wait = __ballot(target && m == 0);
+      int bx0 = sx, bx1 = sx + 3, by0 = sy, by1 = sy + 3, bz0 = sz, bz1 = sz + 3;
+      for (uint32_t c0 = 0; c0 < cnt_round; c0 += 64) {
+        ++scanned;
+        if (held == 0xFFFFFFFFu && wait) {
+          uint32_t zm = 0, rows = 0;
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t g = (uint32_t)(wait >> (16 * k)) & 0xFFFFu;
+            zm |= (g != 0) << k;
+            rows |= g;
+          }
+          const uint32_t ym = ((rows & 0xF) != 0) | (((rows >> 4) & 0xF) != 0) << 1 |
+                              (((rows >> 8) & 0xF) != 0) << 2 | (((rows >> 12) & 0xF) != 0) << 3;
+          const uint32_t xm = (rows | rows >> 4 | rows >> 8 | rows >> 12) & 0xF;
+          bx0 = sx + __ffs(xm) - 1;
+          bx1 = sx + 31 - __clz(xm);
+          by0 = sy + __ffs(ym) - 1;
+          by1 = sy + 31 - __clz(ym);
+          bz0 = sz + __ffs(zm) - 1;
+          bz1 = sz + 31 - __clz(zm);
+        }
+        const uint32_t ci = c0 + lane;
+        uint32_t cp = 0, cr = 0;
+        bool rel = false;
+        if (ci < cnt_round) {
+          cp = s_pos[ci];
+          cr = s_r2[ci];
+          const int cx = (int)(cp & 1023) - 512, cy = (int)((cp >> 10) & 1023) - 512,
+                    cz = (int)(cp >> 20) - 512;
+          const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
+                    ez = max(max(bz0 - cz, cz - bz1), 0);
+          rel = (uint32_t)(ex * ex + ey * ey + ez * ez) <= cr;
+          if (held != 0xFFFFFFFFu && (cr >> gshift) != held) rel = false;
+        }
+        unsigned long long rm = __ballot(rel);
+        while (rm) {
+          const int j = __ffsll((unsigned long long)rm) - 1;
+          rm &= rm - 1;
+          const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)cp, j);
+          const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j);
+          const int dx = vx - ((int)(p & 1023) - 512), dy = vy - ((int)((p >> 10) & 1023) - 512),
+                    dz = vz - ((int)(p >> 20) - 512);
+          if ((uint32_t)(dx * dx + dy * dy + dz * dz) <= r2) m = max(m, r2);
+          ++tests;
+          if (rm) {  // a second candidate per trip
+            const int j2 = __ffsll((unsigned long long)rm) - 1;
+            rm &= rm - 1;
+            const uint32_t p2 = (uint32_t)__builtin_amdgcn_readlane((int)cp, j2);
+            const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j2);
+            const int ex2 = vx - ((int)(p2 & 1023) - 512), ey2 = vy - ((int)((p2 >> 10) & 1023) - 512),
+                      ez2 = vz - ((int)(p2 >> 20) - 512);
+            if ((uint32_t)(ex2 * ex2 + ey2 * ey2 + ez2 * ez2) <= q2) m = max(m, q2);
+            ++tests;
+          }
+        }
+        wait = __ballot(target && m == 0);
+        // Leave once every target is covered: candidates come in decreasing
+        // r^2, so later ones cannot raise a maximum -- except that a bucket
+        // with shift > 0 mixes radii and is finished first (rounds never split
+        // a bucket, so it ends in this round).
+        if (wait == 0 && gshift == 0) {
+          // one r^2 per bucket: a later candidate is never larger
+          done = true;
+          break;
+        }
+        const uint32_t last = min(c0 + 63, cnt_round - 1);
+        const uint32_t lastb = s_r2[last] >> gshift;
+        if (held == 0xFFFFFFFFu && wait == 0) held = lastb;
+        if (held != 0xFFFFFFFFu) {
+          const bool more = last + 1 < cnt_round && (s_r2[last + 1] >> gshift) == held;
+          if (!more) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+    if (diag) clk_p4 += wall_clock64() - c4;
+    if (threadIdx.x == 0) s_pending = 0;
+    __syncthreads();
+    if (!done && lane == 0) s_pending = 1;
+    __syncthreads();
+    if (!s_pending) break;
+    b0 = b1;
+    __syncthreads();
+  }
+  if (valid) M[gidx] = m;
+  // Work counters and phase clocks (diagnostics only): plain stores into this
+  // block's slot, summed afterwards by k_reduce_diag.
+  if (diag && lane == 0) {
+    unsigned long long* slot = diag_buf + ((size_t)blockIdx.x * 8 + wv) * 16;
+    slot[0] = 1;  // waves that reached the end
+    slot[1] = tests;
+    slot[2] = scanned;
+    slot[3] = wv == 0 ? rounds : 0;
+    slot[4] = clk1 - clk0;
+    slot[5] = clk_p3;
+    slot[6] = clk_p4;
+    slot[7] = wall_clock64() - clk0;
+    slot[8] = ta - clk0;
+    slot[9] = tb - ta;
+    slot[10] = tc - tb;
+    slot[11] = td - tc;
+    slot[12] = clk1 - td;
+    slot[13] = wv == 0 ? n : 0;
+    slot[14] = wv == 0 ? nchunks : 0;
+    slot[15] = 0;
+  }
+}
+
+// Sum of the per-wave diagnostic slots (one block).
+__global__ void __launch_bounds__(256)
+k_reduce_diag(const unsigned long long* __restrict__ diag_buf, uint64_t nslots,
+              PhaseStatus* __restrict__ st) {
+  __shared__ unsigned long long s[16];
+  if (threadIdx.x < 16) s[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long acc[16] = {0};
+  for (uint64_t i = threadIdx.x; i < nslots; i += blockDim.x)
+    for (int k = 0; k < 16; ++k) acc[k] += diag_buf[i * 16 + k];
+  for (int k = 0; k < 16; ++k) atomicAdd(&s[k], acc[k]);
+  __syncthreads();
+  if (threadIdx.x < 16) st->fill_diag16[threadIdx.x] = s[threadIdx.x];
+}
+
+// Tiles whose candidate set did not fit in LDS: unsorted exact maximum.
+__global__ void __launch_bounds__(512)
+k_fill_overflow(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t TZ,
+                const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
+                const PhaseStatus* __restrict__ st, uint32_t* __restrict__ M,
+                const uint32_t* __restrict__ overflow) {
+  if (!st->has_site) return;
+  const uint32_t count = overflow[0];
+  const uint32_t max_r2 = st->max_d2;
+  const int R = (int)ceil_sqrt_u32(max_r2);
+  const int K = (R + 7) / 8;
+  const uint32_t gshift = bucket_shift(max_r2);
+  for (uint32_t oi = blockIdx.x; oi < count; oi += gridDim.x) {
+    const uint32_t t = overflow[1 + oi];
+    const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+    const int vx = (int)tx * 8 + (int)(threadIdx.x & 7), vy = (int)ty * 8 + (int)((threadIdx.x >> 3) & 7),
+              vz = (int)tz * 8 + (int)(threadIdx.x >> 6);
+    if (vx >= (int)W || vy >= (int)H || vz >= (int)D) continue;
+    uint32_t m = 0;
+    for (int uz = max(0, (int)tz - K); uz <= min((int)TZ - 1, (int)tz + K); ++uz)
+      for (int uy = max(0, (int)ty - K); uy <= min((int)TY - 1, (int)ty + K); ++uy)
+        for (int ux = max(0, (int)tx - K); ux <= min((int)TX - 1, (int)tx + K); ++ux) {
+          const uint32_t u = ((uint32_t)uz * TY + (uint32_t)uy) * TX + (uint32_t)ux;
+          const TileMeta mt = meta[u];
+          if (mt.maxr2 <= m) continue;  // cannot raise the maximum
+          for (uint32_t e = 0; e < mt.count; ++e) {
+            const uint32_t ent = lists[(size_t)u * 512 + e];
+            const uint32_t r2 = ent >> 9;
+            if (r2 <= m) {
+              if ((r2 >> gshift) < (m >> gshift)) break;  // sorted: the rest is smaller
+              continue;
+            }
+            const uint32_t l = ent & 511;
+            const int dx = vx - (ux * 8 + (int)(l & 7)), dy = vy - (uy * 8 + (int)((l >> 3) & 7)),
+                      dz = vz - (uz * 8 + (int)(l >> 6));
+            if ((uint32_t)(dx * dx + dy * dy + dz * dz) <= r2) m = r2;
+          }
+        }
+    M[((size_t)vz * H + vy) * W + vx] = m;
+  }
+}
+
+// Scatter fallback for very large radii: one wavefront per ridge, lanes over
+// the (2R+1)^2 (y, x) plane of each z, atomic max.
+__global__ void __launch_bounds__(256)
+k_fill_scatter_lists(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
+                     const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
+                     const PhaseStatus* __restrict__ st, uint32_t* __restrict__ M) {
+  if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t t = gw; t < ntiles; t += nw) {
+    const TileMeta mt = meta[t];
+    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    for (uint32_t e = 0; e < mt.count; ++e) {
+      const uint32_t ent = lists[(size_t)t * 512 + e];
+      const uint32_t r2 = ent >> 9, l = ent & 511;
+      const int x = ox + (int)(l & 7), y = oy + (int)((l >> 3) & 7), z = oz + (int)(l >> 6);
+      const int R = (int)ceil_sqrt_u32(r2);
+      const int side = 2 * R + 1, plane = side * side;
+      for (int dz = -R; dz <= R; ++dz) {
+        const int nz = z + dz;
+        const uint32_t dz2 = (uint32_t)(dz * dz);
+        if (nz < 0 || nz >= (int)D || dz2 > r2) continue;
+        for (int k = (int)lane; k < plane; k += 64) {
+          const int dy = k / side - R, dx = k % side - R;
+          const int ny = y + dy, nx = x + dx;
+          if (ny < 0 || ny >= (int)H || nx < 0 || nx >= (int)W) continue;
+          if (dz2 + (uint32_t)(dy * dy) + (uint32_t)(dx * dx) > r2) continue;
+          atomicMax(&M[((size_t)nz * H + ny) * W + nx], r2);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_zero_if_scatter(uint32_t* __restrict__ M, uint64_t n, const PhaseStatus* __restrict__ st) {
+  if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) M[i] = 0;
+}
+
+// Total ridge count (diagnostics) from the tile headers.
+__global__ void __launch_bounds__(256)
+k_count_ridges(const TileMeta* __restrict__ meta, uint32_t ntiles, PhaseStatus* __restrict__ st) {
+  unsigned long long c = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+    c += meta[t].count;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(&st->ridge_count, c);
+}
+
+// Ridge flags (diagnostics) from the tile lists.
+__global__ void __launch_bounds__(256)
+k_lists_to_flags(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
+                 const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
+                 uint8_t* __restrict__ flags) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileMeta mt = meta[t];
+    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    for (uint32_t e = threadIdx.x; e < mt.count; e += blockDim.x) {
+      const uint32_t l = lists[(size_t)t * 512 + e] & 511;
+      const int x = ox + (int)(l & 7), y = oy + (int)((l >> 3) & 7), z = oz + (int)(l >> 6);
+      flags[((size_t)z * H + y) * W + x] = 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Clean-up + phase mask + statistics over 8x8x8 tiles (persistent blocks).
+constexpr uint32_t kOut = 0xFFFFFFFFu;  // outside the array: never zero, never summed
+
+__global__ void __launch_bounds__(512)
+k_cleanup_tiled(const uint32_t* __restrict__ M, const unsigned long long* __restrict__ masks,
+                const float* __restrict__ diam, const PhaseStatus* __restrict__ st, uint32_t W,
+                uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
+                float scale_f32, int lsb_exp, float* __restrict__ map_out,
+                StatsPartial* __restrict__ partials) {
+  __shared__ uint32_t s_m[12 * 12 * 12];
+  __shared__ uint8_t s_mark[10 * 10 * 10];
+  unsigned long long cnt = 0;
+  u128 s = 0, s2 = 0;
+  uint32_t mxb = 0, flags = 0;
+  const bool valid_phase = st->has_site != 0;
+  for (uint32_t t = blockIdx.x; t < ntiles && valid_phase; t += gridDim.x) {
+    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    const unsigned long long selw = masks[(size_t)t * 16 + (threadIdx.x >> 6)];
+    const unsigned long long any = __ballot(selw != 0);  // waves see their own plane
+    (void)any;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 1728; i += blockDim.x) {
+      const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
+      const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
+      uint32_t v = kOut;
+      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
+        v = M[((size_t)gz * H + gy) * W + gx];
+      s_m[i] = v;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
+      const int hx = (int)(i % 10), hy = (int)((i / 10) % 10), hz = (int)(i / 100);
+      const int c = (hx + 1) + 12 * (hy + 1) + 144 * (hz + 1);
+      const uint32_t v = s_m[c];
+      bool mark = false;
+      if (v != 0 && v != kOut) {
+        for (int dz = -1; dz <= 1 && !mark; ++dz)
+          for (int dy = -1; dy <= 1 && !mark; ++dy)
+            for (int dx = -1; dx <= 1; ++dx)
+              if (s_m[c + dx + 12 * dy + 144 * dz] == 0) {
+                mark = true;
+                break;
+              }
+      }
+      s_mark[i] = mark;
+    }
+    __syncthreads();
+    const uint32_t li = threadIdx.x;
+    const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
+    const bool sel = (selw >> (li & 63)) & 1ull;
+    float tv = 0.0f;
+    if (sel) {
+      const int c12 = (lx + 2) + 12 * (ly + 2) + 144 * (lz + 2);
+      const int c10 = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
+      const uint32_t m = s_m[c12];
+      if (!s_mark[c10]) {
+        tv = diam[m];
+      } else {
+        float sum = 0.0f;
+        uint32_t count = 0;
+        for (int k = 0; k < 26; ++k) {
+          const int dx = c_nb26[k][0], dy = c_nb26[k][1], dz = c_nb26[k][2];
+          const uint32_t mn = s_m[c12 + dx + 12 * dy + 144 * dz];
+          if (mn == 0 || mn == kOut || s_mark[c10 + dx + 10 * dy + 100 * dz]) continue;
+          sum += diam[mn];
+          count += 1;
+        }
+        tv = count > 0 ? sum / (float)count : diam[m];
+      }
+      if (tv > 0.0f && tv < INFINITY) acc_value(tv * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
+    }
+    if (map_out && ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D)
+      map_out[((size_t)(oz + lz) * H + (oy + ly)) * W + (ox + lx)] = tv;
+  }
+  block_reduce_store(cnt, s, s2, mxb, flags, partials + blockIdx.x);
+}
+
+}  // namespace bm

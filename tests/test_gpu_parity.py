@@ -122,6 +122,19 @@ def test_local_thickness_stages_match_oracle(seed, shape, frac):
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
 
+@pytest.mark.parametrize("cap", ["64", "257"])
+def test_fill_rounds_match_oracle(cap, monkeypatch):
+    """A small LDS budget forces the fill through many radius rounds per tile
+    (and buckets larger than the budget through the exact overflow kernel)."""
+    monkeypatch.setenv("BM_FILL_CAP", cap)
+    w, h, d = 48, 40, 44
+    a = _structure(9, (d, h, w), 0.8)
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
 def test_local_thickness_plain_small_matches_oracle():
     wl = workloads.plain_small()
     for phase in (bm.Phase.Foreground, bm.Phase.Background):
