@@ -255,7 +255,8 @@ static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_ro
     BM_HIP(hipMemcpyAsync(&sp[ph], b.stats, sizeof(StatsPartial), hipMemcpyDeviceToHost, s));
     if (quantiles) {
       BM_HIP(hipStreamSynchronize(s));
-      if (sp[ph].count > 0) qr[ph] = device_quantiles(e, d_map, n, sp[ph].count, scale_f32, s);
+      if (sp[ph].count > 0)
+        qr[ph] = device_quantiles(e, d_map, n, sp[ph].count, scale_f32, s, b.top16);
     }
   }
   BM_HIP(hipStreamSynchronize(s));

@@ -140,4 +140,29 @@ __device__ inline void block_reduce_store(unsigned long long cnt, u128 s, u128 s
   }
 }
 
+// Adds 1 to hist[bin] for every active lane, one LDS atomic per distinct bin of
+// the wavefront (thickness maps repeat values, so lanes often share a bin).
+__device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint32_t bin) {
+  unsigned long long act = __ballot(active);
+  while (act) {
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)bin, leader);
+    const unsigned long long same = __ballot(active && bin == v);
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[v], (uint32_t)__popcll(same));
+    if (bin == v) active = false;
+    act &= ~same;
+  }
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), each XCD with its own L2.  This maps block b to a tile so that each
+// XCD walks one contiguous range of tiles: neighbouring tiles, whose halos and
+// ridge lists overlap, then share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
+  const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+  const uint32_t t = (b % kXcds) * per + b / kXcds;
+  return t;  // may be >= ntiles for the last XCD's tail: callers skip those
+}
+
 }  // namespace bm

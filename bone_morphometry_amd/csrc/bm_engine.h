@@ -117,10 +117,16 @@ struct PhaseBuffers {
   uint32_t* zmax = nullptr;          // one per Z-pass block
   StatsPartial* partials = nullptr;  // kStatsBlocks
   StatsPartial* stats = nullptr;     // final record
+  uint32_t* top16 = nullptr;         // top-16-bit value histogram (quantiles)
   uint64_t bound = 0;
 };
 
 constexpr int kStatsBlocks = 1024;
+// Top-16-bit histogram of thickness values: f32 bit patterns 0x4000xxxx to
+// 0x47FFxxxx, i.e. values in [2, 65536).  Every diameter of the clean-up is
+// at least 2 and below 2*sqrt(3)*16384.
+constexpr uint32_t kTopBase = 0x4000u;
+constexpr int kTopBins = 2048;
 constexpr int kEdtXBlocks = 4096;
 constexpr uint32_t kMinZChunk = 64;  // smallest z extent of a Z-pass tile
 

@@ -13,6 +13,10 @@
 #include "bm_tiles.h"
 
 namespace bm {
+constexpr uint32_t kXcds = 8;  // must match bm_device.h
+}
+
+namespace bm {
 
 // Kernels (bm_kernels.hip).
 __global__ void k_edt_x(const uint8_t*, const uint8_t*, uint32_t, uint16_t*, uint32_t, uint32_t,
@@ -44,7 +48,7 @@ __global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_
                                  const uint32_t*, const TileMeta*, uint8_t*);
 __global__ void k_cleanup_tiled(const uint32_t*, const unsigned long long*, const float*,
                                 const PhaseStatus*, uint32_t, uint32_t, uint32_t, uint32_t,
-                                uint32_t, uint32_t, float, int, float*, StatsPartial*);
+                                uint32_t, uint32_t, float, int, float*, StatsPartial*, uint32_t*);
 __global__ void k_stats_final(const StatsPartial*, uint32_t, StatsPartial*);
 __global__ void k_fill_u32(uint32_t*, uint64_t, uint32_t);
 __global__ void k_fill_u8(uint8_t*, uint64_t, uint8_t);
@@ -111,6 +115,7 @@ PhaseBuffers Engine::buffers_for(const Dims& dims) {
   const size_t o_zm = take(zblocks * sizeof(uint32_t));
   const size_t o_p = take(kStatsBlocks * sizeof(StatsPartial));
   const size_t o_s = take(sizeof(StatsPartial));
+  const size_t o_top = take(kTopBins * sizeof(uint32_t));
   work_.reserve(off);
   char* b = static_cast<char*>(work_.base());
   PhaseBuffers pb;
@@ -131,6 +136,7 @@ PhaseBuffers Engine::buffers_for(const Dims& dims) {
   pb.zmax = reinterpret_cast<uint32_t*>(b + o_zm);
   pb.partials = reinterpret_cast<StatsPartial*>(b + o_p);
   pb.stats = reinterpret_cast<StatsPartial*>(b + o_s);
+  pb.top16 = reinterpret_cast<uint32_t*>(b + o_top);
   pb.bound = bound;
   return pb;
 }
@@ -196,7 +202,8 @@ void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_o
             bound, b.tmpl, b.diam);
   const uint32_t TX = b.tiles[0], TY = b.tiles[1], TZ = b.tiles[2];
   const uint32_t ntiles = TX * TY * TZ;
-  BM_LAUNCH("ridges", k_ridges_tiled, ntiles, 512, 0, s, b.d2, W, H, D, TX, TY, TZ, b.tmpl, bound,
+  const uint32_t xgrid = ((ntiles + kXcds - 1) / kXcds) * kXcds;  // see xcd_tile
+  BM_LAUNCH("ridges", k_ridges_tiled, xgrid, 512, 0, s, b.d2, W, H, D, TX, TY, TZ, b.tmpl, bound,
             b.status, b.lists, b.meta, b.masks);
   uint32_t* M = b.h;  // the Y-pass buffer is free once the Z pass has run
   k_fill_u32<<<1, 64, 0, s>>>(b.overflow, 1, 0u);
@@ -210,13 +217,13 @@ void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_o
   const uint32_t diag = std::getenv("BM_FILL_DIAG") ? 1u : 0u;  // work counters
   unsigned long long* diag_buf = nullptr;
   if (diag) {
-    diag_buf = static_cast<unsigned long long*>(scratch_bytes((size_t)ntiles * 8 * 16 * 8, 2));
-    k_fill_u32<<<grid_for((uint64_t)ntiles * 8 * 32, 256, 65536), 256, 0, s>>>(
-        reinterpret_cast<uint32_t*>(diag_buf), (uint64_t)ntiles * 8 * 32, 0u);
+    diag_buf = static_cast<unsigned long long*>(scratch_bytes((size_t)xgrid * 8 * 16 * 8, 2));
+    k_fill_u32<<<grid_for((uint64_t)xgrid * 8 * 32, 256, 65536), 256, 0, s>>>(
+        reinterpret_cast<uint32_t*>(diag_buf), (uint64_t)xgrid * 8 * 32, 0u);
   }
-  BM_LAUNCH("fill_gather", k_fill_gather, ntiles, 512, 0, s, W, H, D, TX, TY, TZ, b.lists, b.meta,
+  BM_LAUNCH("fill_gather", k_fill_gather, xgrid, 512, 0, s, W, H, D, TX, TY, TZ, b.lists, b.meta,
             b.masks, b.status, M, b.overflow, cap, diag, diag_buf);
-  if (diag) k_reduce_diag<<<1, 256, 0, s>>>(diag_buf, (uint64_t)ntiles * 8, b.status);
+  if (diag) k_reduce_diag<<<1, 256, 0, s>>>(diag_buf, (uint64_t)xgrid * 8, b.status);
   BM_LAUNCH("fill_overflow", k_fill_overflow, 256, 512, 0, s, W, H, D, TX, TY, TZ, b.lists,
             b.meta, b.status, M, b.overflow);
   BM_LAUNCH("fill_scatter_zero", k_zero_if_scatter, grid_for(n, 256, 65536), 256, 0, s, M, n,
@@ -230,8 +237,10 @@ void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_o
                                           ridge_flags_out);
   }
   if (propagated_out) BM_HIP(hipMemcpyAsync(propagated_out, M, n * 4, hipMemcpyDeviceToDevice, s));
+  uint32_t* top16 = map_out ? b.top16 : nullptr;  // quantiles need the map and its histogram
+  if (top16) k_fill_u32<<<8, 256, 0, s>>>(top16, kTopBins, 0u);
   BM_LAUNCH("cleanup_stats", k_cleanup_tiled, kStatsBlocks, 512, 0, s, M, b.masks, b.diam,
-            b.status, W, H, D, TX, TY, ntiles, scale_f32, lsb_exp, map_out, b.partials);
+            b.status, W, H, D, TX, TY, ntiles, scale_f32, lsb_exp, map_out, b.partials, top16);
   BM_LAUNCH("stats_final", k_stats_final, 1, 256, 0, s, b.partials, kStatsBlocks, b.stats);
   BM_HIP(hipGetLastError());
 }

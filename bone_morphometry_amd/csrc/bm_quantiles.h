@@ -16,7 +16,9 @@ struct QuantileResult {
   double median = 0, p10 = 0, p25 = 0, p75 = 0, p90 = 0;
 };
 
+// top16 (device, may be null): counts of the values' top 16 bits over
+// [kTopBase, kTopBase + kTopBins), from the clean-up kernel.
 QuantileResult device_quantiles(Engine& e, const float* d_map, uint64_t n, uint64_t count,
-                                float scale_f32, hipStream_t s);
+                                float scale_f32, hipStream_t s, const uint32_t* top16 = nullptr);
 
 }  // namespace bm

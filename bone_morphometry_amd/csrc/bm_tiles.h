@@ -23,6 +23,7 @@ constexpr int kBuckets = 1 << kBucketBits;  // radius buckets of the counting so
 constexpr int kFillCap = 1024;              // LDS candidates per round of the fill
 constexpr int kMaxNeighbours = 512;         // tiles in reach kept per tile
 constexpr uint32_t kGatherMaxRadius = 128;  // beyond this the scatter fill is used
+constexpr int kDiamLds = 2048;              // diameters the clean-up keeps in LDS
 
 // Radius thresholds of the prefix counts: (2j + 1)^2.
 inline __host__ __device__ uint32_t kPrefR2(int j) { return (uint32_t)(2 * j + 1) * (2 * j + 1); }

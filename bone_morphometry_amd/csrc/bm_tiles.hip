@@ -106,7 +106,8 @@ k_ridges_tiled(const uint32_t* __restrict__ d2, uint32_t W, uint32_t H, uint32_t
   __shared__ uint32_t s_cnt[kBuckets];
   __shared__ uint32_t s_tmp[16];
   if (!st->has_site) return;
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_tile(blockIdx.x, TX * TY * TZ);
+  if (t >= TX * TY * TZ) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
@@ -266,7 +267,8 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
   unsigned long long clk_p3 = 0, clk_p4 = 0;
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = xcd_tile(blockIdx.x, TX * TY * TZ);
+  if (t >= TX * TY * TZ) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   const int hx = min(ox + 7, (int)W - 1), hy = min(oy + 7, (int)H - 1), hz = min(oz + 7, (int)D - 1);
@@ -697,71 +699,131 @@ k_cleanup_tiled(const uint32_t* __restrict__ M, const unsigned long long* __rest
                 const float* __restrict__ diam, const PhaseStatus* __restrict__ st, uint32_t W,
                 uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
                 float scale_f32, int lsb_exp, float* __restrict__ map_out,
-                StatsPartial* __restrict__ partials) {
-  __shared__ uint32_t s_m[12 * 12 * 12];
-  __shared__ uint8_t s_mark[10 * 10 * 10];
+                StatsPartial* __restrict__ partials, uint32_t* __restrict__ top16) {
+  __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
+  __shared__ uint32_t s_zero[12 * 12];    // per (z, y) row: bit x set where M == 0
+  __shared__ uint32_t s_mark[10 * 10];    // per row of the 1-halo box: surface bits
+  __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
+  __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
+  if (top16)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_top[i] = 0;
   unsigned long long cnt = 0;
   u128 s = 0, s2 = 0;
   uint32_t mxb = 0, flags = 0;
   const bool valid_phase = st->has_site != 0;
-  for (uint32_t t = blockIdx.x; t < ntiles && valid_phase; t += gridDim.x) {
+  const uint32_t nd = valid_phase ? min(st->max_d2 + 1, (uint32_t)kDiamLds) : 0u;
+  for (uint32_t i = threadIdx.x; i < nd; i += blockDim.x) s_diam[i] = diam[i];
+  auto dia = [&](uint32_t m) { return m < (uint32_t)kDiamLds ? s_diam[m] : diam[m]; };
+  const uint32_t li = threadIdx.x;
+  const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
+  // The 12^3 halo region of a tile, 4 values per thread (1728 <= 4 * 512),
+  // is loaded one tile ahead into registers to hide the HBM latency.
+  uint32_t pre[4];
+  unsigned long long pre_sel = 0;
+  auto fetch = [&](uint32_t t) {
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
-    const unsigned long long selw = masks[(size_t)t * 16 + (threadIdx.x >> 6)];
-    const unsigned long long any = __ballot(selw != 0);  // waves see their own plane
-    (void)any;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < 1728; i += blockDim.x) {
-      const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
-      const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
+    pre_sel = masks[(size_t)t * 16 + (threadIdx.x >> 6)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t i = threadIdx.x + 512u * k;
       uint32_t v = kOut;
-      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
-        v = M[((size_t)gz * H + gy) * W + gx];
-      s_m[i] = v;
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
-      const int hx = (int)(i % 10), hy = (int)((i / 10) % 10), hz = (int)(i / 100);
-      const int c = (hx + 1) + 12 * (hy + 1) + 144 * (hz + 1);
-      const uint32_t v = s_m[c];
-      bool mark = false;
-      if (v != 0 && v != kOut) {
-        for (int dz = -1; dz <= 1 && !mark; ++dz)
-          for (int dy = -1; dy <= 1 && !mark; ++dy)
-            for (int dx = -1; dx <= 1; ++dx)
-              if (s_m[c + dx + 12 * dy + 144 * dz] == 0) {
-                mark = true;
-                break;
-              }
+      if (i < 1728) {
+        const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
+        const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
+          v = M[((size_t)gz * H + gy) * W + gx];
       }
-      s_mark[i] = mark;
+      pre[k] = v;
+    }
+  };
+  // Block b walks tiles j = b/8, b/8 + G/8, ... of its XCD's contiguous range
+  // (G blocks, a multiple of 8), so neighbouring tiles share that XCD's L2.
+  const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+  const uint32_t xbase = (blockIdx.x % kXcds) * per, xend = min(xbase + per, ntiles);
+  const uint32_t jstep = gridDim.x / kXcds;
+  const uint32_t t0 = xbase + blockIdx.x / kXcds;
+  if (valid_phase && t0 < xend) fetch(t0);
+  for (uint32_t t = t0; t < xend && valid_phase; t += jstep) {
+    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    const unsigned long long selw = pre_sel;
+    const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
+    const size_t gidx = ((size_t)(oz + lz) * H + (oy + ly)) * W + (ox + lx);
+    const bool any_sel = __syncthreads_or(selw != 0);
+    if (any_sel) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (threadIdx.x + 512u * k < 1728) s_m[threadIdx.x + 512u * k] = pre[k];
+    }
+    if (t + jstep < xend) fetch(t + jstep);  // next tile, in flight meanwhile
+    if (!any_sel) {  // no selected voxel in this tile
+      if (map_out && valid) map_out[gidx] = 0.0f;
+      continue;
     }
     __syncthreads();
-    const uint32_t li = threadIdx.x;
-    const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
+    if (threadIdx.x < 144) {  // zero mask of a 12-voxel row
+      uint32_t z = 0;
+      for (int x = 0; x < 12; ++x) z |= (s_m[threadIdx.x * 12 + x] == 0 ? 1u : 0u) << x;
+      s_zero[threadIdx.x] = z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 100) {
+      // surface (pass 1, src/local_thickness.rs:214-238): M != 0 with a zero
+      // among the 26 neighbours; the neighbourhood test is a 3x3x3 dilation of
+      // the zero bits (out-of-array voxels hold kOut, never zero)
+      const int ry = (int)(threadIdx.x % 10) + 1, rz = (int)(threadIdx.x / 10) + 1;
+      uint32_t dil = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const uint32_t z = s_zero[(rz + dz) * 12 + (ry + dy)];
+          dil |= z | (z << 1) | (z >> 1);
+        }
+      uint32_t nonzero = 0;
+      for (int x = 1; x <= 10; ++x) {
+        const uint32_t v = s_m[(rz * 12 + ry) * 12 + x];
+        nonzero |= (v != 0 && v != kOut ? 1u : 0u) << x;
+      }
+      s_mark[threadIdx.x] = (dil & nonzero) >> 1;  // bit hx (0..9) of the 1-halo box
+    }
+    __syncthreads();
     const bool sel = (selw >> (li & 63)) & 1ull;
     float tv = 0.0f;
     if (sel) {
       const int c12 = (lx + 2) + 12 * (ly + 2) + 144 * (lz + 2);
-      const int c10 = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
       const uint32_t m = s_m[c12];
-      if (!s_mark[c10]) {
-        tv = diam[m];
+      auto marked = [&](int dx, int dy, int dz) {
+        return (s_mark[(lz + 1 + dz) * 10 + (ly + 1 + dy)] >> (lx + 1 + dx)) & 1u;
+      };
+      if (!marked(0, 0, 0)) {
+        tv = dia(m);
       } else {
+        // pass 2 (src/local_thickness.rs:243-277): f32 mean of the positive,
+        // non-surface neighbours in NEIGHBORS_26 order
         float sum = 0.0f;
         uint32_t count = 0;
         for (int k = 0; k < 26; ++k) {
           const int dx = c_nb26[k][0], dy = c_nb26[k][1], dz = c_nb26[k][2];
           const uint32_t mn = s_m[c12 + dx + 12 * dy + 144 * dz];
-          if (mn == 0 || mn == kOut || s_mark[c10 + dx + 10 * dy + 100 * dz]) continue;
-          sum += diam[mn];
+          if (mn == 0 || mn == kOut || marked(dx, dy, dz)) continue;
+          sum += dia(mn);
           count += 1;
         }
-        tv = count > 0 ? sum / (float)count : diam[m];
+        tv = count > 0 ? sum / (float)count : dia(m);
       }
       if (tv > 0.0f && tv < INFINITY) acc_value(tv * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
     }
-    if (map_out && ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D)
-      map_out[((size_t)(oz + lz) * H + (oy + ly)) * W + (ox + lx)] = tv;
+    if (top16) {
+      const uint32_t bin = (__float_as_uint(tv) >> 16) - kTopBase;
+      const bool in = tv > 0.0f && bin < (uint32_t)kTopBins;
+      if (tv > 0.0f && !in) flags |= 2u;  // outside the histogram's range
+      wave_hist_add(s_top, in, in ? bin : 0u);
+    }
+    if (map_out && valid) map_out[gidx] = tv;
+    __syncthreads();  // s_m, s_mark are rewritten by the next tile
+  }
+  if (top16) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x)
+      if (s_top[i]) atomicAdd(&top16[i], s_top[i]);
   }
   block_reduce_store(cnt, s, s2, mxb, flags, partials + blockIdx.x);
 }
