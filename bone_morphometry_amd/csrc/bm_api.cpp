@@ -497,7 +497,7 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
       fprintf(stderr,
               "[fill] waves %llu tests %llu scanned %llu rounds %llu | clocks(10ns) collect %llu "
               "place %llu scan %llu total %llu | masks %llu nbrs %llu chunkscan %llu loads %llu "
-              "rest %llu | kept %llu chunks %llu regtiles %llu overflow %u\n",
+              "rest %llu | kept %llu chunks %llu unstaged %llu overflow %u\n",
               d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[13],
               d[14], d[15], so.status.overflow_tiles);
     }

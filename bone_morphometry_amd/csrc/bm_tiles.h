@@ -21,6 +21,8 @@ struct TileMeta {
 constexpr int kBucketBits = 9;
 constexpr int kBuckets = 1 << kBucketBits;  // radius buckets of the counting sorts
 constexpr int kFillCap = 1024;              // LDS candidates per round of the fill
+constexpr int kStageCap = 2048;             // candidates of a tile staged in LDS
+constexpr int kChunkMap = 2048;             // 64-entry list chunks mapped to their tile in LDS
 constexpr int kMaxNeighbours = 512;         // tiles in reach kept per tile
 constexpr uint32_t kGatherMaxRadius = 128;  // beyond this the scatter fill is used
 constexpr int kDiamLds = 2048;              // diameters the clean-up keeps in LDS

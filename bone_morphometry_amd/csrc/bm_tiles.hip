@@ -257,12 +257,13 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   __shared__ uint32_t s_pos[kFillCap], s_r2[kFillCap];  // one round, sorted by bucket
   __shared__ uint32_t s_cnt[kBuckets];                  // histogram, then cursors
   __shared__ uint32_t s_start[kBuckets + 1];            // bucket starts over all rounds
-  __shared__ uint32_t s_nbr[kMaxNeighbours];            // neighbour tile in reach
-  __shared__ uint32_t s_len[kMaxNeighbours];            // its useful list prefix
+  __shared__ uint16_t s_len[kMaxNeighbours];            // useful list prefix of a tile in reach
   __shared__ uint32_t s_org[kMaxNeighbours];            // its tile coordinates, packed
+  __shared__ uint16_t s_cmap[kChunkMap];                // chunk -> tile in reach
   __shared__ uint32_t s_off[kBuckets];                  // chunk (or entry) offsets
+  __shared__ uint32_t s_spos[kStageCap], s_sr2[kStageCap];  // candidates in list order
   __shared__ uint32_t s_tmp[16];
-  __shared__ uint32_t s_nn, s_over, s_pending;
+  __shared__ uint32_t s_nn, s_over, s_pending, s_nst;
   __shared__ unsigned long long s_tgt[8];
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
@@ -280,6 +281,7 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   if (threadIdx.x == 0) {
     s_nn = 0;
     s_over = 0;
+    s_nst = 0;
   }
   for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) {
     s_cnt[i] = 0;
@@ -332,8 +334,7 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
     if (len == 0) continue;
     const uint32_t slot = atomicAdd(&s_nn, 1u);
     if (slot < kMaxNeighbours) {
-      s_nbr[slot] = u;
-      s_len[slot] = len;
+      s_len[slot] = (uint16_t)len;
       s_org[slot] = (uint32_t)ux | ((uint32_t)uy << 10) | ((uint32_t)uz << 20);
     } else {
       s_over = 1;
@@ -352,6 +353,11 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = (s_len[j] + 63) >> 6;
   __syncthreads();
   const uint32_t nchunks = block_exclusive_scan_buckets(s_off, s_tmp);  // s_off: chunk offsets
+  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
+    const uint32_t c0 = s_off[j], c1 = min(c0 + ((s_len[j] + 63u) >> 6), (uint32_t)kChunkMap);
+    for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = (uint16_t)j;
+  }
+  __syncthreads();
   const unsigned long long tc = diag ? wall_clock64() : 0ull;
   unsigned long long td = tc;
   const uint32_t gshift = bucket_shift(max_r2);
@@ -359,7 +365,6 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   // Chunks q = wv, wv + 8, ... of the flattened (neighbour, 64 entries) range;
   // one chunk is a sorted run of one list.  Read four chunks per trip.
   auto chunk_pass = [&](auto&& use) {
-    int cur = 0;  // neighbour slot of chunk q (q grows)
     for (uint32_t q0 = wv; q0 < nchunks; q0 += 32) {
       uint32_t ent[4], org[4];
 #pragma unroll
@@ -368,10 +373,23 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
         ent[k] = 0;
         org[k] = 0;
         if (q < nchunks) {
-          while (cur + 1 < (int)nn && s_off[cur + 1] <= q) ++cur;
+          uint32_t cur;
+          if (q < (uint32_t)kChunkMap) {
+            cur = s_cmap[q];
+          } else {  // last slot with s_off <= q (offsets increase strictly)
+            uint32_t lo = 0, hi = nn - 1;
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1) >> 1;
+              if (s_off[mid] <= q) lo = mid;
+              else hi = mid - 1;
+            }
+            cur = lo;
+          }
           const uint32_t e = (q - s_off[cur]) * 64 + lane;
-          org[k] = s_org[cur];
-          if (e < s_len[cur]) ent[k] = lists[(size_t)s_nbr[cur] * 512 + e];
+          const uint32_t o = s_org[cur];
+          org[k] = o;
+          const uint32_t u = ((o >> 20) * TY + ((o >> 10) & 1023)) * TX + (o & 1023);
+          if (e < s_len[cur]) ent[k] = lists[(size_t)u * 512 + e];
         }
       }
 #pragma unroll
@@ -385,16 +403,38 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
       }
     }
   };
-  // P2: histogram by bucket, one LDS atomic per run of equal buckets.
+  // P2: histogram by bucket, one LDS atomic per run of equal buckets; the
+  // candidates are also staged in LDS (one slot reservation per chunk) so that
+  // the placement passes need not read the lists again.
   chunk_pass([&](bool keep, uint32_t pos, uint32_t r2) {
-    (void)pos;
     const uint32_t b = keep ? gtop - (r2 >> gshift) : 0u;
     uint32_t rank, len;
     bool head;
     run_of_bucket(keep, b, lane, rank, len, head);
     if (head) atomicAdd(&s_cnt[b], len);
+    const unsigned long long km = __ballot(keep);
+    if (km) {
+      uint32_t k0 = 0;
+      if (lane == 0) k0 = atomicAdd(&s_nst, (uint32_t)__popcll(km));
+      k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k0);
+      const uint32_t k = k0 + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+      if (keep && k < (uint32_t)kStageCap) {
+        s_spos[k] = pos;
+        s_sr2[k] = r2;
+      }
+    }
   });
   __syncthreads();
+  const uint32_t nst = s_nst;
+  const bool staged = nst <= (uint32_t)kStageCap;  // else the placement re-reads the lists
+  // Staged candidates, 64 per wave and trip (the trip count is uniform over a wave).
+  auto staged_pass = [&](auto&& use) {
+    for (uint32_t i0 = wv * 64; i0 < nst; i0 += blockDim.x) {
+      const uint32_t i = i0 + lane;
+      const bool keep = i < nst;
+      use(keep, keep ? s_spos[i] : 0u, keep ? s_sr2[i] : 0u);
+    }
+  };
   const uint32_t n = block_exclusive_scan_buckets(s_cnt, s_tmp);
   for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_start[i] = s_cnt[i];
   if (threadIdx.x == 0) s_start[kBuckets] = n;
@@ -429,7 +469,7 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
     const uint32_t base = s_start[b0], cnt_round = s_start[b1] - base;
     const unsigned long long c3 = diag ? wall_clock64() : 0ull;
     // P3: place this round's candidates at their sorted positions.
-    chunk_pass([&](bool keep0, uint32_t pos, uint32_t r2) {
+    auto place = [&](bool keep0, uint32_t pos, uint32_t r2) {
       const uint32_t b = keep0 ? gtop - (r2 >> gshift) : 0u;
       const bool keep = keep0 && b >= b0 && b < b1;
       uint32_t rank, len;
@@ -446,7 +486,9 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
         s_pos[k] = pos;
         s_r2[k] = r2;
       }
-    });
+    };
+    if (staged) staged_pass(place);
+    else chunk_pass(place);
     __syncthreads();
     const unsigned long long c4 = diag ? wall_clock64() : 0ull;
     clk_p3 += c4 - c3;
@@ -561,7 +603,7 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
     slot[12] = clk1 - td;
     slot[13] = wv == 0 ? n : 0;
     slot[14] = wv == 0 ? nchunks : 0;
-    slot[15] = 0;
+    slot[15] = wv == 0 && !staged ? 1 : 0;
   }
 }
 
