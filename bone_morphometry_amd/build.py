@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["bm_kernels.hip", "bm_tiles.hip", "bm_engine.hip", "bm_quantiles.hip",
-               "bm_batch.hip", "bm_api.cpp"]
+               "bm_tune.cpp", "bm_serve.cpp", "bm_api.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

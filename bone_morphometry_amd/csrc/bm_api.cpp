@@ -22,7 +22,6 @@
 
 #include "../../include/bm.h"
 #include "bm_engine.h"
-#include "bm_quantiles.h"
 
 namespace bm {
 
@@ -97,7 +96,19 @@ static Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
   return dims;
 }
 
-static Tuning default_tuning() { return Tuning(); }
+// Kernel tunables for a volume: the autotuner's choice for its shape class,
+// with test overrides from the environment (BM_FILL_CAP: fill candidates per
+// round).
+static Tuning tuning_for(const Dims& dims) {
+  Tuning t = tuned_for(dims);
+  if (const char* env = std::getenv("BM_FILL_CAP")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 1 && v <= 1024) t.fill_cap = (uint32_t)v;
+  }
+  return t;
+}
+
+static bool fill_diag_requested() { return std::getenv("BM_FILL_DIAG") != nullptr; }
 
 // f64 in the shortest form that reads back exactly, like Rust's {:?}.
 static std::string rust_f64(double v) {
@@ -133,18 +144,24 @@ static void fail_all_selected(Phase p) {
 }
 
 // ---------------------------------------------------------------------------
-// euclidean_distance_transform_squared (src/edt.rs:57-156), device resident.
-static void edt_device(Engine& e, const uint8_t* d_data, const Dims& dims, Phase p,
-                       uint32_t* d_out, hipStream_t s) {
-  PhaseBuffers b = e.buffers_for(dims);
-  PhaseSource src;
-  src.data = d_data;
-  src.want_nonzero = p == Phase::Foreground;
-  e.run_edt(src, dims, b, d_out, s, default_tuning());
-  PhaseStatus st;
-  BM_HIP(hipMemcpyAsync(&st, b.status, sizeof st, hipMemcpyDeviceToHost, s));
-  BM_HIP(hipStreamSynchronize(s));
-  if (!st.has_site) fail_all_selected(p);
+// euclidean_distance_transform_squared (src/edt.rs:57-156), device resident:
+// a plan of one item without the thickness stages.  d_out may be null (the
+// result stays in the plan's buffer).
+static const Plan& edt_device(Engine& e, const uint8_t* d_data, const Dims& dims, Phase p,
+                              uint32_t* d_out, hipStream_t s) {
+  ItemSpec sp;
+  sp.src.data = d_data;
+  sp.src.want_nonzero = p == Phase::Foreground;
+  sp.dims = dims;
+  sp.tune = tuning_for(dims);
+  sp.d2_out = d_out;
+  sp.thickness = false;
+  Plan& pl = e.plain_plan();
+  pl.build({sp}, s);
+  pl.launch(s);
+  pl.fetch(s);
+  if (!pl.result(0).status.has_site) fail_all_selected(p);
+  return pl;
 }
 
 void euclidean_distance_transform_squared(const BinaryVolume& v, Phase p, uint32_t* out) {
@@ -155,34 +172,47 @@ void euclidean_distance_transform_squared(const BinaryVolume& v, Phase p, uint32
   const uint64_t n = dims.n();
   uint8_t* d_in = static_cast<uint8_t*>(e.scratch_bytes(n, 0));
   BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
-  PhaseBuffers b = e.buffers_for(dims);
-  edt_device(e, d_in, dims, p, b.d2, s);
-  BM_HIP(hipMemcpyAsync(out, b.d2, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  const Plan& pl = edt_device(e, d_in, dims, p, nullptr, s);
+  BM_HIP(hipMemcpyAsync(out, pl.desc(0).d2, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   BM_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
-// local_thickness (src/local_thickness.rs:16-44), device resident.
+// local_thickness (src/local_thickness.rs:16-44), device resident: a plan of
+// one item.  With stages, the ridge flags and the propagated map are written
+// too and the status is read back after them (ridge count).
 struct StageOutputs {
   uint8_t* ridge_flags = nullptr;
   uint32_t* propagated = nullptr;
   PhaseStatus status{};
 };
 
-static void local_thickness_device(Engine& e, const uint8_t* d_data, const Dims& dims, Phase p,
-                                   float* d_out, hipStream_t s, StageOutputs* stages = nullptr) {
-  PhaseBuffers b = e.buffers_for(dims);
-  PhaseSource src;
-  src.data = d_data;
-  src.want_nonzero = p == Phase::Foreground;
-  e.run_edt(src, dims, b, b.d2, s, default_tuning());
-  e.run_thickness(dims, b, d_out, 1.0f, stats_lsb_exp(1.0f), s,
-                  stages ? stages->ridge_flags : nullptr, stages ? stages->propagated : nullptr);
-  PhaseStatus st;
-  BM_HIP(hipMemcpyAsync(&st, b.status, sizeof st, hipMemcpyDeviceToHost, s));
-  BM_HIP(hipStreamSynchronize(s));
-  if (!st.has_site) fail_all_selected(p);
-  if (stages) stages->status = st;
+static const Plan& local_thickness_device(Engine& e, const uint8_t* d_data, const Dims& dims,
+                                          Phase p, float* d_out, hipStream_t s,
+                                          StageOutputs* stages = nullptr) {
+  ItemSpec sp;
+  sp.src.data = d_data;
+  sp.src.want_nonzero = p == Phase::Foreground;
+  sp.dims = dims;
+  sp.tune = tuning_for(dims);
+  sp.map_out = d_out;
+  sp.want_map = true;
+  sp.thickness = true;
+  sp.diag = stages && fill_diag_requested();
+  sp.scale = 1.0f;
+  sp.lsb = stats_lsb_exp(1.0f);
+  Plan& pl = e.plain_plan();
+  pl.build({sp}, s);
+  pl.launch(s);
+  if (stages) e.stage_outputs(pl, stages->ridge_flags, stages->propagated, s);
+  pl.fetch(s);
+  if (!pl.result(0).status.has_site) fail_all_selected(p);
+  if (stages) {
+    BM_HIP(hipMemcpyAsync(&stages->status, pl.desc(0).status, sizeof(PhaseStatus),
+                          hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+  }
+  return pl;
 }
 
 void local_thickness(const BinaryVolume& v, Phase p, float* out) {
@@ -229,58 +259,82 @@ static bm_distribution_stats distribution_stats(const StatsResult& r, const Quan
 // bone and ROI.  Both phases are derived on the fly from (bone, roi)
 // (src/volume.rs:101); the map of a phase is only materialised when quantiles
 // are requested.
-static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_roi,
-                           const Dims& dims, const double spacing[3], int mode,
-                           bm_thickness_spacing* out, hipStream_t s) {
+static void check_spacing(const double spacing[3]) {
   for (int i = 0; i < 3; ++i)
     if (!std::isfinite(spacing[i]) || spacing[i] <= 0.0)
       throw Error("voxel spacings must be finite and positive; got " + spacing_list(spacing));
+}
+
+// The two items (bone, marrow) of one analyze call.
+static void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims,
+                          const double spacing[3], int mode, std::vector<ItemSpec>& out) {
+  const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
+  const float scale_f32 = (float)spacing[0];
+  for (int ph = 0; ph < 2; ++ph) {
+    ItemSpec sp;
+    sp.src.data = d_bone;
+    sp.src.roi = d_roi;
+    sp.src.want_nonzero = ph == 0 ? 1u : 0u;
+    sp.dims = dims;
+    sp.tune = tuning_for(dims);
+    sp.thickness = true;
+    sp.quantiles = quantiles;
+    sp.want_map = quantiles;
+    sp.keep_d2 = false;
+    sp.scale = scale_f32;
+    sp.lsb = stats_lsb_exp(scale_f32);
+    out.push_back(sp);
+  }
+}
+
+// Results of analyze from the two items' results, with the error precedence
+// of src/metrics.rs:90-104.
+static void analyze_finish(const ItemResult& bone, const ItemResult& marrow, uint64_t n,
+                           const double spacing[3], int mode, bm_thickness_spacing* out) {
   const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
   const float scale_f32 = (float)spacing[0];
   const int lsb = stats_lsb_exp(scale_f32);
-  const uint64_t n = dims.n();
-  PhaseBuffers b = e.buffers_for(dims);
-  float* d_map = quantiles ? static_cast<float*>(e.scratch_bytes(n * sizeof(float), 1)) : nullptr;
-  PhaseStatus st[2];
-  StatsPartial sp[2];
-  QuantileResult qr[2];
-  for (int ph = 0; ph < 2; ++ph) {
-    PhaseSource src;
-    src.data = d_bone;
-    src.roi = d_roi;
-    src.want_nonzero = ph == 0 ? 1u : 0u;
-    e.run_edt(src, dims, b, b.d2, s, default_tuning());
-    e.run_thickness(dims, b, d_map, scale_f32, lsb, s);
-    BM_HIP(hipMemcpyAsync(&st[ph], b.status, sizeof(PhaseStatus), hipMemcpyDeviceToHost, s));
-    BM_HIP(hipMemcpyAsync(&sp[ph], b.stats, sizeof(StatsPartial), hipMemcpyDeviceToHost, s));
-    if (quantiles) {
-      BM_HIP(hipStreamSynchronize(s));
-      if (sp[ph].count > 0)
-        qr[ph] = device_quantiles(e, d_map, n, sp[ph].count, scale_f32, s, b.top16);
-    }
-  }
-  BM_HIP(hipStreamSynchronize(s));
-  // Error precedence of src/metrics.rs:90-104.
-  const uint64_t bone_voxels = st[0].selected, marrow_voxels = st[1].selected;
+  const uint64_t bone_voxels = bone.status.selected, marrow_voxels = marrow.status.selected;
   const uint64_t roi_voxels = bone_voxels + marrow_voxels;
   if (roi_voxels == 0) throw Error("ROI mask contains no included voxels");
   if (mode == BM_MODE_BONEJ && roi_voxels == n)
     throw Error("BoneJ compatibility mode requires an ROI mask with both inside and outside voxels");
   if (bone_voxels == 0) throw Error("ROI contains no bone voxels");
   if (marrow_voxels == 0) throw Error("ROI contains no marrow voxels");
-  out->thickness = distribution_stats(finish_stats(sp[0], lsb), quantiles ? &qr[0] : nullptr);
-  out->spacing = distribution_stats(finish_stats(sp[1], lsb), quantiles ? &qr[1] : nullptr);
+  if ((bone.status.error | marrow.status.error) != 0)
+    throw DeviceError("squared distance beyond the tile lists' range");
+  QuantileResult q[2];
+  if (quantiles) {
+    q[0] = quantiles_from(bone.sel, bone.stats.count, scale_f32);
+    q[1] = quantiles_from(marrow.sel, marrow.stats.count, scale_f32);
+  }
+  out->thickness = distribution_stats(finish_stats(bone.stats, lsb), quantiles ? &q[0] : nullptr);
+  out->spacing = distribution_stats(finish_stats(marrow.stats, lsb), quantiles ? &q[1] : nullptr);
   const double voxel_volume = spacing[0] * spacing[1] * spacing[2];
   out->bone_volume = (double)bone_voxels * voxel_volume;
   out->total_volume = (double)roi_voxels * voxel_volume;
   out->bone_volume_fraction = out->bone_volume / out->total_volume;
 }
 
+// Thickness/spacing part of analyze (src/metrics.rs:64-132) on device-resident
+// bone and ROI: the bone and marrow phases (src/volume.rs:101) are two items
+// of one plan, so both run in the same launches.
+static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_roi,
+                           const Dims& dims, const double spacing[3], int mode,
+                           bm_thickness_spacing* out, hipStream_t s) {
+  check_spacing(spacing);
+  std::vector<ItemSpec> specs;
+  analyze_items(d_bone, d_roi, dims, spacing, mode, specs);
+  Plan& pl = e.plain_plan();
+  pl.build(specs, s);
+  pl.launch(s);
+  pl.fetch(s);
+  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, out);
+}
+
 void analyze_thickness_spacing(const BinaryVolume& bone, const BinaryVolume& roi,
                                const double spacing[3], int mode, bm_thickness_spacing* out) {
-  for (int i = 0; i < 3; ++i)
-    if (!std::isfinite(spacing[i]) || spacing[i] <= 0.0)
-      throw Error("voxel spacings must be finite and positive; got " + spacing_list(spacing));
+  check_spacing(spacing);
   if (!bone.same_shape(roi)) throw Error(shape_mismatch(bone, roi));
   const Dims dims = device_dims(bone.width, bone.height, bone.depth);
   Engine& e = engine_for_device(current_device());
@@ -483,9 +537,9 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
     StageOutputs so;
     so.ridge_flags = d_flags;
     so.propagated = d_prop;
-    local_thickness_device(e, d_in, dims, p, d_map, s, &so);
-    PhaseBuffers b = e.buffers_for(dims);
-    if (squared_edt) BM_HIP(hipMemcpyAsync(squared_edt, b.d2, n * 4, hipMemcpyDeviceToHost, s));
+    const Plan& pl = local_thickness_device(e, d_in, dims, p, d_map, s, &so);
+    if (squared_edt)
+      BM_HIP(hipMemcpyAsync(squared_edt, pl.desc(0).d2, n * 4, hipMemcpyDeviceToHost, s));
     if (ridge_flags) BM_HIP(hipMemcpyAsync(ridge_flags, d_flags, n, hipMemcpyDeviceToHost, s));
     if (propagated) BM_HIP(hipMemcpyAsync(propagated, d_prop, n * 4, hipMemcpyDeviceToHost, s));
     if (thickness) BM_HIP(hipMemcpyAsync(thickness, d_map, n * 4, hipMemcpyDeviceToHost, s));

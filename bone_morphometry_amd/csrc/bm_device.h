@@ -38,22 +38,6 @@ __device__ __forceinline__ uint32_t ceil_sqrt_u32(uint32_t v) {
   return f * f == v ? f : f + 1;
 }
 
-// scan_cube (src/local_thickness.rs:75-102) for one r^2.
-__device__ inline uint32_t scan_cube(uint32_t dx, uint32_t dy, uint32_t dz, uint32_t r2) {
-  const uint32_t radius = isqrt_u32(r2) + 1;
-  uint32_t maximum = 0;
-  for (uint32_t k = 0; k <= radius; ++k) {
-    const uint32_t sk = k * k, dk = (k + dz) * (k + dz);
-    for (uint32_t j = 0; j <= radius; ++j) {
-      const uint32_t skj = sk + j * j;
-      if (skj > r2) break;  // j*j grows with j: the rest of the row is outside too
-      const uint32_t ip = isqrt_u32(r2 - skj) + dx;
-      maximum = max(maximum, dk + (j + dy) * (j + dy) + ip * ip);
-    }
-  }
-  return maximum;
-}
-
 // Diameter (2 * sqrt_f64(m)) as f32 (src/local_thickness.rs:206).  For
 // m < 2^24, (float)m is exact and the correctly rounded f32 sqrt scaled by 2
 // equals RN32(2 sqrt(m)); the reference's RN32(RN64(2 sqrt(m))) is the same
@@ -158,11 +142,26 @@ __device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint3
 // b % 8), each XCD with its own L2.  This maps block b to a tile so that each
 // XCD walks one contiguous range of tiles: neighbouring tiles, whose halos and
 // ridge lists overlap, then share an L2.
-constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
   const uint32_t per = (ntiles + kXcds - 1) / kXcds;
   const uint32_t t = (b % kXcds) * per + b / kXcds;
   return t;  // may be >= ntiles for the last XCD's tail: callers skip those
+}
+
+// Item of block b in a stage: the last item whose start is <= b (items with
+// no block in the stage share their start with the next one).  starts has
+// nitems + 1 entries; the result and the block of the item are uniform over
+// the block, so the search runs on scalar loads.
+__device__ __forceinline__ uint32_t item_of(const uint32_t* __restrict__ starts, uint32_t nitems,
+                                            uint32_t b, uint32_t& vb) {
+  uint32_t lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  vb = b - starts[lo];
+  return lo;
 }
 
 }  // namespace bm

@@ -1,9 +1,14 @@
 // bm_engine.h -- device engine for the exact squared EDT and local thickness.
 //
-// Everything here is stream-ordered device work on one GPU.  The host-facing
-// mirror of the reference crate (BinaryVolume, Phase, euclidean_distance_
-// transform_squared, local_thickness, analyze) lives in bm_api.cpp and calls
-// this engine; the C-ABI in include/bm.h is built on top of both.
+// Everything here is stream-ordered device work on one GPU.  The unit of work
+// is an *item*: one phase (bone or marrow, or a Phase of a BinaryVolume) of one
+// volume.  A Plan holds the device buffers and descriptors of a list of items
+// and launches each stage of the pipeline once for all of them; the kernels
+// find their item from the block index.  The one-volume calls of the
+// reference's API are plans of one item (EDT, local thickness) or two items
+// (the bone and marrow phases of analyze); the batched and serving forms are
+// plans of many items.  The host-facing mirror of the reference crate lives in
+// bm_api.cpp, the C-ABI in include/bm.h.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,11 +55,11 @@ struct Dims {
   }
 };
 
-// Per-phase device status, written by the kernels of one phase.
+// Per-item device status, written by the kernels of the item.
 struct PhaseStatus {
   unsigned long long selected;     // voxels of the phase (X pass)
-  unsigned long long ridge_count;  // distance ridges found
-  unsigned long long fill_visits;  // voxel-candidate tests of the fill
+  unsigned long long ridge_count;  // distance ridges found (stage outputs only)
+  unsigned long long fill_visits;  // unused (kept for the layout)
   uint32_t has_site;               // any non-selected voxel (X pass)
   uint32_t max_d2;                 // maximum squared distance (Z pass)
   uint32_t error;                  // bit 0: r^2 too large for the tile lists
@@ -75,7 +80,7 @@ struct StatsPartial {
   unsigned long long sum_lo, sum_hi;
   unsigned long long sq_lo, sq_hi;
   uint32_t max_bits;  // f32 bits of the largest value (positive floats order as u32)
-  uint32_t flags;
+  uint32_t flags;     // bit 0: fixed-point range exceeded; bit 1: value outside top16 range
 };
 
 struct StatsResult {
@@ -84,7 +89,18 @@ struct StatsResult {
   uint32_t flags = 0;
 };
 
-// Tunables of the kernels (chosen by the autotuner for the batched path).
+// Radix select of the order statistics behind the five quantiles of
+// nonzero_stats: rank[2q] and rank[2q+1] are the lower and upper ranks of
+// quantile q (src/metrics.rs:271-291), computed on the device from the count.
+constexpr int kMaxRanks = 10;
+struct SelectState {
+  uint32_t prefix[kMaxRanks];          // value bits found so far (the value at the end)
+  unsigned long long rank[kMaxRanks];  // rank within the values sharing prefix
+  uint32_t nranks;
+  uint32_t pad;
+};
+
+// Tunables of the kernels (chosen per shape class by the autotuner).
 // A Y-pass tile holds up to y_rows rows of 64 columns (u16) in LDS: the whole
 // column when H <= y_rows, otherwise a chunk of y_rows - 2*y_halo rows plus a
 // halo on each side (candidates beyond the halo are read from HBM).  The Z
@@ -99,29 +115,7 @@ struct Tuning {
   uint32_t fill_cap = 0;  // 0: the kernel's capacity
 };
 
-// Device buffers for one phase of one volume.
-struct TileMeta;
-struct PhaseBuffers {
-  uint16_t* dx = nullptr;       // X pass output (N u16)
-  uint32_t* h = nullptr;        // Y pass output, then the propagated map (N u32)
-  uint32_t* d2 = nullptr;       // squared EDT (N u32)
-  uint32_t* lists = nullptr;    // per-tile sorted ridge lists (512 per tile)
-  TileMeta* meta = nullptr;     // per-tile list header
-  unsigned long long* masks = nullptr;  // per tile: 8 selected + 8 target words
-  uint32_t* overflow = nullptr; // [count, tile ids...]
-  uint32_t tiles[3] = {0, 0, 0};
-  uint32_t* tmpl = nullptr;     // 3 x (bound+1) templates
-  float* diam = nullptr;        // (bound+1) diameters 2*sqrt(m)
-  PhaseStatus* status = nullptr;
-  BlockPartial* xpart = nullptr;     // kEdtXBlocks
-  uint32_t* zmax = nullptr;          // one per Z-pass block
-  StatsPartial* partials = nullptr;  // kStatsBlocks
-  StatsPartial* stats = nullptr;     // final record
-  uint32_t* top16 = nullptr;         // top-16-bit value histogram (quantiles)
-  uint64_t bound = 0;
-};
-
-constexpr int kStatsBlocks = 1024;
+constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
 // Top-16-bit histogram of thickness values: f32 bit patterns 0x4000xxxx to
 // 0x47FFxxxx, i.e. values in [2, 65536).  Every diameter of the clean-up is
 // at least 2 and below 2*sqrt(3)*16384.
@@ -129,10 +123,94 @@ constexpr uint32_t kTopBase = 0x4000u;
 constexpr int kTopBins = 2048;
 constexpr int kEdtXBlocks = 4096;
 constexpr uint32_t kMinZChunk = 64;  // smallest z extent of a Z-pass tile
+constexpr uint32_t kXcds = 8;        // XCDs of the device (round-robin block dispatch)
 
-// A growable device arena, freed by the engine.
+// Item flags.
+constexpr uint32_t kItemThickness = 1u;  // run the local-thickness stages
+constexpr uint32_t kItemQuantiles = 2u;  // run the quantile selection (needs the map)
+constexpr uint32_t kItemDiag = 4u;       // fill work counters
+constexpr uint32_t kItemMap = 8u;        // the clean-up writes the thickness map
+
+struct TileMeta;
+
+// Everything the kernels need of one item (device memory, read with scalar
+// loads).  Block counts are per stage; the Plan's start tables turn a block
+// index into (item, block of the item).
+struct ItemDesc {
+  const uint8_t* data;
+  const uint8_t* roi;
+  uint16_t* dx;        // X pass output (N u16)
+  uint32_t* h;         // Y pass output, then the propagated map (N u32)
+  uint32_t* d2;        // squared EDT (N u32)
+  uint32_t* lists;     // per-tile sorted ridge lists (512 per tile)
+  TileMeta* meta;      // per-tile list header
+  unsigned long long* masks;  // per tile: 8 selected + 8 target words
+  uint32_t* overflow;  // [count, tile ids...]
+  uint32_t* tmpl;      // 3 x (bound+1) templates
+  float* diam;         // (bound+1) diameters 2*sqrt(m)
+  PhaseStatus* status;
+  BlockPartial* xpart;     // one per X-pass block
+  uint32_t* zmax;          // one per Z-pass block
+  StatsPartial* partials;  // one per clean-up block
+  StatsPartial* stats;     // final record
+  uint32_t* top16;         // top-16-bit value histogram (quantiles)
+  float* map;              // thickness map (kItemMap)
+  SelectState* sel;
+  unsigned long long* qhist;  // kMaxRanks x 256
+  unsigned long long* diag;   // 8 x 16 words per fill block (kItemDiag)
+  uint64_t n;
+  uint32_t W, H, D, TX, TY, TZ, ntiles, bound;
+  uint32_t want_nonzero, flags;
+  uint32_t xblocks;
+  uint32_t ygx, ygy, ygz, ychunk, yhalo;
+  uint32_t zgx, zgy, zgz, zchunk, zhalo;
+  uint32_t lblocks, fill_cap, cblocks, qblocks;
+  float scale;
+  int lsb;
+};
+
+// Stages with per-item block counts (start tables).
+enum Stage : int {
+  kStEdtX = 0,
+  kStEdtY,
+  kStEdtZ,
+  kStLuts,
+  kStTiles,     // ridges, fill gather (one block per tile, padded to a multiple of 8)
+  kStOverflow,  // fill overflow and scatter fallback
+  kStCleanup,
+  kStQHist,
+  kNumStages
+};
+
+// Host description of one item.
+struct ItemSpec {
+  PhaseSource src;
+  Dims dims;
+  Tuning tune;
+  uint32_t* d2_out = nullptr;  // squared EDT destination (null: internal)
+  float* map_out = nullptr;    // thickness map destination (null: internal if needed)
+  bool thickness = true;
+  bool quantiles = false;
+  bool want_map = false;       // keep the map (internal when map_out is null)
+  bool keep_d2 = true;         // false: an internal map may reuse the squared EDT buffer
+  bool diag = false;
+  float scale = 1.0f;          // voxel_spacing[0] as f32 (src/metrics.rs:114,120)
+  int lsb = 0;
+};
+
+// What comes back of one item.
+struct ItemResult {
+  PhaseStatus status;
+  StatsPartial stats;
+  SelectState sel;
+};
+
+// A growable device arena.
 class Arena {
  public:
+  Arena() = default;
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
   ~Arena();
   void reserve(size_t bytes);
   void* base() const { return base_; }
@@ -143,6 +221,48 @@ class Arena {
   size_t cap_ = 0;
 };
 
+// Device buffers, descriptors and launch geometry of a list of items.
+class Plan {
+ public:
+  Plan() = default;
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+  ~Plan();
+  // Lays out and (re)allocates the buffers of these items and uploads the
+  // descriptors (synchronously on s).  With ext, the buffers are laid out in
+  // that device region instead (at most ext_cap bytes; nothing is allocated).
+  void build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext = nullptr,
+             size_t ext_cap = 0);
+  // Enqueues every stage for every item on s.  No allocation, no host
+  // synchronisation, no memset nodes: safe under stream capture.
+  void launch(hipStream_t s) const;
+  // Gathers the per-item results on the device (part of launch) and copies
+  // them to the host; waits for s.
+  void fetch(hipStream_t s);
+  size_t size() const { return desc_.size(); }
+  const ItemDesc& desc(size_t i) const { return desc_[i]; }
+  const ItemResult& result(size_t i) const { return results_[i]; }
+  size_t device_bytes() const { return bytes_; }
+  // Device bytes a plan of these items needs (for admission control).
+  static size_t bytes_for(const std::vector<ItemSpec>& specs);
+
+ private:
+  std::vector<ItemDesc> desc_;
+  std::vector<uint32_t> starts_;  // kNumStages x (nitems + 1)
+  uint32_t total_[kNumStages] = {};
+  uint32_t y_lds_ = 0, z_lds_ = 0;
+  uint32_t any_ = 0;  // union of item flags
+  size_t bytes_ = 0;
+  Arena mem_;                         // item buffers
+  Arena meta_;                        // descriptors, start tables, results
+  ItemDesc* d_desc_ = nullptr;
+  uint32_t* d_starts_ = nullptr;
+  ItemResult* d_results_ = nullptr;
+  ItemResult* h_results_ = nullptr;  // pinned
+  size_t h_results_cap_ = 0;
+  std::vector<ItemResult> results_;
+};
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -150,28 +270,21 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   std::mutex& mutex() { return mu_; }
-
-  // Workspace for a volume of these dims (both phases share it sequentially).
-  PhaseBuffers buffers_for(const Dims& dims);
-  void* scratch_bytes(size_t bytes, int slot);  // extra staging buffers (H2D/D2H)
-
-  // Squared EDT into d2 (device pointer, may be buffers.d2).
-  void run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffers& b, uint32_t* d2,
-               hipStream_t s, const Tuning& t = Tuning());
+  Plan& plain_plan() { return plain_; }  // reused by the one-volume calls
+  void* scratch_bytes(size_t bytes, int slot);  // staging buffers (H2D/D2H)
   // phase_inside (src/volume.rs:84-105) as a device map.
   void run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t n, bool select_bone,
                         uint8_t* out, hipStream_t s);
-  // Full local thickness from an EDT already in b.d2: ridges, fill, clean-up,
-  // phase mask, statistics.  map_out may be null (stats only).
-  void run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out, float scale_f32,
-                     int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out = nullptr,
-                     uint32_t* propagated_out = nullptr, const Tuning& t = Tuning());
+  // Diagnostic products of a one-item plan that has run: ridge flags from the
+  // tile lists (and the ridge count into its status) and the propagated map.
+  void stage_outputs(const Plan& p, uint8_t* ridge_flags_out, uint32_t* propagated_out,
+                     hipStream_t s);
 
  private:
   int device_;
   hipStream_t stream_ = nullptr;
   std::mutex mu_;
-  Arena work_;
+  Plan plain_;
   Arena stage_[3];
 };
 
@@ -187,6 +300,8 @@ bool profile_enabled();
 hipEvent_t profile_begin();
 void profile_end(const char* name, hipEvent_t start, hipStream_t s);
 int profile_read(KernelTime* out, int max);  // synchronises, aggregates, resets
+// Profiling is suspended while a stream is being captured into a graph.
+void profile_suspend(bool on);
 
 // Launch a kernel, bracketed by profiling events when profiling is on.
 #define BM_LAUNCH(name, kernel, grid, block, lds, stream, ...)           \
@@ -205,5 +320,16 @@ int current_device();
 // Host-side conversion of the exact accumulator into f64 sums.
 StatsResult finish_stats(const StatsPartial& p, int lsb_exp);
 int stats_lsb_exp(float scale_f32);
+
+// Tunables per shape class, from the autotuner's JSON cache (defaults when
+// the cache has no entry): see bm_tune.cpp.
+Tuning tuned_for(const Dims& dims);
+std::string shape_class(const Dims& dims);
+
+// The five quantiles of nonzero_stats from a finished selection.
+struct QuantileResult {
+  double median = 0, p10 = 0, p25 = 0, p75 = 0, p90 = 0;
+};
+QuantileResult quantiles_from(const SelectState& sel, uint64_t count, float scale_f32);
 
 }  // namespace bm

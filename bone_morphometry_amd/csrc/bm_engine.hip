@@ -1,6 +1,8 @@
-// bm_engine.hip -- host side of the device engine: arenas, launch sequences.
+// bm_engine.hip -- host side of the device engine: plans (item layouts, start
+// tables, launch sequences), engines, kernel profiling, exact statistics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -13,46 +15,36 @@
 #include "bm_tiles.h"
 
 namespace bm {
-constexpr uint32_t kXcds = 8;  // must match bm_device.h
-}
-
-namespace bm {
 
 // Kernels (bm_kernels.hip).
-__global__ void k_edt_x(const uint8_t*, const uint8_t*, uint32_t, uint16_t*, uint32_t, uint32_t,
-                        BlockPartial*);
-__global__ void k_reduce_status(const BlockPartial*, uint32_t, const uint32_t*, uint32_t,
-                                PhaseStatus*);
-__global__ void k_edt_y(const uint16_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t);
-__global__ void k_edt_z(const uint32_t*, uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t,
-                        uint32_t, uint32_t*);
-__global__ void k_luts(const PhaseStatus*, uint32_t, uint32_t*, float*);
-// Tile kernels (bm_tiles.hip).
-__global__ void k_ridges_tiled(const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
-                               uint32_t, const uint32_t*, uint32_t, PhaseStatus*, uint32_t*,
-                               TileMeta*, unsigned long long*);
-__global__ void k_fill_gather(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
-                              const uint32_t*, const TileMeta*, const unsigned long long*,
-                              PhaseStatus*, uint32_t*, uint32_t*, uint32_t, uint32_t,
-                              unsigned long long*);
-__global__ void k_reduce_diag(const unsigned long long*, uint64_t, PhaseStatus*);
-__global__ void k_count_ridges(const TileMeta*, uint32_t, PhaseStatus*);
-__global__ void k_fill_overflow(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
-                                const uint32_t*, const TileMeta*, const PhaseStatus*, uint32_t*,
-                                const uint32_t*);
-__global__ void k_fill_scatter_lists(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
-                                     const uint32_t*, const TileMeta*, const PhaseStatus*,
-                                     uint32_t*);
-__global__ void k_zero_if_scatter(uint32_t*, uint64_t, const PhaseStatus*);
-__global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
-                                 const uint32_t*, const TileMeta*, uint8_t*);
-__global__ void k_cleanup_tiled(const uint32_t*, const unsigned long long*, const float*,
-                                const PhaseStatus*, uint32_t, uint32_t, uint32_t, uint32_t,
-                                uint32_t, uint32_t, float, int, float*, StatsPartial*, uint32_t*);
-__global__ void k_stats_final(const StatsPartial*, uint32_t, StatsPartial*);
+__global__ void k_prepare(const ItemDesc*);
+__global__ void k_edt_x(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_edt_y(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_edt_z(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_reduce_status(const ItemDesc*);
+__global__ void k_luts(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_stats_final(const ItemDesc*);
 __global__ void k_fill_u32(uint32_t*, uint64_t, uint32_t);
 __global__ void k_fill_u8(uint8_t*, uint64_t, uint8_t);
 __global__ void k_phase_inside(const uint8_t*, const uint8_t*, uint64_t, uint32_t, uint8_t*);
+__global__ void k_collect(const ItemDesc*, ItemResult*);
+// Tile kernels (bm_tiles.hip).
+__global__ void k_ridges_tiled(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_gather(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_gather_diag(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_reduce_diag(const ItemDesc*);
+__global__ void k_fill_overflow(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_zero_if_scatter(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_scatter_lists(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_cleanup_tiled(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_count_ridges(const TileMeta*, uint32_t, PhaseStatus*);
+__global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                 const uint32_t*, const TileMeta*, uint8_t*);
+// Quantile kernels (bm_quantiles.hip).
+__global__ void k_select_init(const ItemDesc*);
+__global__ void k_top16_pick(const ItemDesc*);
+__global__ void k_digit_hist(const ItemDesc*, const uint32_t*, uint32_t, int);
+__global__ void k_digit_pick(const ItemDesc*, int);
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
@@ -77,6 +69,240 @@ void Arena::reserve(size_t bytes) {
   cap_ = want;
 }
 
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static uint32_t grid_for(uint64_t items, unsigned per_block, unsigned cap) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (uint32_t)g;
+}
+
+static uint32_t round8(uint64_t v) { return (uint32_t)((v + kXcds - 1) / kXcds * kXcds); }
+
+// ---------------------------------------------------------------------------
+// Item layout: geometry, block counts and buffer offsets (relative to the
+// item's base when base is null).  Returns the bytes the item needs.
+static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* blocks) {
+  const Dims& dims = s.dims;
+  std::memset(&d, 0, sizeof d);
+  d.data = s.src.data;
+  d.roi = s.src.roi;
+  d.want_nonzero = s.src.want_nonzero;
+  d.W = dims.w;
+  d.H = dims.h;
+  d.D = dims.d;
+  d.n = dims.n();
+  d.TX = tiles_of(dims.w);
+  d.TY = tiles_of(dims.h);
+  d.TZ = tiles_of(dims.d);
+  d.ntiles = d.TX * d.TY * d.TZ;
+  d.bound = (uint32_t)dims.max_d2();
+  d.scale = s.scale;
+  d.lsb = s.lsb;
+  const bool lt = s.thickness;
+  const bool quant = lt && s.quantiles;
+  const bool mapped = lt && (s.want_map || quant || s.map_out);
+  d.flags = (lt ? kItemThickness : 0u) | (quant ? kItemQuantiles : 0u) |
+            (lt && s.diag ? kItemDiag : 0u) | (mapped ? kItemMap : 0u);
+  const Tuning& t = s.tune;
+  d.xblocks = grid_for((uint64_t)dims.h * dims.d * 64, 256, kEdtXBlocks);
+  {
+    const uint32_t H = dims.h;
+    d.ychunk = H <= t.y_rows ? H : t.y_rows - 2 * t.y_halo;
+    d.yhalo = H <= t.y_rows ? 0 : t.y_halo;
+    d.ygx = (dims.w + 63) / 64;
+    d.ygy = dims.d;
+    d.ygz = (H + d.ychunk - 1) / d.ychunk;
+  }
+  {
+    const uint32_t D = dims.d;
+    d.zchunk = D <= t.z_rows ? D : t.z_rows - 2 * t.z_halo;
+    d.zhalo = D <= t.z_rows ? 0 : t.z_halo;
+    if (d.zchunk < kMinZChunk && d.zchunk < D)
+      throw std::invalid_argument("z tile chunk below kMinZChunk");
+    d.zgx = (dims.w + 31) / 32;
+    d.zgy = dims.h;
+    d.zgz = (D + d.zchunk - 1) / d.zchunk;
+  }
+  d.lblocks = lt ? std::min<uint32_t>(std::max<uint32_t>((d.bound + 4) / 4, 1), 128) : 0;
+  d.fill_cap = t.fill_cap ? std::min<uint32_t>(t.fill_cap, kFillCap) : kFillCap;
+  d.cblocks = lt ? std::min<uint32_t>(std::max<uint32_t>(round8((d.ntiles + 7) / 8), 8),
+                                      kStatsBlocks)
+                 : 0;
+  d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 16384), 8), 2048) : 0;
+  if (blocks) {
+    blocks[kStEdtX] = d.xblocks;
+    blocks[kStEdtY] = d.ygx * d.ygy * d.ygz;
+    blocks[kStEdtZ] = d.zgx * d.zgy * d.zgz;
+    blocks[kStLuts] = d.lblocks;
+    blocks[kStTiles] = lt ? round8(d.ntiles) : 0;
+    blocks[kStOverflow] = lt ? kOverflowBlocks : 0;
+    blocks[kStCleanup] = d.cblocks;
+    blocks[kStQHist] = d.qblocks;
+  }
+  // Buffers.
+  const uint64_t n = d.n;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return base + o;
+  };
+  d.status = reinterpret_cast<PhaseStatus*>(take(sizeof(PhaseStatus)));
+  d.xpart = reinterpret_cast<BlockPartial*>(take((size_t)d.xblocks * sizeof(BlockPartial)));
+  d.zmax = reinterpret_cast<uint32_t*>(take((size_t)d.zgx * d.zgy * d.zgz * sizeof(uint32_t)));
+  d.dx = reinterpret_cast<uint16_t*>(take(n * sizeof(uint16_t)));
+  d.h = reinterpret_cast<uint32_t*>(take(n * sizeof(uint32_t)));
+  d.d2 = s.d2_out ? s.d2_out : reinterpret_cast<uint32_t*>(take(n * sizeof(uint32_t)));
+  if (lt) {
+    d.lists = reinterpret_cast<uint32_t*>(take((size_t)d.ntiles * 512 * sizeof(uint32_t)));
+    d.meta = reinterpret_cast<TileMeta*>(take((size_t)d.ntiles * sizeof(TileMeta)));
+    d.masks = reinterpret_cast<unsigned long long*>(take((size_t)d.ntiles * 16 * 8));
+    d.overflow = reinterpret_cast<uint32_t*>(take(((size_t)d.ntiles + 1) * sizeof(uint32_t)));
+    d.tmpl = reinterpret_cast<uint32_t*>(take(3 * ((size_t)d.bound + 1) * sizeof(uint32_t)));
+    d.diam = reinterpret_cast<float*>(take(((size_t)d.bound + 1) * sizeof(float)));
+    d.partials = reinterpret_cast<StatsPartial*>(take((size_t)d.cblocks * sizeof(StatsPartial)));
+    d.stats = reinterpret_cast<StatsPartial*>(take(sizeof(StatsPartial)));
+    if (mapped) {
+      if (s.map_out) d.map = s.map_out;
+      else if (!s.keep_d2 && !s.d2_out) d.map = reinterpret_cast<float*>(d.d2);  // d2 is dead by then
+      else d.map = reinterpret_cast<float*>(take(n * sizeof(float)));
+    }
+    if (quant) {
+      d.top16 = reinterpret_cast<uint32_t*>(take(kTopBins * sizeof(uint32_t)));
+      d.sel = reinterpret_cast<SelectState*>(take(sizeof(SelectState)));
+      d.qhist = reinterpret_cast<unsigned long long*>(take(kMaxRanks * 256 * 8));
+    }
+    if (d.flags & kItemDiag)
+      d.diag = reinterpret_cast<unsigned long long*>(take((size_t)round8(d.ntiles) * 8 * 16 * 8));
+  }
+  return off;
+}
+
+size_t Plan::bytes_for(const std::vector<ItemSpec>& specs) {
+  size_t total = 0;
+  for (const auto& s : specs) {
+    ItemDesc d;
+    total += lay_out(s, d, nullptr, nullptr);
+  }
+  return total;
+}
+
+Plan::~Plan() {
+  if (h_results_) (void)hipHostFree(h_results_);
+}
+
+void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, size_t ext_cap) {
+  const size_t nit = specs.size();
+  if (nit == 0) throw std::invalid_argument("a plan needs at least one item");
+  desc_.assign(nit, ItemDesc{});
+  std::vector<size_t> offs(nit);
+  std::vector<uint32_t> blocks(nit * kNumStages);
+  size_t total = 0;
+  any_ = 0;
+  y_lds_ = z_lds_ = 0;
+  for (size_t i = 0; i < nit; ++i) {
+    offs[i] = total;
+    total += lay_out(specs[i], desc_[i], nullptr, &blocks[i * kNumStages]);
+    any_ |= desc_[i].flags;
+    y_lds_ = std::max(y_lds_, std::min(desc_[i].H, desc_[i].ychunk + 2 * desc_[i].yhalo) * 64u * 2u);
+    z_lds_ = std::max(z_lds_, std::min(desc_[i].D, desc_[i].zchunk + 2 * desc_[i].zhalo) * 32u * 4u);
+  }
+  if (y_lds_ > 65536 || z_lds_ > 65536) throw std::invalid_argument("EDT tile exceeds 64 KiB of LDS");
+  char* base;
+  if (ext) {
+    if (total > ext_cap) throw std::range_error("plan exceeds the memory given to it");
+    base = static_cast<char*>(ext);
+  } else {
+    mem_.reserve(total);
+    base = static_cast<char*>(mem_.base());
+  }
+  bytes_ = total;
+  for (size_t i = 0; i < nit; ++i) lay_out(specs[i], desc_[i], base + offs[i], nullptr);
+  // Start tables: stage-major, nit + 1 entries each.
+  starts_.assign((size_t)kNumStages * (nit + 1), 0);
+  for (int st = 0; st < kNumStages; ++st) {
+    uint64_t acc = 0;
+    for (size_t i = 0; i < nit; ++i) {
+      starts_[st * (nit + 1) + i] = (uint32_t)acc;
+      acc += blocks[i * kNumStages + st];
+    }
+    if (acc > 0x7FFFFFFFull) throw std::range_error("too many blocks in one launch");
+    starts_[st * (nit + 1) + nit] = (uint32_t)acc;
+    total_[st] = (uint32_t)acc;
+  }
+  const size_t o_desc = 0;
+  const size_t o_starts = align_up(nit * sizeof(ItemDesc), 256);
+  const size_t o_res = align_up(o_starts + starts_.size() * sizeof(uint32_t), 256);
+  meta_.reserve(o_res + nit * sizeof(ItemResult));
+  char* mb = static_cast<char*>(meta_.base());
+  d_desc_ = reinterpret_cast<ItemDesc*>(mb + o_desc);
+  d_starts_ = reinterpret_cast<uint32_t*>(mb + o_starts);
+  d_results_ = reinterpret_cast<ItemResult*>(mb + o_res);
+  BM_HIP(hipMemcpyAsync(d_desc_, desc_.data(), nit * sizeof(ItemDesc), hipMemcpyHostToDevice, s));
+  BM_HIP(hipMemcpyAsync(d_starts_, starts_.data(), starts_.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, s));
+  BM_HIP(hipStreamSynchronize(s));
+  if (h_results_cap_ < nit) {
+    if (h_results_) BM_HIP(hipHostFree(h_results_));
+    h_results_ = nullptr;
+    BM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_results_), nit * sizeof(ItemResult)));
+    h_results_cap_ = nit;
+  }
+  results_.assign(nit, ItemResult{});
+}
+
+void Plan::launch(hipStream_t s) const {
+  const uint32_t nit = (uint32_t)desc_.size();
+  auto st = [&](int stage) { return d_starts_ + (size_t)stage * (nit + 1); };
+  k_prepare<<<nit, 256, 0, s>>>(d_desc_);
+  BM_LAUNCH("edt_x", k_edt_x, total_[kStEdtX], 256, 0, s, d_desc_, st(kStEdtX), nit);
+  BM_LAUNCH("edt_y", k_edt_y, total_[kStEdtY], 256, y_lds_, s, d_desc_, st(kStEdtY), nit);
+  BM_LAUNCH("edt_z", k_edt_z, total_[kStEdtZ], 256, z_lds_, s, d_desc_, st(kStEdtZ), nit);
+  BM_LAUNCH("reduce_status", k_reduce_status, nit, 1024, 0, s, d_desc_);
+  if (any_ & kItemThickness) {
+    BM_LAUNCH("luts", k_luts, total_[kStLuts], 256, 0, s, d_desc_, st(kStLuts), nit);
+    BM_LAUNCH("ridges", k_ridges_tiled, total_[kStTiles], 512, 0, s, d_desc_, st(kStTiles), nit);
+    if (any_ & kItemDiag)
+      BM_LAUNCH("fill_gather", k_fill_gather_diag, total_[kStTiles], 512, 0, s, d_desc_,
+                st(kStTiles), nit);
+    else
+      BM_LAUNCH("fill_gather", k_fill_gather, total_[kStTiles], 512, 0, s, d_desc_,
+                st(kStTiles), nit);
+    if (any_ & kItemDiag) k_reduce_diag<<<nit, 256, 0, s>>>(d_desc_);
+    BM_LAUNCH("fill_overflow", k_fill_overflow, total_[kStOverflow], 512, 0, s, d_desc_,
+              st(kStOverflow), nit);
+    BM_LAUNCH("fill_scatter_zero", k_zero_if_scatter, total_[kStOverflow], 256, 0, s, d_desc_,
+              st(kStOverflow), nit);
+    BM_LAUNCH("fill_scatter", k_fill_scatter_lists, total_[kStOverflow], 256, 0, s, d_desc_,
+              st(kStOverflow), nit);
+    BM_LAUNCH("cleanup_stats", k_cleanup_tiled, total_[kStCleanup], 512, 0, s, d_desc_,
+              st(kStCleanup), nit);
+    BM_LAUNCH("stats_final", k_stats_final, nit, 256, 0, s, d_desc_);
+    if (any_ & kItemQuantiles) {
+      BM_LAUNCH("quantile_pick", k_select_init, nit, 256, 0, s, d_desc_);
+      BM_LAUNCH("quantile_pick", k_top16_pick, nit, 256, 0, s, d_desc_);
+      for (int shift = 8; shift >= 0; shift -= 8) {
+        BM_LAUNCH("quantile_hist", k_digit_hist, total_[kStQHist], 256, 0, s, d_desc_,
+                  st(kStQHist), nit, shift);
+        BM_LAUNCH("quantile_pick", k_digit_pick, nit, 256, 0, s, d_desc_, shift);
+      }
+    }
+  }
+  k_collect<<<nit, 64, 0, s>>>(d_desc_, d_results_);
+  BM_HIP(hipGetLastError());
+}
+
+void Plan::fetch(hipStream_t s) {
+  const size_t nit = desc_.size();
+  BM_HIP(hipMemcpyAsync(h_results_, d_results_, nit * sizeof(ItemResult), hipMemcpyDeviceToHost,
+                        s));
+  BM_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < nit; ++i) results_[i] = h_results_[i];
+}
+
+// ---------------------------------------------------------------------------
 Engine::Engine(int device) : device_(device) {
   BM_HIP(hipSetDevice(device));
   BM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -86,103 +312,9 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-PhaseBuffers Engine::buffers_for(const Dims& dims) {
-  const uint64_t n = dims.n();
-  const uint64_t bound = dims.max_d2();
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off = align_up(off + bytes, 256);
-    return o;
-  };
-  const size_t o_dx = take(n * sizeof(uint16_t));
-  const size_t o_h = take(n * sizeof(uint32_t));
-  const size_t o_d2 = take(n * sizeof(uint32_t));
-  const uint32_t TX = tiles_of(dims.w), TY = tiles_of(dims.h), TZ = tiles_of(dims.d);
-  const uint64_t ntiles = (uint64_t)TX * TY * TZ;
-  const size_t o_l = take(ntiles * 512 * sizeof(uint32_t));
-  const size_t o_meta = take(ntiles * sizeof(TileMeta));
-  const size_t o_mask = take(ntiles * 16 * sizeof(unsigned long long));
-  const size_t o_of = take((ntiles + 1) * sizeof(uint32_t));
-  const size_t o_t = take(3 * (bound + 1) * sizeof(uint32_t));
-  const size_t o_dm = take((bound + 1) * sizeof(float));
-  const size_t o_st = take(sizeof(PhaseStatus));
-  const size_t o_xp = take(kEdtXBlocks * sizeof(BlockPartial));
-  // Z-pass blocks for any z chunk of at least kMinZChunk lines
-  const uint64_t zblocks = (uint64_t)((dims.w + 31) / 32) * dims.h * ((dims.d + kMinZChunk - 1) / kMinZChunk);
-  const size_t o_zm = take(zblocks * sizeof(uint32_t));
-  const size_t o_p = take(kStatsBlocks * sizeof(StatsPartial));
-  const size_t o_s = take(sizeof(StatsPartial));
-  const size_t o_top = take(kTopBins * sizeof(uint32_t));
-  work_.reserve(off);
-  char* b = static_cast<char*>(work_.base());
-  PhaseBuffers pb;
-  pb.dx = reinterpret_cast<uint16_t*>(b + o_dx);
-  pb.h = reinterpret_cast<uint32_t*>(b + o_h);
-  pb.d2 = reinterpret_cast<uint32_t*>(b + o_d2);
-  pb.lists = reinterpret_cast<uint32_t*>(b + o_l);
-  pb.meta = reinterpret_cast<TileMeta*>(b + o_meta);
-  pb.masks = reinterpret_cast<unsigned long long*>(b + o_mask);
-  pb.overflow = reinterpret_cast<uint32_t*>(b + o_of);
-  pb.tiles[0] = TX;
-  pb.tiles[1] = TY;
-  pb.tiles[2] = TZ;
-  pb.tmpl = reinterpret_cast<uint32_t*>(b + o_t);
-  pb.diam = reinterpret_cast<float*>(b + o_dm);
-  pb.status = reinterpret_cast<PhaseStatus*>(b + o_st);
-  pb.xpart = reinterpret_cast<BlockPartial*>(b + o_xp);
-  pb.zmax = reinterpret_cast<uint32_t*>(b + o_zm);
-  pb.partials = reinterpret_cast<StatsPartial*>(b + o_p);
-  pb.stats = reinterpret_cast<StatsPartial*>(b + o_s);
-  pb.top16 = reinterpret_cast<uint32_t*>(b + o_top);
-  pb.bound = bound;
-  return pb;
-}
-
 void* Engine::scratch_bytes(size_t bytes, int slot) {
   stage_[slot].reserve(bytes);
   return stage_[slot].base();
-}
-
-static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap) {
-  uint64_t g = (items + per_block - 1) / per_block;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (unsigned)g;
-}
-
-void Engine::run_edt(const PhaseSource& src, const Dims& dims, const PhaseBuffers& b,
-                     uint32_t* d2, hipStream_t s, const Tuning& t) {
-  const uint32_t W = dims.w, H = dims.h, D = dims.d;
-  // status <- 0 (a small fill kernel, never a memset node: graph-safe)
-  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t*>(b.status),
-                              sizeof(PhaseStatus) / sizeof(uint32_t), 0u);
-  const uint32_t rows = H * D;
-  const unsigned xblocks = grid_for((uint64_t)rows * 64, 256, kEdtXBlocks);
-  BM_LAUNCH("edt_x", k_edt_x, xblocks, 256, 0, s, src.data, src.roi, src.want_nonzero, b.dx, W,
-            rows, b.xpart);
-  {
-    const uint32_t chunk = H <= t.y_rows ? H : t.y_rows - 2 * t.y_halo;
-    const uint32_t halo = H <= t.y_rows ? 0 : t.y_halo;
-    const uint32_t rows_lds = std::min<uint32_t>(H, chunk + 2 * halo);
-    dim3 grid((W + 63) / 64, D, (H + chunk - 1) / chunk);
-    BM_LAUNCH("edt_y", k_edt_y, grid, 256, rows_lds * 64 * sizeof(uint16_t), s, b.dx, b.h, W, H,
-              chunk, halo);
-  }
-  {
-    const uint32_t chunk = D <= t.z_rows ? D : t.z_rows - 2 * t.z_halo;
-    const uint32_t halo = D <= t.z_rows ? 0 : t.z_halo;
-    const uint32_t rows_lds = std::min<uint32_t>(D, chunk + 2 * halo);
-    dim3 grid((W + 31) / 32, H, (D + chunk - 1) / chunk);
-    if (chunk < kMinZChunk && chunk < D) throw std::invalid_argument("z tile chunk below kMinZChunk");
-    BM_LAUNCH("edt_z", k_edt_z, grid, 256, rows_lds * 32 * sizeof(uint32_t), s, b.h, d2, W, H, D,
-              chunk, halo, b.zmax);
-    BM_LAUNCH("reduce_status", k_reduce_status, 1, 1024, 0, s, b.xpart, xblocks, b.zmax,
-              grid.x * grid.y * grid.z, b.status);
-  }
-  BM_HIP(hipGetLastError());
 }
 
 void Engine::run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t n,
@@ -192,56 +324,18 @@ void Engine::run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t 
   BM_HIP(hipGetLastError());
 }
 
-void Engine::run_thickness(const Dims& dims, const PhaseBuffers& b, float* map_out,
-                           float scale_f32, int lsb_exp, hipStream_t s, uint8_t* ridge_flags_out,
-                           uint32_t* propagated_out, const Tuning& tune) {
-  const uint32_t W = dims.w, H = dims.h, D = dims.d;
-  const uint64_t n = dims.n();
-  const uint32_t bound = (uint32_t)b.bound;
-  BM_LAUNCH("luts", k_luts, grid_for((uint64_t)bound + 1, 256, 1u << 30), 256, 0, s, b.status,
-            bound, b.tmpl, b.diam);
-  const uint32_t TX = b.tiles[0], TY = b.tiles[1], TZ = b.tiles[2];
-  const uint32_t ntiles = TX * TY * TZ;
-  const uint32_t xgrid = ((ntiles + kXcds - 1) / kXcds) * kXcds;  // see xcd_tile
-  BM_LAUNCH("ridges", k_ridges_tiled, xgrid, 512, 0, s, b.d2, W, H, D, TX, TY, TZ, b.tmpl, bound,
-            b.status, b.lists, b.meta, b.masks);
-  uint32_t* M = b.h;  // the Y-pass buffer is free once the Z pass has run
-  k_fill_u32<<<1, 64, 0, s>>>(b.overflow, 1, 0u);
-  // The gather fills every tile when the largest radius is at most
-  // kGatherMaxRadius; otherwise it returns at once and the scatter runs.
-  uint32_t cap = tune.fill_cap ? std::min<uint32_t>(tune.fill_cap, kFillCap) : kFillCap;
-  if (const char* env = std::getenv("BM_FILL_CAP")) {  // tuning / test override
-    const long v = std::strtol(env, nullptr, 10);
-    if (v >= 1 && v <= kFillCap) cap = (uint32_t)v;
-  }
-  const uint32_t diag = std::getenv("BM_FILL_DIAG") ? 1u : 0u;  // work counters
-  unsigned long long* diag_buf = nullptr;
-  if (diag) {
-    diag_buf = static_cast<unsigned long long*>(scratch_bytes((size_t)xgrid * 8 * 16 * 8, 2));
-    k_fill_u32<<<grid_for((uint64_t)xgrid * 8 * 32, 256, 65536), 256, 0, s>>>(
-        reinterpret_cast<uint32_t*>(diag_buf), (uint64_t)xgrid * 8 * 32, 0u);
-  }
-  BM_LAUNCH("fill_gather", k_fill_gather, xgrid, 512, 0, s, W, H, D, TX, TY, TZ, b.lists, b.meta,
-            b.masks, b.status, M, b.overflow, cap, diag, diag_buf);
-  if (diag) k_reduce_diag<<<1, 256, 0, s>>>(diag_buf, (uint64_t)xgrid * 8, b.status);
-  BM_LAUNCH("fill_overflow", k_fill_overflow, 256, 512, 0, s, W, H, D, TX, TY, TZ, b.lists,
-            b.meta, b.status, M, b.overflow);
-  BM_LAUNCH("fill_scatter_zero", k_zero_if_scatter, grid_for(n, 256, 65536), 256, 0, s, M, n,
-            b.status);
-  BM_LAUNCH("fill_scatter", k_fill_scatter_lists, 2048, 256, 0, s, W, H, D, TX, TY, ntiles, b.lists,
-            b.meta, b.status, M);
+void Engine::stage_outputs(const Plan& p, uint8_t* ridge_flags_out, uint32_t* propagated_out,
+                           hipStream_t s) {
+  const ItemDesc& d = p.desc(0);
+  if (!(d.flags & kItemThickness)) return;
   if (ridge_flags_out) {
-    k_count_ridges<<<256, 256, 0, s>>>(b.meta, ntiles, b.status);
-    k_fill_u8<<<grid_for(n, 256, 65536), 256, 0, s>>>(ridge_flags_out, n, 0);
-    k_lists_to_flags<<<4096, 256, 0, s>>>(W, H, D, TX, TY, ntiles, b.lists, b.meta,
+    k_count_ridges<<<256, 256, 0, s>>>(d.meta, d.ntiles, d.status);
+    k_fill_u8<<<grid_for(d.n, 256, 65536), 256, 0, s>>>(ridge_flags_out, d.n, 0);
+    k_lists_to_flags<<<4096, 256, 0, s>>>(d.W, d.H, d.D, d.TX, d.TY, d.ntiles, d.lists, d.meta,
                                           ridge_flags_out);
   }
-  if (propagated_out) BM_HIP(hipMemcpyAsync(propagated_out, M, n * 4, hipMemcpyDeviceToDevice, s));
-  uint32_t* top16 = map_out ? b.top16 : nullptr;  // quantiles need the map and its histogram
-  if (top16) k_fill_u32<<<8, 256, 0, s>>>(top16, kTopBins, 0u);
-  BM_LAUNCH("cleanup_stats", k_cleanup_tiled, kStatsBlocks, 512, 0, s, M, b.masks, b.diam,
-            b.status, W, H, D, TX, TY, ntiles, scale_f32, lsb_exp, map_out, b.partials, top16);
-  BM_LAUNCH("stats_final", k_stats_final, 1, 256, 0, s, b.partials, kStatsBlocks, b.stats);
+  if (propagated_out)
+    BM_HIP(hipMemcpyAsync(propagated_out, d.h, d.n * 4, hipMemcpyDeviceToDevice, s));
   BM_HIP(hipGetLastError());
 }
 
@@ -279,6 +373,7 @@ struct ProfRecord {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+bool g_prof_suspended = false;
 std::vector<ProfRecord>* g_prof_records = new std::vector<ProfRecord>();
 std::vector<hipEvent_t>* g_prof_pool = new std::vector<hipEvent_t>();
 
@@ -299,7 +394,12 @@ void profile_enable(bool on) {
   g_prof_on = on;
 }
 
-bool profile_enabled() { return g_prof_on; }
+void profile_suspend(bool on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_suspended = on;
+}
+
+bool profile_enabled() { return g_prof_on && !g_prof_suspended; }
 
 hipEvent_t profile_begin() {
   std::lock_guard<std::mutex> g(g_prof_mu);

@@ -14,6 +14,10 @@
 // Local thickness (src/local_thickness.rs:16-44): templates and diameters as
 // look-up tables indexed by r^2, the 26-neighbour ridge filter, the sphere
 // fill, the surface clean-up with the phase mask, and exact statistics.
+//
+// Every stage kernel serves a list of items (ItemDesc): a block finds its item
+// and its block within the item from the stage's start table; kernels with one
+// block per item take the item from blockIdx.x.
 #include <hip/hip_runtime.h>
 
 #include "bm_device.h"
@@ -58,12 +62,30 @@ __device__ __forceinline__ uint32_t site_bits8(const uint8_t* __restrict__ data,
 constexpr int kNoneL = -(1 << 20);
 constexpr int kNoneR = (1 << 21);
 
+// Zeroes the status (and the counters the later stages accumulate into) of
+// each item; one block per item.  A kernel, not a memset: graph-safe.
+__global__ void __launch_bounds__(256) k_prepare(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[blockIdx.x];
+  uint32_t* st = reinterpret_cast<uint32_t*>(I.status);
+  for (uint32_t i = threadIdx.x; i < sizeof(PhaseStatus) / 4; i += blockDim.x) st[i] = 0;
+  if (I.flags & kItemThickness) {
+    if (threadIdx.x == 0) I.overflow[0] = 0;
+    if (I.flags & kItemQuantiles)
+      for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) I.top16[i] = 0;
+  }
+}
+
 __global__ void __launch_bounds__(256)
-k_edt_x(const uint8_t* __restrict__ data, const uint8_t* __restrict__ roi, uint32_t want,
-        uint16_t* __restrict__ out, uint32_t W, uint32_t rows, BlockPartial* __restrict__ part) {
+k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint8_t* __restrict__ data = I.data;
+  const uint8_t* __restrict__ roi = I.roi;
+  const uint32_t want = I.want_nonzero, W = I.W, rows = I.H * I.D;
+  uint16_t* __restrict__ out = I.dx;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t gw = (vb * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (I.xblocks * blockDim.x) >> 6;
   const uint32_t nseg = (W + 511) / 512;
   uint32_t selcnt = 0, anysite = 0;
   for (uint32_t r = gw; r < rows; r += nw) {
@@ -152,19 +174,24 @@ k_edt_x(const uint8_t* __restrict__ data, const uint8_t* __restrict__ roi, uint3
       a += s_sel[i];
       b |= s_any[i];
     }
-    part[blockIdx.x].selected = a;
-    part[blockIdx.x].flag = b;
+    I.xpart[vb].selected = a;
+    I.xpart[vb].flag = b;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Y pass: tile of 64 x-columns x rows [ya, yb) of one z-slice in LDS (u16).
 __global__ void __launch_bounds__(256)
-k_edt_y(const uint16_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W, uint32_t H,
-        uint32_t chunk, uint32_t halo) {
+k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
   extern __shared__ uint16_t ytile[];
-  const uint32_t x0 = blockIdx.x * 64, z = blockIdx.y;
-  const uint32_t y0 = blockIdx.z * chunk, y1 = min(H, y0 + chunk);
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint16_t* __restrict__ in = I.dx;
+  uint32_t* __restrict__ out = I.h;
+  const uint32_t W = I.W, H = I.H, chunk = I.ychunk, halo = I.yhalo;
+  const uint32_t bx = vb % I.ygx, bz = vb / I.ygx / I.ygy;
+  const uint32_t x0 = bx * 64, z = (vb / I.ygx) % I.ygy;
+  const uint32_t y0 = bz * chunk, y1 = min(H, y0 + chunk);
   const uint32_t ya = y0 > halo ? y0 - halo : 0, yb = min(H, y1 + halo);
   const size_t sbase = (size_t)z * W * H;
   const uint32_t nrows = yb - ya;
@@ -213,11 +240,15 @@ k_edt_y(const uint16_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W,
 // ---------------------------------------------------------------------------
 // Z pass: tile of 32 x-columns x z-range [za, zb) at one y in LDS (u32).
 __global__ void __launch_bounds__(256)
-k_edt_z(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W, uint32_t H,
-        uint32_t D, uint32_t chunk, uint32_t halo, uint32_t* __restrict__ zmax) {
+k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
   extern __shared__ uint32_t ztile[];
-  const uint32_t x0 = blockIdx.x * 32, y = blockIdx.y;
-  const uint32_t z0 = blockIdx.z * chunk, z1 = min(D, z0 + chunk);
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t* __restrict__ in = I.h;
+  uint32_t* __restrict__ out = I.d2;
+  const uint32_t W = I.W, H = I.H, D = I.D, chunk = I.zchunk, halo = I.zhalo;
+  const uint32_t x0 = (vb % I.zgx) * 32, y = (vb / I.zgx) % I.zgy;
+  const uint32_t z0 = (vb / I.zgx / I.zgy) * chunk, z1 = min(D, z0 + chunk);
   const uint32_t za = z0 > halo ? z0 - halo : 0, zb = min(D, z1 + halo);
   const size_t slice = (size_t)W * H;
   const size_t base = (size_t)y * W + x0;
@@ -273,14 +304,17 @@ k_edt_z(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t W,
   if (threadIdx.x == 0) {
     uint32_t r = 0;
     for (uint32_t i = 0; i < nwv; ++i) r = max(r, s_mx[i]);
-    zmax[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+    I.zmax[vb] = r;
   }
 }
 
 // Status of a phase from the EDT passes' per-block partials (one block).
-__global__ void __launch_bounds__(1024)
-k_reduce_status(const BlockPartial* __restrict__ xpart, uint32_t nx, const uint32_t* __restrict__ zmax,
-                uint32_t nz, PhaseStatus* __restrict__ st) {
+__global__ void __launch_bounds__(1024) k_reduce_status(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[blockIdx.x];
+  const BlockPartial* __restrict__ xpart = I.xpart;
+  const uint32_t* __restrict__ zmax = I.zmax;
+  const uint32_t nx = I.xblocks, nz = I.zgx * I.zgy * I.zgz;
+  PhaseStatus* __restrict__ st = I.status;
   unsigned long long sel = 0;
   uint32_t any = 0, mx = 0;
   for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -311,24 +345,57 @@ k_reduce_status(const BlockPartial* __restrict__ xpart, uint32_t nx, const uint3
   }
 }
 
-// Templates T_c[r^2] and diameters for every r^2 in [0, max_d2].
+// Templates T_c[r^2] (scan_cube, src/local_thickness.rs:75-102, for the
+// offsets (1,0,0), (1,1,0), (1,1,1)) and diameters for every r^2 in
+// [0, max_d2].  One wavefront per r^2: lanes take k = lane, lane + 64, ...;
+// along j the integer square root isqrt(r^2 - k^2 - j^2) only decreases, so it
+// is stepped down instead of recomputed.
 __global__ void __launch_bounds__(256)
-k_luts(const PhaseStatus* __restrict__ st, uint32_t bound, uint32_t* __restrict__ tmpl,
-       float* __restrict__ diam) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m > bound || !st->has_site) return;
+k_luts(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const PhaseStatus* __restrict__ st = I.status;
+  if (!st->has_site) return;
   const uint32_t mx = st->max_d2;
-  if (m > mx) return;
-  diam[m] = diameter_of(m);
-  if (m == 0) return;
-  tmpl[m] = scan_cube(1, 0, 0, m);
-  tmpl[(bound + 1) + m] = scan_cube(1, 1, 0, m);
-  tmpl[2 * (bound + 1) + m] = scan_cube(1, 1, 1, m);
+  const uint32_t lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+  const uint32_t nw = I.lblocks * wpb, L = I.bound + 1;
+  for (uint32_t m = vb * wpb + (threadIdx.x >> 6); m <= mx; m += nw) {
+    if (lane == 0) I.diam[m] = diameter_of(m);
+    if (m == 0) continue;
+    const uint32_t radius = isqrt_u32(m) + 1;
+    uint32_t t1 = 0, t2 = 0, t3 = 0;
+    for (uint32_t k = lane; k <= radius; k += 64) {
+      const uint32_t sk = k * k;
+      if (sk > m) break;
+      uint32_t i = isqrt_u32(m - sk);
+      for (uint32_t j = 0;; ++j) {
+        const uint32_t skj = sk + j * j;
+        if (skj > m) break;
+        while (i * i > m - skj) --i;
+        const uint32_t ip = (i + 1) * (i + 1), j1 = (j + 1) * (j + 1);
+        t1 = max(t1, skj + ip);                       // (k, j, i+1)
+        t2 = max(t2, sk + j1 + ip);                   // (k, j+1, i+1)
+        t3 = max(t3, (k + 1) * (k + 1) + j1 + ip);    // (k+1, j+1, i+1)
+      }
+    }
+    t1 = wave_max_u32(t1);
+    t2 = wave_max_u32(t2);
+    t3 = wave_max_u32(t3);
+    if (lane == 0) {
+      I.tmpl[m] = t1;
+      I.tmpl[L + m] = t2;
+      I.tmpl[2 * L + m] = t3;
+    }
+  }
 }
 
-__global__ void __launch_bounds__(256)
-k_stats_final(const StatsPartial* __restrict__ partials, uint32_t count,
-              StatsPartial* __restrict__ out) {
+// Exact statistics of an item from its clean-up partials; one block per item.
+__global__ void __launch_bounds__(256) k_stats_final(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[blockIdx.x];
+  if (!(I.flags & kItemThickness)) return;
+  const StatsPartial* __restrict__ partials = I.partials;
+  const uint32_t count = I.cblocks;
+  StatsPartial* __restrict__ out = I.stats;
   unsigned long long cnt = 0;
   u128 s = 0, s2 = 0;
   uint32_t mxb = 0, flags = 0;
@@ -341,6 +408,27 @@ k_stats_final(const StatsPartial* __restrict__ partials, uint32_t count,
     flags |= p.flags;
   }
   block_reduce_store(cnt, s, s2, mxb, flags, out);
+}
+
+// Per-item results (status, statistics, selection) gathered into one array
+// for a single copy to the host; one block per item.
+__global__ void __launch_bounds__(64) k_collect(const ItemDesc* __restrict__ items,
+                                               ItemResult* __restrict__ out) {
+  const ItemDesc& I = items[blockIdx.x];
+  uint32_t* r = reinterpret_cast<uint32_t*>(out + blockIdx.x);
+  const uint32_t* st = reinterpret_cast<const uint32_t*>(I.status);
+  const uint32_t* sp = (I.flags & kItemThickness) ? reinterpret_cast<const uint32_t*>(I.stats) : nullptr;
+  const uint32_t* se = (I.flags & kItemQuantiles) ? reinterpret_cast<const uint32_t*>(I.sel) : nullptr;
+  constexpr uint32_t n0 = sizeof(PhaseStatus) / 4, n1 = sizeof(StatsPartial) / 4,
+                     n2 = sizeof(SelectState) / 4;
+  static_assert(sizeof(ItemResult) == 4 * (n0 + n1 + n2), "ItemResult has no padding");
+  for (uint32_t i = threadIdx.x; i < n0 + n1 + n2; i += blockDim.x) {
+    uint32_t v = 0;
+    if (i < n0) v = st[i];
+    else if (i < n0 + n1) v = sp ? sp[i - n0] : 0u;
+    else v = se ? se[i - n0 - n1] : 0u;
+    r[i] = v;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {

@@ -1,40 +1,120 @@
-// bm_quantiles.hip -- radix select of up to 10 order statistics at once.
+// bm_quantiles.hip -- radix select of the order statistics behind the five
+// quantiles of nonzero_stats (src/metrics.rs:242-255,271-298), per item.
 //
 // The values are positive finite f32, whose bit patterns sort like the values,
-// so the k-th smallest value is found digit by digit (4 passes of 8 bits, most
-// significant first).  Each pass histograms, for every wanted rank, the next
-// digit of the values that share the rank's prefix found so far; a one-block
-// kernel then picks the digit holding the rank.  The result is the exact
-// order statistic, the same value select_nth_unstable_by returns in the
-// reference (src/metrics.rs:293-298), independent of any ordering of the map.
+// so the k-th smallest value is found digit by digit, most significant first.
+// The clean-up kernel already counts the top 16 bits (kTopBase + bin); two
+// 8-bit passes over the map follow.  Each pass histograms, for every wanted
+// rank, the next digit of the values that share the rank's prefix found so
+// far; a one-block-per-item kernel then picks the digit holding the rank.  The
+// result is the exact order statistic, the same value select_nth_unstable_by
+// returns in the reference, independent of any ordering of the map.  The ranks
+// themselves are computed on the device from the item's count, so the whole
+// selection runs without the host.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "bm_device.h"
-#include "bm_quantiles.h"
+#include "bm_engine.h"
 
 namespace bm {
 
-constexpr int kMaxRanks = 10;
+// The quantiles in the order of src/metrics.rs:242-255.
+__constant__ double c_qs[5] = {0.5, 0.1, 0.25, 0.75, 0.9};
+static const double h_qs[5] = {0.5, 0.1, 0.25, 0.75, 0.9};
 
-struct SelectState {
-  uint32_t prefix[kMaxRanks];
-  unsigned long long rank[kMaxRanks];  // rank within the values sharing prefix
-  uint32_t nranks;
-  uint32_t pad;
-};
+// Ranks of linear_quantile_unstable (src/metrics.rs:271-291): slot 2q holds
+// floor(q (count - 1)), slot 2q + 1 ceil(q (count - 1)).  One block per item.
+__global__ void __launch_bounds__(256) k_select_init(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[blockIdx.x];
+  if (!(I.flags & kItemQuantiles)) return;
+  SelectState* __restrict__ st = I.sel;
+  const unsigned long long count = I.stats->count;
+  if (threadIdx.x < 5) {
+    const uint32_t q = threadIdx.x;
+    const double position = c_qs[q] * (double)(count > 0 ? count - 1 : 0);
+    st->rank[2 * q] = (unsigned long long)floor(position);
+    st->rank[2 * q + 1] = (unsigned long long)ceil(position);
+    st->prefix[2 * q] = 0;
+    st->prefix[2 * q + 1] = 0;
+  }
+  if (threadIdx.x == 0) {
+    st->nranks = count > 0 ? (uint32_t)kMaxRanks : 0u;
+    st->pad = 0;
+  }
+}
+
+// Inclusive scan of one u64 per thread over a 256-thread block.
+__device__ __forceinline__ unsigned long long block_scan256(unsigned long long v,
+                                                            unsigned long long* s_w) {
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long y = __shfl_up(v, off);
+    if (lane >= (uint32_t)off) v += y;
+  }
+  __syncthreads();
+  if (lane == 63) s_w[wv] = v;
+  __syncthreads();
+  for (uint32_t i = 0; i < wv; ++i) v += s_w[i];
+  return v;
+}
+
+// First digit from the clean-up's 16-bit histogram: bin b counts values whose
+// top 16 bits are kTopBase + b.  Each thread owns 8 consecutive bins.  One
+// block per item; also clears the digit histograms.
+__global__ void __launch_bounds__(256)
+k_top16_pick(const ItemDesc* __restrict__ items) {
+  __shared__ unsigned long long s_w[4];
+  const ItemDesc& I = items[blockIdx.x];
+  if (!(I.flags & kItemQuantiles)) return;
+  SelectState* __restrict__ st = I.sel;
+  const uint32_t* __restrict__ top16 = I.top16;
+  for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) I.qhist[i] = 0;
+  constexpr int per = kTopBins / 256;
+  uint32_t c[per];
+  unsigned long long own = 0;
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    c[k] = top16[threadIdx.x * per + k];
+    own += c[k];
+  }
+  const unsigned long long incl = block_scan256(own, s_w);
+  const unsigned long long excl = incl - own;
+  const uint32_t nr = st->nranks;
+  __syncthreads();
+  for (uint32_t r = 0; r < nr; ++r) {
+    const unsigned long long k = st->rank[r];
+    __syncthreads();
+    if (k >= excl && k < incl) {  // exactly one thread holds the rank
+      unsigned long long cum = excl;
+      uint32_t b = 0;
+      for (; b < (uint32_t)per - 1; ++b) {
+        if (k < cum + c[b]) break;
+        cum += c[b];
+      }
+      st->prefix[r] = (threadIdx.x * per + b + kTopBase) << 16;
+      st->rank[r] = k - cum;
+    }
+  }
+}
 
 // One 8-bit digit (bits shift..shift+7) of the values sharing each rank's
 // prefix; ranks with equal prefixes share one histogram row.
 __global__ void __launch_bounds__(256)
-k_digit_hist(const float* __restrict__ map, uint64_t n, const SelectState* __restrict__ st,
-             int shift, unsigned long long* __restrict__ hist /* [kMaxRanks][256] */) {
+k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+             uint32_t nitems, int shift) {
   __shared__ uint32_t h[kMaxRanks][256];
   __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng;
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const SelectState* __restrict__ st = I.sel;
+  const float* __restrict__ map = I.map;
+  const uint64_t n = I.n;
+  unsigned long long* __restrict__ hist = I.qhist;
   const uint32_t nr = st->nranks;
+  if (nr == 0) return;
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   const uint32_t hi_mask = shift >= 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
   if (threadIdx.x == 0) {  // distinct prefixes
@@ -49,10 +129,10 @@ k_digit_hist(const float* __restrict__ map, uint64_t n, const SelectState* __res
   }
   __syncthreads();
   const uint32_t ng = s_ng;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t stride = (uint64_t)I.qblocks * blockDim.x;
   const uint64_t iters = (n + stride - 1) / stride;  // uniform trip count (wave ballots)
   for (uint64_t it = 0; it < iters; ++it) {
-    const uint64_t i = it * stride + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = it * stride + (uint64_t)vb * blockDim.x + threadIdx.x;
     uint32_t g = ng, digit = 0;
     if (i < n) {
       const uint32_t b = __float_as_uint(map[i]);
@@ -75,111 +155,53 @@ k_digit_hist(const float* __restrict__ map, uint64_t n, const SelectState* __res
   }
 }
 
-__global__ void k_digit_pick(SelectState* __restrict__ st, int shift,
-                             unsigned long long* __restrict__ hist) {
-  const uint32_t r = threadIdx.x;
-  if (r < st->nranks) {
-    unsigned long long k = st->rank[r], cum = 0;
-    uint32_t d = 0;
-    for (; d < 256; ++d) {
-      const unsigned long long c = hist[r * 256 + d];
-      if (k < cum + c) break;
-      cum += c;
+// The digit holding each rank; one block (256 threads = 256 digits) per item.
+// Clears the histograms for the next pass.
+__global__ void __launch_bounds__(256) k_digit_pick(const ItemDesc* __restrict__ items, int shift) {
+  __shared__ unsigned long long s_w[4];
+  const ItemDesc& I = items[blockIdx.x];
+  if (!(I.flags & kItemQuantiles)) return;
+  SelectState* __restrict__ st = I.sel;
+  unsigned long long* __restrict__ hist = I.qhist;
+  const uint32_t nr = st->nranks;
+  const uint32_t d = threadIdx.x;
+  for (uint32_t r = 0; r < nr; ++r) {
+    const unsigned long long c = hist[r * 256 + d];
+    const unsigned long long incl = block_scan256(c, s_w);
+    const unsigned long long excl = incl - c;
+    const unsigned long long k = st->rank[r];
+    __syncthreads();
+    if (k >= excl && k < incl) {
+      st->prefix[r] |= d << shift;
+      st->rank[r] = k - excl;
     }
-    st->prefix[r] |= d << shift;
-    st->rank[r] = k - cum;
+    __syncthreads();
   }
-  __syncthreads();
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) hist[i] = 0;
 }
 
-// First level from the 16-bit histogram the clean-up kernel accumulates:
-// bin b counts values whose top 16 bits are kTopBase + b.
-__global__ void k_top16_pick(SelectState* __restrict__ st, const uint32_t* __restrict__ top16,
-                             unsigned long long* __restrict__ hist) {
-  const uint32_t r = threadIdx.x;
-  if (r < st->nranks) {
-    unsigned long long k = st->rank[r], cum = 0;
-    uint32_t d = 0;
-    for (; d < (uint32_t)kTopBins; ++d) {
-      const unsigned long long c = top16[d];
-      if (k < cum + c) break;
-      cum += c;
-    }
-    st->prefix[r] = (d + kTopBase) << 16;
-    st->rank[r] = k - cum;
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) hist[i] = 0;
-}
-
-__global__ void k_select_init(SelectState* __restrict__ st, unsigned long long* __restrict__ hist,
-                              SelectState init) {
-  if (threadIdx.x == 0) *st = init;
-  for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) hist[i] = 0;
-}
-
-// linear_quantile_unstable (src/metrics.rs:271-291) on selected order statistics.
-static double linear_quantile(double q, uint64_t len, const double* vals, int lo_slot,
-                              int hi_slot, float scale) {
+// linear_quantile_unstable (src/metrics.rs:271-291) on the selected values.
+static double linear_quantile(double q, uint64_t len, double lower_value, double upper_value,
+                              float scale) {
   const double position = q * (double)(len - 1);
   const uint64_t lower = (uint64_t)std::floor(position);
   const double fraction = position - (double)lower;
-  const double lower_value = vals[lo_slot];
-  const double upper_value = vals[hi_slot];
   const double quantile_value = lower_value + fraction * (upper_value - lower_value);
   return (double)((float)quantile_value * scale);
 }
 
-QuantileResult device_quantiles(Engine& e, const float* d_map, uint64_t n, uint64_t count,
-                                float scale_f32, hipStream_t s, const uint32_t* top16) {
-  static const double qs[5] = {0.5, 0.1, 0.25, 0.75, 0.9};
-  SelectState init{};
-  int lo_slot[5], hi_slot[5];
-  auto slot_of = [&](unsigned long long rank) {
-    for (uint32_t i = 0; i < init.nranks; ++i)
-      if (init.rank[i] == rank) return (int)i;
-    init.rank[init.nranks] = rank;
-    init.prefix[init.nranks] = 0;
-    return (int)init.nranks++;
-  };
-  for (int i = 0; i < 5; ++i) {
-    const double position = qs[i] * (double)(count - 1);
-    const uint64_t lower = (uint64_t)std::floor(position);
-    const uint64_t upper = (uint64_t)std::ceil(position);
-    lo_slot[i] = slot_of(lower);
-    hi_slot[i] = upper == lower ? lo_slot[i] : slot_of(upper);
-  }
-  char* scratch = static_cast<char*>(
-      e.scratch_bytes(sizeof(SelectState) + kMaxRanks * 256 * sizeof(unsigned long long) + 256, 2));
-  SelectState* d_st = reinterpret_cast<SelectState*>(scratch);
-  unsigned long long* d_hist = reinterpret_cast<unsigned long long*>(scratch + 256);
-  k_select_init<<<1, 256, 0, s>>>(d_st, d_hist, init);
-  unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
-  int first = 24;
-  if (top16) {  // the top 16 bits are already counted: two 8-bit passes remain
-    BM_LAUNCH("quantile_pick", k_top16_pick, 1, 256, 0, s, d_st, top16, d_hist);
-    first = 8;
-  }
-  for (int shift = first; shift >= 0; shift -= 8) {
-    BM_LAUNCH("quantile_hist", k_digit_hist, grid, 256, 0, s, d_map, n, d_st, shift, d_hist);
-    BM_LAUNCH("quantile_pick", k_digit_pick, 1, 256, 0, s, d_st, shift, d_hist);
-  }
-  SelectState out{};
-  BM_HIP(hipMemcpyAsync(&out, d_st, sizeof out, hipMemcpyDeviceToHost, s));
-  BM_HIP(hipStreamSynchronize(s));
-  double vals[kMaxRanks];
-  for (uint32_t i = 0; i < out.nranks; ++i) {
-    float f;
-    std::memcpy(&f, &out.prefix[i], 4);
-    vals[i] = (double)f;
-  }
+QuantileResult quantiles_from(const SelectState& sel, uint64_t count, float scale_f32) {
   QuantileResult r;
-  r.median = linear_quantile(qs[0], count, vals, lo_slot[0], hi_slot[0], scale_f32);
-  r.p10 = linear_quantile(qs[1], count, vals, lo_slot[1], hi_slot[1], scale_f32);
-  r.p25 = linear_quantile(qs[2], count, vals, lo_slot[2], hi_slot[2], scale_f32);
-  r.p75 = linear_quantile(qs[3], count, vals, lo_slot[3], hi_slot[3], scale_f32);
-  r.p90 = linear_quantile(qs[4], count, vals, lo_slot[4], hi_slot[4], scale_f32);
+  if (count == 0 || sel.nranks != (uint32_t)kMaxRanks) return r;
+  double v[kMaxRanks];
+  for (int i = 0; i < kMaxRanks; ++i) {
+    float f;
+    std::memcpy(&f, &sel.prefix[i], 4);
+    v[i] = (double)f;
+  }
+  double* out[5] = {&r.median, &r.p10, &r.p25, &r.p75, &r.p90};
+  for (int q = 0; q < 5; ++q)
+    *out[q] = linear_quantile(h_qs[q], count, v[2 * q], v[2 * q + 1], scale_f32);
   return r;
 }
 

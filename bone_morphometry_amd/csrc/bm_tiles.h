@@ -26,6 +26,7 @@ constexpr int kChunkMap = 2048;             // 64-entry list chunks mapped to th
 constexpr int kMaxNeighbours = 512;         // tiles in reach kept per tile
 constexpr uint32_t kGatherMaxRadius = 128;  // beyond this the scatter fill is used
 constexpr int kDiamLds = 2048;              // diameters the clean-up keeps in LDS
+constexpr uint32_t kOverflowBlocks = 64;    // blocks per item of the fill fallbacks
 
 // Radius thresholds of the prefix counts: (2j + 1)^2.
 inline __host__ __device__ uint32_t kPrefR2(int j) { return (uint32_t)(2 * j + 1) * (2 * j + 1); }

@@ -98,15 +98,22 @@ __device__ uint32_t block_exclusive_scan_buckets(uint32_t* s_cnt, uint32_t* s_tm
 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(512)
-k_ridges_tiled(const uint32_t* __restrict__ d2, uint32_t W, uint32_t H, uint32_t D, uint32_t TX,
-               uint32_t TY, uint32_t TZ, const uint32_t* __restrict__ tmpl, uint32_t bound,
-               PhaseStatus* __restrict__ st, uint32_t* __restrict__ lists,
-               TileMeta* __restrict__ meta, unsigned long long* __restrict__ masks) {
+k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+               uint32_t nitems) {
   __shared__ uint32_t s_d2[10 * 10 * 10];
   __shared__ uint32_t s_cnt[kBuckets];
   __shared__ uint32_t s_tmp[16];
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t* __restrict__ d2 = I.d2;
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ, bound = I.bound;
+  const uint32_t* __restrict__ tmpl = I.tmpl;
+  PhaseStatus* __restrict__ st = I.status;
+  uint32_t* __restrict__ lists = I.lists;
+  TileMeta* __restrict__ meta = I.meta;
+  unsigned long long* __restrict__ masks = I.masks;
   if (!st->has_site) return;
-  const uint32_t t = xcd_tile(blockIdx.x, TX * TY * TZ);
+  const uint32_t t = xcd_tile(vb, TX * TY * TZ);
   if (t >= TX * TY * TZ) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
@@ -248,12 +255,12 @@ __device__ __forceinline__ bool entry_candidate(uint32_t ent, int ux, int uy, in
   return ex * ex + ey * ey + ez * ez <= r2;
 }
 
-__global__ void __launch_bounds__(512)
-k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t TZ,
-              const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
-              const unsigned long long* __restrict__ masks, PhaseStatus* __restrict__ st,
-              uint32_t* __restrict__ M, uint32_t* __restrict__ overflow, uint32_t cap,
-              uint32_t diag, unsigned long long* __restrict__ diag_buf) {
+// kDiag: work counters and phase clocks into the item's diag buffer (a
+// separate instantiation, so the counters cost nothing otherwise).
+template <bool kDiag>
+__device__ __forceinline__ void
+k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                uint32_t nitems) {
   __shared__ uint32_t s_pos[kFillCap], s_r2[kFillCap];  // one round, sorted by bucket
   __shared__ uint32_t s_cnt[kBuckets];                  // histogram, then cursors
   __shared__ uint32_t s_start[kBuckets + 1];            // bucket starts over all rounds
@@ -265,10 +272,22 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   __shared__ uint32_t s_tmp[16];
   __shared__ uint32_t s_nn, s_over, s_pending, s_nst;
   __shared__ unsigned long long s_tgt[8];
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ;
+  const uint32_t* __restrict__ lists = I.lists;
+  const TileMeta* __restrict__ meta = I.meta;
+  const unsigned long long* __restrict__ masks = I.masks;
+  PhaseStatus* __restrict__ st = I.status;
+  uint32_t* __restrict__ M = I.h;  // the Y-pass buffer is free once the Z pass has run
+  uint32_t* __restrict__ overflow = I.overflow;
+  const uint32_t cap = I.fill_cap;
+  constexpr bool diag = kDiag;
+  unsigned long long* __restrict__ diag_buf = I.diag;
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
   unsigned long long clk_p3 = 0, clk_p4 = 0;
-  const uint32_t t = xcd_tile(blockIdx.x, TX * TY * TZ);
+  const uint32_t t = xcd_tile(vb, TX * TY * TZ);
   if (t >= TX * TY * TZ) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
@@ -587,7 +606,7 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   // Work counters and phase clocks (diagnostics only): plain stores into this
   // block's slot, summed afterwards by k_reduce_diag.
   if (diag && lane == 0) {
-    unsigned long long* slot = diag_buf + ((size_t)blockIdx.x * 8 + wv) * 16;
+    unsigned long long* slot = diag_buf + ((size_t)vb * 8 + wv) * 16;
     slot[0] = 1;  // waves that reached the end
     slot[1] = tests;
     slot[2] = scanned;
@@ -607,10 +626,24 @@ k_fill_gather(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint
   }
 }
 
+__global__ void __launch_bounds__(512)
+k_fill_gather(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+              uint32_t nitems) {
+  k_fill_gather_t<false>(items, starts, nitems);
+}
+__global__ void __launch_bounds__(512)
+k_fill_gather_diag(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                   uint32_t nitems) {
+  k_fill_gather_t<true>(items, starts, nitems);
+}
+
 // Sum of the per-wave diagnostic slots (one block).
-__global__ void __launch_bounds__(256)
-k_reduce_diag(const unsigned long long* __restrict__ diag_buf, uint64_t nslots,
-              PhaseStatus* __restrict__ st) {
+__global__ void __launch_bounds__(256) k_reduce_diag(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[blockIdx.x];
+  if (!(I.flags & kItemDiag)) return;
+  const unsigned long long* __restrict__ diag_buf = I.diag;
+  const uint64_t nslots = (uint64_t)((I.ntiles + kXcds - 1) / kXcds * kXcds) * 8;
+  PhaseStatus* __restrict__ st = I.status;
   __shared__ unsigned long long s[16];
   if (threadIdx.x < 16) s[threadIdx.x] = 0;
   __syncthreads();
@@ -624,17 +657,23 @@ k_reduce_diag(const unsigned long long* __restrict__ diag_buf, uint64_t nslots,
 
 // Tiles whose candidate set did not fit in LDS: unsorted exact maximum.
 __global__ void __launch_bounds__(512)
-k_fill_overflow(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t TZ,
-                const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
-                const PhaseStatus* __restrict__ st, uint32_t* __restrict__ M,
-                const uint32_t* __restrict__ overflow) {
+k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                uint32_t nitems) {
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ;
+  const uint32_t* __restrict__ lists = I.lists;
+  const TileMeta* __restrict__ meta = I.meta;
+  const PhaseStatus* __restrict__ st = I.status;
+  uint32_t* __restrict__ M = I.h;
+  const uint32_t* __restrict__ overflow = I.overflow;
   if (!st->has_site) return;
   const uint32_t count = overflow[0];
   const uint32_t max_r2 = st->max_d2;
   const int R = (int)ceil_sqrt_u32(max_r2);
   const int K = (R + 7) / 8;
   const uint32_t gshift = bucket_shift(max_r2);
-  for (uint32_t oi = blockIdx.x; oi < count; oi += gridDim.x) {
+  for (uint32_t oi = vb; oi < count; oi += kOverflowBlocks) {
     const uint32_t t = overflow[1 + oi];
     const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
     const int vx = (int)tx * 8 + (int)(threadIdx.x & 7), vy = (int)ty * 8 + (int)((threadIdx.x >> 3) & 7),
@@ -667,13 +706,19 @@ k_fill_overflow(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, ui
 // Scatter fallback for very large radii: one wavefront per ridge, lanes over
 // the (2R+1)^2 (y, x) plane of each z, atomic max.
 __global__ void __launch_bounds__(256)
-k_fill_scatter_lists(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
-                     const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
-                     const PhaseStatus* __restrict__ st, uint32_t* __restrict__ M) {
+k_fill_scatter_lists(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                     uint32_t nitems) {
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
+  const uint32_t* __restrict__ lists = I.lists;
+  const TileMeta* __restrict__ meta = I.meta;
+  const PhaseStatus* __restrict__ st = I.status;
+  uint32_t* __restrict__ M = I.h;
   if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t gw = (vb * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nw = (kOverflowBlocks * blockDim.x) >> 6;
   for (uint32_t t = gw; t < ntiles; t += nw) {
     const TileMeta mt = meta[t];
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
@@ -700,10 +745,16 @@ k_fill_scatter_lists(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t T
 }
 
 __global__ void __launch_bounds__(256)
-k_zero_if_scatter(uint32_t* __restrict__ M, uint64_t n, const PhaseStatus* __restrict__ st) {
+k_zero_if_scatter(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                  uint32_t nitems) {
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const PhaseStatus* __restrict__ st = I.status;
   if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) M[i] = 0;
+  uint32_t* __restrict__ M = I.h;
+  const uint64_t n = I.n;
+  const uint64_t stride = (uint64_t)kOverflowBlocks * blockDim.x;
+  for (uint64_t i = (uint64_t)vb * blockDim.x + threadIdx.x; i < n; i += stride) M[i] = 0;
 }
 
 // Total ridge count (diagnostics) from the tile headers.
@@ -737,16 +788,24 @@ k_lists_to_flags(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, u
 constexpr uint32_t kOut = 0xFFFFFFFFu;  // outside the array: never zero, never summed
 
 __global__ void __launch_bounds__(512)
-k_cleanup_tiled(const uint32_t* __restrict__ M, const unsigned long long* __restrict__ masks,
-                const float* __restrict__ diam, const PhaseStatus* __restrict__ st, uint32_t W,
-                uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, uint32_t ntiles,
-                float scale_f32, int lsb_exp, float* __restrict__ map_out,
-                StatsPartial* __restrict__ partials, uint32_t* __restrict__ top16) {
+k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                uint32_t nitems) {
   __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
   __shared__ uint32_t s_zero[12 * 12];    // per (z, y) row: bit x set where M == 0
   __shared__ uint32_t s_mark[10 * 10];    // per row of the 1-halo box: surface bits
   __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
   __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  const uint32_t* __restrict__ M = I.h;
+  const unsigned long long* __restrict__ masks = I.masks;
+  const float* __restrict__ diam = I.diam;
+  const PhaseStatus* __restrict__ st = I.status;
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
+  const float scale_f32 = I.scale;
+  const int lsb_exp = I.lsb;
+  float* __restrict__ map_out = (I.flags & kItemMap) ? I.map : nullptr;
+  uint32_t* __restrict__ top16 = (I.flags & kItemQuantiles) ? I.top16 : nullptr;
   if (top16)
     for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_top[i] = 0;
   unsigned long long cnt = 0;
@@ -781,9 +840,9 @@ k_cleanup_tiled(const uint32_t* __restrict__ M, const unsigned long long* __rest
   // Block b walks tiles j = b/8, b/8 + G/8, ... of its XCD's contiguous range
   // (G blocks, a multiple of 8), so neighbouring tiles share that XCD's L2.
   const uint32_t per = (ntiles + kXcds - 1) / kXcds;
-  const uint32_t xbase = (blockIdx.x % kXcds) * per, xend = min(xbase + per, ntiles);
-  const uint32_t jstep = gridDim.x / kXcds;
-  const uint32_t t0 = xbase + blockIdx.x / kXcds;
+  const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, ntiles);
+  const uint32_t jstep = I.cblocks / kXcds;
+  const uint32_t t0 = xbase + vb / kXcds;
   if (valid_phase && t0 < xend) fetch(t0);
   for (uint32_t t = t0; t < xend && valid_phase; t += jstep) {
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
@@ -867,7 +926,7 @@ k_cleanup_tiled(const uint32_t* __restrict__ M, const unsigned long long* __rest
     for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x)
       if (s_top[i]) atomicAdd(&top16[i], s_top[i]);
   }
-  block_reduce_store(cnt, s, s2, mxb, flags, partials + blockIdx.x);
+  block_reduce_store(cnt, s, s2, mxb, flags, I.partials + vb);
 }
 
 }  // namespace bm
