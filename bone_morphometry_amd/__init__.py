@@ -10,13 +10,21 @@ interface for this path (bone-morphometry 0.7.0-rc.3):
   analyze_thickness_spacing           src/metrics.rs:64-132 (thickness/spacing part)
   FeatureMode                         src/metrics.rs:12-24
 
+and two forms with no counterpart in the reference:
+
+  Batch                               many device-resident analyze calls as one
+                                      plan, replayed from a hipGraph
+  Server                              request loop with HBM-budget admission and
+                                      continuous batching
+  autotune / tuning_load              per-shape-class kernel tunables (JSON cache)
+
 Every call runs on the GPU through libbm.so; errors raise BmError carrying the
 reference's message.
 """
-from .api import (BinaryVolume, BmError, DistributionStats, FeatureMode, Phase,  # noqa: F401
-                  ThicknessSpacing, analyze_thickness_spacing,
-                  euclidean_distance_transform_squared, local_thickness,
-                  local_thickness_stages)
+from .api import (Batch, BinaryVolume, BmError, DistributionStats, FeatureMode,  # noqa: F401
+                  Phase, Server, ThicknessSpacing, analyze_thickness_spacing,
+                  autotune, euclidean_distance_transform_squared, local_thickness,
+                  local_thickness_stages, tuning_load)
 from . import synth  # noqa: F401
 
 __version__ = "0.1.0"

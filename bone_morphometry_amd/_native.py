@@ -22,6 +22,10 @@ EXPORTS = (
     "bm_analyze_thickness_spacing", "bm_euclidean_distance_transform_squared_device",
     "bm_local_thickness_device", "bm_analyze_thickness_spacing_device",
     "bm_local_thickness_stages", "bm_profile_enable", "bm_profile_read",
+    "bm_batch_create", "bm_batch_run", "bm_batch_device_bytes", "bm_batch_destroy",
+    "bm_autotune", "bm_tuning_load",
+    "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_stats_get",
+    "bm_server_destroy",
 )
 
 
@@ -41,6 +45,31 @@ class BmThicknessSpacing(ctypes.Structure):
     _fields_ = [("thickness", BmDistributionStats), ("spacing", BmDistributionStats),
                 ("bone_volume", ctypes.c_double), ("total_volume", ctypes.c_double),
                 ("bone_volume_fraction", ctypes.c_double)]
+
+
+class BmBatchItem(ctypes.Structure):
+    _fields_ = [("d_bone", ctypes.c_void_p), ("d_roi", ctypes.c_void_p),
+                ("width", ctypes.c_uint64), ("height", ctypes.c_uint64),
+                ("depth", ctypes.c_uint64), ("voxel_spacing", ctypes.c_double * 3),
+                ("feature_mode", ctypes.c_int)]
+
+
+class BmBatchResult(ctypes.Structure):
+    _fields_ = [("metrics", BmThicknessSpacing), ("status", ctypes.c_int),
+                ("message", ctypes.c_char * 252)]
+
+
+class BmRequestTiming(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("start_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("batch_size", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class BmServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("requests_done", ctypes.c_uint64),
+                ("max_batch", ctypes.c_uint64), ("sum_batch", ctypes.c_uint64),
+                ("peak_device_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
+                ("slots", ctypes.c_uint32), ("busy_s", ctypes.c_double)]
 
 
 class BmKernelTime(ctypes.Structure):
@@ -79,6 +108,20 @@ def lib():
             ctypes.POINTER(BmThicknessSpacing), VP]
         L.bm_local_thickness_stages.argtypes = [VOL, I, P, P, P, P, ctypes.POINTER(U64),
                                                 ctypes.POINTER(U64)]
+        ITEMS = ctypes.POINTER(BmBatchItem)
+        L.bm_batch_create.argtypes = [ITEMS, U64, I, ctypes.POINTER(VP)]
+        L.bm_batch_run.argtypes = [VP, ctypes.POINTER(BmBatchResult)]
+        L.bm_batch_device_bytes.argtypes = [VP, ctypes.POINTER(U64)]
+        L.bm_batch_destroy.argtypes = [VP]
+        L.bm_autotune.argtypes = [ITEMS, U64, ctypes.c_char_p, I]
+        L.bm_tuning_load.argtypes = [ctypes.c_char_p]
+        L.bm_server_create.argtypes = [U64, I, I, ctypes.POINTER(VP)]
+        L.bm_server_submit.argtypes = [VP, P, P, U64, U64, U64, ctypes.POINTER(ctypes.c_double),
+                                       I, ctypes.POINTER(U64)]
+        L.bm_server_poll.argtypes = [VP, U64, I, ctypes.POINTER(BmBatchResult),
+                                     ctypes.POINTER(BmRequestTiming), ctypes.POINTER(I)]
+        L.bm_server_stats_get.argtypes = [VP, ctypes.POINTER(BmServerStats)]
+        L.bm_server_destroy.argtypes = [VP]
         L.bm_profile_enable.argtypes = [I]
         L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
         for name in EXPORTS:

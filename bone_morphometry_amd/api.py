@@ -160,3 +160,126 @@ def analyze_thickness_spacing_device(d_bone_ptr, d_roi_ptr, width, height, depth
         d_bone_ptr, d_roi_ptr, width, height, depth, sp, int(feature_mode), ctypes.byref(out),
         stream))
     return ThicknessSpacing._from_c(out)
+
+
+# ---------------------------------------------------------------------------
+# Batched and serving forms (include/bm.h: bm_batch_*, bm_server_*).
+
+def _batch_items(items):
+    """items: (d_bone_ptr, d_roi_ptr, width, height, depth, voxel_spacing, feature_mode)."""
+    arr = (_native.BmBatchItem * len(items))()
+    for i, (b, r, w, h, d, sp, mode) in enumerate(items):
+        arr[i].d_bone, arr[i].d_roi = b, r
+        arr[i].width, arr[i].height, arr[i].depth = int(w), int(h), int(d)
+        for k in range(3):
+            arr[i].voxel_spacing[k] = float(sp[k])
+        arr[i].feature_mode = int(mode)
+    return arr
+
+
+def _result(r):
+    if r.status != _native.BM_OK:
+        return BmError(r.status, r.message.decode())
+    return ThicknessSpacing._from_c(r.metrics)
+
+
+class Batch:
+    """Many device-resident analyze() calls as one plan (one launch per stage),
+    optionally captured once into a hipGraph.  run() returns, per volume, a
+    ThicknessSpacing or the BmError the one-volume call would raise."""
+
+    def __init__(self, items, use_graph=True):
+        self._items = _batch_items(items)
+        self.count = len(items)
+        self._h = ctypes.c_void_p()
+        _check(_native.lib().bm_batch_create(self._items, self.count, 1 if use_graph else 0,
+                                             ctypes.byref(self._h)))
+        self._res = (_native.BmBatchResult * self.count)()
+
+    def run(self):
+        _check(_native.lib().bm_batch_run(self._h, self._res))
+        return [_result(self._res[i]) for i in range(self.count)]
+
+    def device_bytes(self):
+        v = ctypes.c_uint64()
+        _check(_native.lib().bm_batch_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._h:
+            _check(_native.lib().bm_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(items, cache_path, verbose=False):
+    """Times the tunables on the largest volume of each shape class of items
+    (device-resident, as for Batch) and writes them to the JSON cache."""
+    arr = _batch_items(items)
+    _check(_native.lib().bm_autotune(arr, len(items), str(cache_path).encode(),
+                                     1 if verbose else 0))
+
+
+def tuning_load(cache_path):
+    _check(_native.lib().bm_tuning_load(str(cache_path).encode()))
+
+
+class Server:
+    """The request loop of the serving form: submit() host-resident volumes,
+    poll() their results with per-request timestamps (seconds since the
+    server started).  The arrays passed to submit() are kept referenced until
+    their request is polled done."""
+
+    def __init__(self, budget_bytes, max_batch=32, slots=2):
+        self._h = ctypes.c_void_p()
+        _check(_native.lib().bm_server_create(int(budget_bytes), int(max_batch), int(slots),
+                                              ctypes.byref(self._h)))
+        self._keep = {}
+
+    def submit(self, bone, roi, width, height, depth, voxel_spacing,
+               feature_mode=FeatureMode.Standard):
+        bone = np.ascontiguousarray(bone, dtype=np.uint8).reshape(-1)
+        roi = np.ascontiguousarray(roi, dtype=np.uint8).reshape(-1)
+        sp = (ctypes.c_double * 3)(*[float(v) for v in voxel_spacing])
+        t = ctypes.c_uint64()
+        _check(_native.lib().bm_server_submit(self._h, bone.ctypes.data, roi.ctypes.data,
+                                              int(width), int(height), int(depth), sp,
+                                              int(feature_mode), ctypes.byref(t)))
+        self._keep[t.value] = (bone, roi)
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """None while running, else (ThicknessSpacing or BmError, timing dict)."""
+        res = _native.BmBatchResult()
+        tim = _native.BmRequestTiming()
+        done = ctypes.c_int(0)
+        _check(_native.lib().bm_server_poll(self._h, int(ticket), 1 if wait else 0,
+                                            ctypes.byref(res), ctypes.byref(tim),
+                                            ctypes.byref(done)))
+        if not done.value:
+            return None
+        self._keep.pop(ticket, None)
+        return _result(res), dict(submit_s=tim.submit_s, start_s=tim.start_s, done_s=tim.done_s,
+                                  batch_size=tim.batch_size)
+
+    def stats(self):
+        s = _native.BmServerStats()
+        _check(_native.lib().bm_server_stats_get(self._h, ctypes.byref(s)))
+        return {f[0]: getattr(s, f[0]) for f in s._fields_}
+
+    def close(self):
+        if self._h:
+            _check(_native.lib().bm_server_destroy(self._h))
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -18,10 +18,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
 
 #include "../../include/bm.h"
+#include "bm_analyze.h"
 #include "bm_engine.h"
+#include "bm_serve.h"
 
 namespace bm {
 
@@ -80,7 +84,7 @@ static std::string shape_mismatch(const BinaryVolume& a, const BinaryVolume& b) 
 }
 
 // Limits of the device path (u16 row distances, u32 voxel indices).
-static Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
+Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
   if (w < 2 || h < 2 || d < 2)
     throw Error(fmt("all dimensions must be at least 2 for 3-D morphometry; got %llux%llux%llu",
                     (unsigned long long)w, (unsigned long long)h, (unsigned long long)d));
@@ -259,15 +263,20 @@ static bm_distribution_stats distribution_stats(const StatsResult& r, const Quan
 // bone and ROI.  Both phases are derived on the fly from (bone, roi)
 // (src/volume.rs:101); the map of a phase is only materialised when quantiles
 // are requested.
-static void check_spacing(const double spacing[3]) {
+void check_spacing(const double spacing[3]) {
   for (int i = 0; i < 3; ++i)
     if (!std::isfinite(spacing[i]) || spacing[i] <= 0.0)
       throw Error("voxel spacings must be finite and positive; got " + spacing_list(spacing));
 }
 
+void check_mode(int mode) {
+  if (mode < BM_MODE_BONEJ || mode > BM_MODE_EXPERIMENTAL)
+    throw Error(fmt("unknown feature mode %d", mode));
+}
+
 // The two items (bone, marrow) of one analyze call.
-static void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims,
-                          const double spacing[3], int mode, std::vector<ItemSpec>& out) {
+void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims,
+                   const double spacing[3], int mode, std::vector<ItemSpec>& out) {
   const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
   const float scale_f32 = (float)spacing[0];
   for (int ph = 0; ph < 2; ++ph) {
@@ -289,8 +298,8 @@ static void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dim
 
 // Results of analyze from the two items' results, with the error precedence
 // of src/metrics.rs:90-104.
-static void analyze_finish(const ItemResult& bone, const ItemResult& marrow, uint64_t n,
-                           const double spacing[3], int mode, bm_thickness_spacing* out) {
+void analyze_finish(const ItemResult& bone, const ItemResult& marrow, uint64_t n,
+                    const double spacing[3], int mode, bm_thickness_spacing* out) {
   const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
   const float scale_f32 = (float)spacing[0];
   const int lsb = stats_lsb_exp(scale_f32);
@@ -323,6 +332,7 @@ static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_ro
                            const Dims& dims, const double spacing[3], int mode,
                            bm_thickness_spacing* out, hipStream_t s) {
   check_spacing(spacing);
+  check_mode(mode);
   std::vector<ItemSpec> specs;
   analyze_items(d_bone, d_roi, dims, spacing, mode, specs);
   Plan& pl = e.plain_plan();
@@ -373,6 +383,12 @@ static int guarded(F&& f) {
     g_last_error = "unknown error";
     return BM_ERR_DEVICE;
   }
+}
+
+int capture_errors(void (*f)(void*), void* ctx, std::string* message) {
+  const int rc = guarded([&] { f(ctx); });
+  if (message) *message = g_last_error;
+  return rc;
 }
 
 static BinaryVolume from_c(const bm_volume* v) {
@@ -555,6 +571,129 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
               d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[13],
               d[14], d[15], so.status.overflow_tiles);
     }
+  });
+}
+
+// ---- batched form ----------------------------------------------------------
+struct bm_batch {
+  bm::Batch* impl;
+};
+
+static void to_c(const VolumeResult& v, bm_batch_result* out) {
+  std::memset(out, 0, sizeof *out);
+  out->metrics = v.metrics;
+  out->status = v.status;
+  std::strncpy(out->message, v.message.c_str(), sizeof out->message - 1);
+}
+
+int bm_batch_create(const bm_batch_item* items, uint64_t count, int use_graph, bm_batch** out) {
+  return guarded([&] {
+    if (!out) throw Error("null output");
+    *out = nullptr;
+    if (count && !items) throw Error("null items");
+    if (count == 0) throw Error("a batch needs at least one volume");
+    std::unique_ptr<bm_batch> b(new bm_batch{nullptr});
+    b->impl = new Batch(items, count, use_graph != 0);
+    *out = b.release();
+  });
+}
+
+int bm_batch_run(bm_batch* batch, bm_batch_result* results) {
+  return guarded([&] {
+    if (!batch || !batch->impl || !results) throw Error("null batch or results");
+    std::vector<VolumeResult> res;
+    batch->impl->run(res);
+    for (size_t i = 0; i < res.size(); ++i) to_c(res[i], &results[i]);
+  });
+}
+
+int bm_batch_device_bytes(const bm_batch* batch, uint64_t* bytes) {
+  return guarded([&] {
+    if (!batch || !batch->impl || !bytes) throw Error("null batch or output");
+    *bytes = batch->impl->device_bytes();
+  });
+}
+
+int bm_batch_destroy(bm_batch* batch) {
+  return guarded([&] {
+    if (!batch) return;
+    delete batch->impl;
+    delete batch;
+  });
+}
+
+int bm_autotune(const bm_batch_item* items, uint64_t count, const char* cache_path, int verbose) {
+  return guarded([&] {
+    if (!cache_path || (count && !items)) throw Error("null argument");
+    std::map<std::string, TuneSample> largest;
+    for (uint64_t i = 0; i < count; ++i) {
+      const bm_batch_item& it = items[i];
+      const Dims dims = device_dims(it.width, it.height, it.depth);
+      const std::string cls = shape_class(dims);
+      auto f = largest.find(cls);
+      if (f == largest.end() || f->second.dims.n() < dims.n())
+        largest[cls] = TuneSample{it.d_bone, it.d_roi, dims};
+    }
+    std::vector<TuneSample> samples;
+    for (auto& kv : largest) samples.push_back(kv.second);
+    autotune(samples, cache_path, verbose != 0);
+  });
+}
+
+int bm_tuning_load(const char* cache_path) {
+  return guarded([&] {
+    if (!cache_path) throw Error("null path");
+    tuning_load(cache_path);
+  });
+}
+
+// ---- serving form -------------------------------------------------------------
+struct bm_server {
+  bm::Server* impl;
+};
+
+int bm_server_create(uint64_t budget_bytes, int max_batch, int slots, bm_server** out) {
+  return guarded([&] {
+    if (!out) throw Error("null output");
+    *out = nullptr;
+    if (budget_bytes == 0) throw Error("the HBM budget must be positive");
+    std::unique_ptr<bm_server> s(new bm_server{nullptr});
+    s->impl = new Server(budget_bytes, max_batch, slots);
+    *out = s.release();
+  });
+}
+
+int bm_server_submit(bm_server* server, const uint8_t* bone, const uint8_t* roi, uint64_t width,
+                     uint64_t height, uint64_t depth, const double voxel_spacing[3],
+                     int feature_mode, uint64_t* ticket) {
+  return guarded([&] {
+    if (!server || !server->impl || !ticket || !voxel_spacing) throw Error("null argument");
+    *ticket = server->impl->submit(bone, roi, width, height, depth, voxel_spacing, feature_mode);
+  });
+}
+
+int bm_server_poll(bm_server* server, uint64_t ticket, int wait, bm_batch_result* result,
+                   bm_request_timing* timing, int* done) {
+  return guarded([&] {
+    if (!server || !server->impl || !done) throw Error("null argument");
+    VolumeResult v;
+    *done = server->impl->poll(ticket, wait != 0, &v, timing) ? 1 : 0;
+    if (*done && result) to_c(v, result);
+  });
+}
+
+int bm_server_stats_get(bm_server* server, bm_server_stats* out) {
+  return guarded([&] {
+    if (!server || !server->impl || !out) throw Error("null argument");
+    server->impl->stats(out);
+  });
+}
+
+int bm_server_destroy(bm_server* server) {
+  return guarded([&] {
+    if (!server) return;
+    delete server->impl;
+    delete server;
   });
 }
 

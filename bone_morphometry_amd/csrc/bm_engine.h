@@ -245,6 +245,9 @@ class Plan {
   size_t device_bytes() const { return bytes_; }
   // Device bytes a plan of these items needs (for admission control).
   static size_t bytes_for(const std::vector<ItemSpec>& specs);
+  // Allocates the descriptor and result space of plans of up to nitems items
+  // now, so that later builds of that size allocate nothing.
+  void reserve_items(size_t nitems);
 
  private:
   std::vector<ItemDesc> desc_;

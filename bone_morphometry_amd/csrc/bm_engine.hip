@@ -235,7 +235,7 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
   const size_t o_desc = 0;
   const size_t o_starts = align_up(nit * sizeof(ItemDesc), 256);
   const size_t o_res = align_up(o_starts + starts_.size() * sizeof(uint32_t), 256);
-  meta_.reserve(o_res + nit * sizeof(ItemResult));
+  reserve_items(nit);
   char* mb = static_cast<char*>(meta_.base());
   d_desc_ = reinterpret_cast<ItemDesc*>(mb + o_desc);
   d_starts_ = reinterpret_cast<uint32_t*>(mb + o_starts);
@@ -244,13 +244,19 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
   BM_HIP(hipMemcpyAsync(d_starts_, starts_.data(), starts_.size() * sizeof(uint32_t),
                         hipMemcpyHostToDevice, s));
   BM_HIP(hipStreamSynchronize(s));
+  results_.assign(nit, ItemResult{});
+}
+
+void Plan::reserve_items(size_t nit) {
+  const size_t o_starts = align_up(nit * sizeof(ItemDesc), 256);
+  const size_t o_res = align_up(o_starts + (size_t)kNumStages * (nit + 1) * sizeof(uint32_t), 256);
+  meta_.reserve(o_res + nit * sizeof(ItemResult));
   if (h_results_cap_ < nit) {
     if (h_results_) BM_HIP(hipHostFree(h_results_));
     h_results_ = nullptr;
     BM_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_results_), nit * sizeof(ItemResult)));
     h_results_cap_ = nit;
   }
-  results_.assign(nit, ItemResult{});
 }
 
 void Plan::launch(hipStream_t s) const {

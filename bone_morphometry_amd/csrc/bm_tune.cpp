@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "bm_analyze.h"
 #include "bm_engine.h"
 
 namespace bm {
@@ -38,14 +39,22 @@ std::string shape_class(const Dims& dims) {
   return buf;
 }
 
+// Architecture of the current device ("gfx950"), looked up once per device.
 static std::string device_arch() {
+  static std::mutex mu;
+  static std::map<int, std::string> cache;
   int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return "unknown";
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
   hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
-    return "unknown";
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
   std::string a = prop.gcnArchName;
   const size_t c = a.find(':');
-  return c == std::string::npos ? a : a.substr(0, c);
+  a = c == std::string::npos ? a : a.substr(0, c);
+  cache[dev] = a;
+  return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -68,7 +77,7 @@ struct JsonReader {
   }
   std::string str() {
     ws();
-    if (i >= s.size() || s[i] != '"') throw std::runtime_error("tuning cache: string expected");
+    if (i >= s.size() || s[i] != '"') throw Error("tuning cache: string expected");
     ++i;
     std::string out;
     while (i < s.size() && s[i] != '"') {
@@ -83,21 +92,21 @@ struct JsonReader {
     const char* p = s.c_str() + i;
     char* end = nullptr;
     const double v = std::strtod(p, &end);
-    if (end == p) throw std::runtime_error("tuning cache: number expected");
+    if (end == p) throw Error("tuning cache: number expected");
     i += (size_t)(end - p);
     return v;
   }
   // object of numbers
   std::map<std::string, double> flat() {
     std::map<std::string, double> m;
-    if (!eat('{')) throw std::runtime_error("tuning cache: object expected");
+    if (!eat('{')) throw Error("tuning cache: object expected");
     if (eat('}')) return m;
     do {
       const std::string k = str();
-      if (!eat(':')) throw std::runtime_error("tuning cache: ':' expected");
+      if (!eat(':')) throw Error("tuning cache: ':' expected");
       m[k] = num();
     } while (eat(','));
-    if (!eat('}')) throw std::runtime_error("tuning cache: '}' expected");
+    if (!eat('}')) throw Error("tuning cache: '}' expected");
     return m;
   }
 };
@@ -116,16 +125,16 @@ std::map<std::string, std::map<std::string, TuneEntry>> g_tune;
 void parse_cache(const std::string& text) {
   JsonReader r(text);
   g_tune.clear();
-  if (!r.eat('{')) throw std::runtime_error("tuning cache: object expected");
+  if (!r.eat('{')) throw Error("tuning cache: object expected");
   if (r.eat('}')) return;
   do {
     const std::string arch = r.str();
-    if (!r.eat(':') || !r.eat('{')) throw std::runtime_error("tuning cache: object expected");
+    if (!r.eat(':') || !r.eat('{')) throw Error("tuning cache: object expected");
     auto& classes = g_tune[arch];
     if (!r.eat('}')) {
       do {
         const std::string cls = r.str();
-        if (!r.eat(':')) throw std::runtime_error("tuning cache: ':' expected");
+        if (!r.eat(':')) throw Error("tuning cache: ':' expected");
         const auto m = r.flat();
         TuneEntry e;
         auto get = [&](const char* k, uint32_t& dst) {
@@ -141,7 +150,7 @@ void parse_cache(const std::string& text) {
           if (kv.first.find("_ms") != std::string::npos) e.extra[kv.first] = kv.second;
         classes[cls] = e;
       } while (r.eat(','));
-      if (!r.eat('}')) throw std::runtime_error("tuning cache: '}' expected");
+      if (!r.eat('}')) throw Error("tuning cache: '}' expected");
     }
   } while (r.eat(','));
 }
@@ -221,12 +230,6 @@ void tuning_load(const std::string& path) {
 // The tuner.  For each sample (the largest volume of a class, device-resident
 // bone and ROI) it times, on the marrow phase (the slower one), every
 // candidate of each pass with the others at their defaults.
-struct TuneSample {
-  const uint8_t* bone;
-  const uint8_t* roi;
-  Dims dims;
-};
-
 static double time_kernel(Engine& e, const ItemSpec& sp, const char* kernel, int reps) {
   Plan& pl = e.plain_plan();
   hipStream_t s = e.stream();

@@ -138,6 +138,81 @@ int bm_local_thickness_stages(const bm_volume *volume, int phase, uint32_t *squa
                               uint8_t *ridge_flags, uint32_t *propagated, float *thickness,
                               uint64_t *ridge_count, uint64_t *fill_visits);
 
+/* ---- Batched form (no counterpart in the reference: many analyze() calls,
+ * src/metrics.rs:64-132, in one launch sequence).  The volumes are
+ * device-resident, of any shapes; each is analyzed exactly as by
+ * bm_analyze_thickness_spacing_device (same results, bit for bit).
+ * bm_batch_create lays out one plan for all of them (bone and marrow of every
+ * volume are items of the same launches) and, with use_graph, captures the
+ * launch sequence into a hipGraph once; bm_batch_run replays it and waits.
+ * Inputs may change between runs (same pointers and shapes). */
+typedef struct bm_batch_item {
+  const uint8_t *d_bone; /* device pointers, width*height*depth bytes each */
+  const uint8_t *d_roi;
+  uint64_t width, height, depth;
+  double voxel_spacing[3];
+  int feature_mode;
+} bm_batch_item;
+
+/* Per-volume outcome: status is what bm_analyze_thickness_spacing_device
+ * would return for the volume, message its bm_last_error() text. */
+typedef struct bm_batch_result {
+  bm_thickness_spacing metrics;
+  int status;
+  char message[252];
+} bm_batch_result;
+
+typedef struct bm_batch bm_batch;
+int bm_batch_create(const bm_batch_item *items, uint64_t count, int use_graph, bm_batch **out);
+int bm_batch_run(bm_batch *batch, bm_batch_result *results /* count entries */);
+int bm_batch_device_bytes(const bm_batch *batch, uint64_t *bytes);
+int bm_batch_destroy(bm_batch *batch);
+
+/* Kernel tunables per shape class: bm_autotune times the candidates on the
+ * largest volume of each class among items and writes the choices to the
+ * JSON cache at cache_path (merged with what it holds); bm_tuning_load makes
+ * a cache current.  BM_TUNE_CACHE names the cache read at first use. */
+int bm_autotune(const bm_batch_item *items, uint64_t count, const char *cache_path, int verbose);
+int bm_tuning_load(const char *cache_path);
+
+/* ---- Serving form (no counterpart in the reference).  A server owns `slots`
+ * device regions of budget_bytes/slots each.  bm_server_submit queues a
+ * host-resident volume (bone and roi stay valid and unchanged until the
+ * request is polled done) and returns a ticket; a worker thread admits queued
+ * requests, first come first served, into the next batch of a free slot while
+ * the batch's device bytes fit the slot and it holds at most max_batch
+ * volumes, uploads them and launches the batch; batches of different slots
+ * overlap on the device.  bm_server_poll returns the result once done (with
+ * wait != 0 it blocks until then) and releases the request. */
+typedef struct bm_request_timing {
+  double submit_s; /* seconds since the server started */
+  double start_s;  /* its batch was launched */
+  double done_s;   /* its batch completed */
+  uint32_t batch_size;
+  uint32_t pad;
+} bm_request_timing;
+
+typedef struct bm_server_stats {
+  uint64_t batches;
+  uint64_t requests_done;
+  uint64_t max_batch;
+  uint64_t sum_batch;
+  uint64_t peak_device_bytes; /* largest sum of in-flight batches' bytes */
+  uint32_t max_inflight;      /* most batches in flight at once */
+  uint32_t slots;
+  double busy_s;              /* time with at least one batch in flight */
+} bm_server_stats;
+
+typedef struct bm_server bm_server;
+int bm_server_create(uint64_t budget_bytes, int max_batch, int slots, bm_server **out);
+int bm_server_submit(bm_server *server, const uint8_t *bone, const uint8_t *roi, uint64_t width,
+                     uint64_t height, uint64_t depth, const double voxel_spacing[3],
+                     int feature_mode, uint64_t *ticket);
+int bm_server_poll(bm_server *server, uint64_t ticket, int wait, bm_batch_result *result,
+                   bm_request_timing *timing, int *done);
+int bm_server_stats_get(bm_server *server, bm_server_stats *out);
+int bm_server_destroy(bm_server *server);
+
 /* Kernel timing for benchmarks and diagnostics (no counterpart in the
  * reference).  bm_profile_enable(1) brackets every kernel launch with HIP
  * events on its stream; bm_profile_read() waits for them and returns, per

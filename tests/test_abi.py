@@ -72,3 +72,16 @@ def test_unsupported_sizes_are_reported():
                                                                       None)
     assert rc == _native.BM_ERR_UNSUPPORTED
     assert b"exceeds the device path's limits" in _native.lib().bm_last_error()
+
+
+def test_tuning_cache_is_parsed_without_a_device(tmp_path):
+    good = tmp_path / "good.json"
+    good.write_text('{"gfx950": {"h384_d320": {"y_rows": 256, "y_halo": 32, "z_rows": 512, '
+                    '"z_halo": 64, "fill_cap": 512, "edt_y_ms": 0.1}}}')
+    bm.tuning_load(good)
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"gfx950": {"h384_d320": {"y_rows": }}}')
+    with pytest.raises(bm.BmError) as e:
+        bm.tuning_load(bad)
+    assert e.value.code == _native.BM_ERR_INVALID and "tuning cache" in str(e.value)
+    bm.tuning_load(tmp_path / "absent.json")  # no file: the defaults

@@ -1,0 +1,160 @@
+"""GPU tests of the batched and serving forms: every volume's result must be
+bit-identical to the one-volume call (bm_analyze_thickness_spacing_device) on
+the same input, with a hipGraph or without, alone or mixed with other
+requests, and whatever the tunables; per-volume errors are the one-volume
+call's errors."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import bone_morphometry_amd as bm
+from bone_morphometry_amd import _native, hip, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _volumes(seed, count, lo=24, hi=72):
+    """Small volumes of uneven shapes from the synthetic generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        w, h, d = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        bone = synth.volume(w, h, d, float(rng.uniform(1.5, 3.0)), float(rng.uniform(0.1, 0.35)),
+                            int(rng.integers(1, 2**31)), 0)
+        roi = synth.roi_cylinder(w, h, d, float(rng.uniform(0.7, 0.95)) * min(w, h))
+        out.append(dict(w=w, h=h, d=d, bone=bone, roi=roi,
+                        spacing=(float(rng.uniform(6, 20)),) * 3))
+    return out
+
+
+def _upload(vols):
+    bufs = []
+    for v in vols:
+        b = hip.DeviceBuffer(v["bone"].size)
+        r = hip.DeviceBuffer(v["roi"].size)
+        b.upload(v["bone"])
+        r.upload(v["roi"])
+        bufs.append((b, r))
+    return bufs
+
+
+def _items(vols, bufs, mode=bm.FeatureMode.Standard):
+    return [(b.value, r.value, v["w"], v["h"], v["d"], v["spacing"], mode)
+            for v, (b, r) in zip(vols, bufs)]
+
+
+def _plain(item):
+    b, r, w, h, d, sp, mode = item
+    try:
+        return bm.api.analyze_thickness_spacing_device(b, r, w, h, d, sp, mode)
+    except bm.BmError as e:
+        return e
+
+
+def _same(a, b):
+    """Bit-identical results (NaN quantiles compare equal), or equal errors."""
+    if isinstance(a, bm.BmError) or isinstance(b, bm.BmError):
+        return (isinstance(a, bm.BmError) and isinstance(b, bm.BmError) and str(a) == str(b)
+                and a.code == b.code)
+    for x, y in ((a.thickness, b.thickness), (a.spacing, b.spacing)):
+        for f in ("mean", "standard_deviation", "maximum", "median", "percentile_10",
+                  "percentile_25", "percentile_75", "percentile_90", "interquartile_range",
+                  "coefficient_of_variation", "count"):
+            u, v = getattr(x, f), getattr(y, f)
+            if not (u == v or (isinstance(u, float) and math.isnan(u) and math.isnan(v))):
+                return False
+    return (a.bone_volume, a.total_volume, a.bone_volume_fraction) == \
+        (b.bone_volume, b.total_volume, b.bone_volume_fraction)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_matches_one_volume_calls(use_graph):
+    vols = _volumes(11, 7)
+    bufs = _upload(vols)
+    items = _items(vols, bufs)
+    items[3] = items[3][:6] + (bm.FeatureMode.Classic,)  # quantiles off for one volume
+    ref = [_plain(it) for it in items]
+    batch = bm.Batch(items, use_graph=use_graph)
+    assert batch.device_bytes() > 0
+    for _ in range(2):  # the graph is replayed: every run starts from a clean state
+        got = batch.run()
+        assert all(_same(g, r) for g, r in zip(got, ref))
+    batch.close()
+
+
+def test_batch_reports_per_volume_errors():
+    vols = _volumes(12, 3)
+    bufs = _upload(vols)
+    items = _items(vols, bufs)
+    # an empty ROI, a bad spacing and a bad shape, among valid volumes
+    empty = hip.DeviceBuffer(vols[0]["roi"].size)
+    empty.upload(np.zeros_like(vols[0]["roi"]))
+    items.append((bufs[0][0].value, empty.value, vols[0]["w"], vols[0]["h"], vols[0]["d"],
+                  vols[0]["spacing"], bm.FeatureMode.Standard))
+    items.append(items[1][:5] + ((1.0, float("nan"), 1.0), bm.FeatureMode.Standard))
+    items.append((items[1][0], items[1][1], 1, 4, 4, (1.0, 1.0, 1.0), bm.FeatureMode.Standard))
+    got = bm.Batch(items).run()
+    ref = [_plain(it) for it in items]
+    assert isinstance(got[3], bm.BmError) and "ROI mask contains no included voxels" in str(got[3])
+    assert isinstance(got[4], bm.BmError) and "[1.0, NaN, 1.0]" in str(got[4])
+    assert isinstance(got[5], bm.BmError)
+    assert all(_same(g, r) for g, r in zip(got, ref))
+
+
+def test_server_result_alone_equals_result_in_mix():
+    vols = _volumes(13, 10)
+    srv = bm.Server(budget_bytes=4 << 30, max_batch=4, slots=2)
+    a = vols[0]
+    t = srv.submit(a["bone"], a["roi"], a["w"], a["h"], a["d"], a["spacing"])
+    alone, tim = srv.poll(t, wait=True)
+    assert tim["batch_size"] == 1 and tim["done_s"] >= tim["start_s"] >= tim["submit_s"]
+    tickets = [srv.submit(v["bone"], v["roi"], v["w"], v["h"], v["d"], v["spacing"])
+               for v in vols[1:5] + [a] + vols[5:]]
+    res = [srv.poll(t, wait=True) for t in tickets]
+    mixed = res[4][0]
+    assert _same(alone, mixed)
+    assert max(r[1]["batch_size"] for r in res) > 1  # they were batched together
+    bufs = _upload(vols)
+    ref = [_plain(it) for it in _items(vols, bufs)]
+    order = [1, 2, 3, 4, 0, 5, 6, 7, 8, 9]
+    assert all(_same(res[k][0], ref[order[k]]) for k in range(len(res)))
+    st = srv.stats()
+    assert st["requests_done"] == 11 and st["batches"] >= 3 and st["max_batch"] <= 4
+    srv.close()
+
+
+def test_server_rejects_requests_beyond_the_budget():
+    vols = _volumes(14, 1, 64, 64)
+    v = vols[0]
+    srv = bm.Server(budget_bytes=1 << 20, max_batch=4, slots=2)  # 512 KiB per slot
+    t = srv.submit(v["bone"], v["roi"], v["w"], v["h"], v["d"], v["spacing"])
+    res, _ = srv.poll(t, wait=True)
+    assert isinstance(res, bm.BmError) and res.code == _native.BM_ERR_UNSUPPORTED
+    t = srv.submit(v["bone"], v["roi"], v["w"], v["h"], v["d"], (0.0, 1.0, 1.0))
+    res, _ = srv.poll(t, wait=True)
+    assert isinstance(res, bm.BmError) and "finite and positive" in str(res)
+    srv.close()
+
+
+def test_autotune_writes_cache_and_keeps_results(tmp_path):
+    vols = _volumes(15, 3, 40, 140)
+    bufs = _upload(vols)
+    items = _items(vols, bufs)
+    before = [_plain(it) for it in items]
+    path = tmp_path / "tune.json"
+    bm.autotune(items, path)
+    cache = json.loads(path.read_text())
+    (arch, classes), = cache.items()
+    assert arch.startswith("gfx")
+    assert len(classes) >= 1
+    for entry in classes.values():
+        assert entry["y_rows"] in (128, 256, 512) and entry["z_rows"] in (128, 256, 512)
+        assert entry["fill_cap"] in (256, 512, 1024)
+    bm.tuning_load(path)
+    after = [_plain(it) for it in items]
+    assert all(_same(a, b) for a, b in zip(after, before))
+    batch = bm.Batch(items).run()
+    assert all(_same(a, b) for a, b in zip(batch, before))
+    bm.tuning_load(tmp_path / "absent.json")  # back to the defaults
