@@ -1,21 +1,31 @@
 """Benchmark of the exact squared EDT + local thickness (Tb.Th and Tb.Sp) path.
 
-One step = one bm_analyze_thickness_spacing_device() call on the 512^3
-workload (plain_512: sigma 4, BV/TV 0.15, z-cylinder ROI of diameter 460,
-10.5 um, Standard mode): both phases through EDT, ridges, sphere fill,
-clean-up, mask and statistics, with bone and ROI already resident in HBM.
+One JSON line.  The headline (metric/value) is the plain one-volume form: one
+step = one bm_analyze_thickness_spacing_device() call on the 512^3 workload
+(plain_512: sigma 4, BV/TV 0.15, z-cylinder ROI of diameter 460, 10.5 um,
+Standard mode): both phases through EDT, ridges, sphere fill, clean-up, mask,
+statistics and quantiles, with bone and ROI already resident in HBM.
 value = voxels / step time (Gvoxel/s, the volume counted once).
 
-The roofline object describes the kernel with the largest share of the timed
-region, from per-kernel HIP events recorded on the launching stream;
-algorithmic bytes per voxel of each kernel are listed in KERNEL_BYTES.  The
-CPU baseline runs the oracle (a C restatement of the reference algorithm,
-OpenMP) on the same workload on this host.
+Beside it:
+  batched  96 volumes of uneven shapes (axes U{64..384}) analyzed as one plan
+           replayed from a hipGraph, against 96 sequential one-volume calls on
+           the same resident inputs (Gvoxel/s of each, speed-up, equality).
+  serving  a seeded Poisson stream (150 requests/s for 60 s, lognormal sizes
+           of median 192^3) through the request loop with a 200 GB HBM budget:
+           p50/p99 latency (completion - submission) and Gvoxel/s.
+
+The roofline object describes the kernel with the largest share of the plain
+step, from per-kernel HIP events recorded on the launching stream; the
+algorithmic bytes per voxel of each kernel are listed in KERNEL_BYTES.  The CPU
+baseline runs the oracle (a C restatement of the reference algorithm, OpenMP)
+on a bounded sample of the plain workload on this host.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -23,16 +33,19 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per voxel of one launch (one phase), plus bytes per ridge.
+# Algorithmic HBM bytes per voxel of one launch over one item (one phase of a
+# volume); every launch of the plain step serves the bone and marrow items.
+# The EDT figures are the 22 B/voxel/phase of the algorithm (bone + ROI in,
+# an int32 distance out per pass); this build moves 18 (its X pass writes u16).
 KERNEL_BYTES = {
-    "edt_x": (6.0, 0.0),        # bone + ROI u8 in, int32 row distance out
-    "edt_y": (8.0, 0.0),        # int32 in, int32 out
-    "edt_z": (8.0, 0.0),        # int32 in, int32 out
-    "ridges": (4.0, 8.0),       # squared EDT in; (index, r^2) per ridge out
-    "fill_scatter": (4.0, 8.0),  # propagated map out; ridge list in
-    "fill_gather": (4.0, 8.0),
-    "cleanup_stats": (12.0, 0.0),  # propagated map + squared EDT in, map out
+    "edt_x": 6.0,           # bone + ROI u8 in, int32 row distance out
+    "edt_y": 8.0,           # int32 in, int32 out
+    "edt_z": 8.0,           # int32 in, int32 out
+    "ridges": 4.0,          # squared EDT in (ridge lists are per ridge, excluded)
+    "fill_gather": 4.0,     # propagated map out (ridge lists are per ridge, excluded)
+    "cleanup_stats": 12.0,  # propagated map and squared EDT in, thickness map out
 }
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_512.txt")
 
 
 def parse():
@@ -42,31 +55,58 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_512", choices=["plain_512", "plain_small"])
     ap.add_argument("--mode", default="standard", choices=["standard", "classic"])
+    ap.add_argument("--forms", default="plain,batched,serving",
+                    help="comma list of plain, batched, serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample-s", type=float, default=15.0,
+                    help="target seconds of CPU work for the baseline sample")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-rate", type=float, default=150.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     return ap.parse_args()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (volumes shard "
-              "across GPUs by whole volumes; see DESIGN.md). Run with --gpus 1.",
-              file=sys.stderr)
-        return 2
-    import numpy as np
-    import bone_morphometry_amd as bm
-    from bone_morphometry_amd import _native, hip, workloads
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    if _native.device_count() < 1:
-        print("no GPU visible", file=sys.stderr)
-        return 1
-    cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+
+def same_result(a, b):
+    import math
+    if isinstance(a, Exception) or isinstance(b, Exception):
+        return type(a) is type(b) and str(a) == str(b)
+    for x, y in ((a.thickness, b.thickness), (a.spacing, b.spacing)):
+        for f in ("mean", "standard_deviation", "maximum", "median", "percentile_10",
+                  "percentile_25", "percentile_75", "percentile_90", "count"):
+            u, v = getattr(x, f), getattr(y, f)
+            if not (u == v or (isinstance(u, float) and math.isnan(u) and math.isnan(v))):
+                return False
+    return a.bone_volume_fraction == b.bone_volume_fraction
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    try:
+        vals = {}
+        for line in open(TRAFFIC_FILE):
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+        return {k: float(vals[k]) for k in ("fill_gather_bytes_per_launch",
+                                            "edt_bytes_per_step") if k in vals}
+    except (OSError, ValueError):
+        return {}
+
+
+def plain_form(args, bm, native, hip, workloads, cpu_threads):
     t0 = time.time()
     wl = getattr(workloads, args.workload)(threads=cpu_threads)
-    print(f"[bench] generated {wl.name} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    log(f"generated {wl.name} in {time.time() - t0:.1f}s")
     mode = bm.FeatureMode.Standard if args.mode == "standard" else bm.FeatureMode.Classic
-
     d_bone = hip.DeviceBuffer(wl.n)
     d_roi = hip.DeviceBuffer(wl.n)
     d_bone.upload(wl.bone)
@@ -81,8 +121,8 @@ def main():
     for _ in range(args.warmup):
         res = step()
     hip.synchronize()
-    _native.profile_read()  # drop warm-up records
-    _native.profile_enable(True)
+    native.profile_read()  # drop warm-up records
+    native.profile_enable(True)
     e0, e1 = hip.Event(), hip.Event()
     hip.synchronize()
     e0.record(stream)
@@ -91,28 +131,30 @@ def main():
     e1.record(stream)
     hip.synchronize()
     total_ms = e0.elapsed_ms(e1)
-    _native.profile_enable(False)
-    kernels = _native.profile_read()
-    print(f"[bench] {args.steps} steps in {total_ms:.2f} ms", file=sys.stderr, flush=True)
-
+    native.profile_enable(False)
+    kernels = native.profile_read()
+    log(f"plain: {args.steps} steps in {total_ms:.2f} ms")
     ms_per_step = total_ms / args.steps
     gvox = wl.n / (ms_per_step * 1e-3) / 1e9
 
-    # Roofline of the kernel with the largest time share.
-    phases = 2
-    dom = max(kernels.items(), key=lambda kv: kv[1][1])
-    name, (launches, kms) = dom
+    # Roofline of the kernel with the largest time share (one launch serves
+    # both phases: 2 items of wl.n voxels).
+    name, (launches, kms) = max(kernels.items(), key=lambda kv: kv[1][1])
     avg_ms = kms / launches
-    per_vox, per_ridge = KERNEL_BYTES.get(name, (0.0, 0.0))
-    alg_bytes = per_vox * wl.n
+    per_vox = KERNEL_BYTES.get(name, 0.0)
+    alg_bytes = per_vox * wl.n * 2
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if per_vox else None
+    traffic = read_traffic()
     roofline = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                "traffic": None, "avg_launch_ms": avg_ms,
-                "algorithmic_bytes_per_launch": alg_bytes}
+                "traffic": traffic.get("fill_gather_bytes_per_launch") if name == "fill_gather"
+                else None,
+                "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes,
+                "note": "the sphere fill is bound by VALU/LDS issue, not HBM"
+                if name == "fill_gather" else None}
     edt = {k: kernels[k][1] / kernels[k][0] for k in ("edt_x", "edt_y", "edt_z") if k in kernels}
     edt_ms = sum(edt.values())
-    edt_gbs = 22.0 * wl.n / (edt_ms * 1e-3) / 1e9 if edt_ms else None
+    edt_gbs = 22.0 * 2 * wl.n / (edt_ms * 1e-3) / 1e9 if edt_ms else None
 
     cpu = None
     parity = None
@@ -120,15 +162,18 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle as orc
         orc.set_threads(cpu_threads)
+        # The sample is the whole volume: about 20 s of CPU work at 16 threads.
+        depth = wl.depth
         t1 = time.time()
-        ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, wl.depth,
-                                            wl.spacing, include_quantiles=(mode ==
-                                                                           bm.FeatureMode.Standard))
+        ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, depth,
+                                            wl.spacing, include_quantiles=(
+                                                mode == bm.FeatureMode.Standard))
         cpu_s = time.time() - t1
-        cpu = {"value": wl.n / cpu_s / 1e9, "unit": "Gvoxel/s", "cores": cpu_threads,
+        nvox = wl.width * wl.height * depth
+        cpu = {"value": nvox / cpu_s / 1e9, "unit": "Gvoxel/s", "cores": cpu_threads,
                "kind": "port",
-               "sample": f"the full {wl.name} workload, both phases, oracle (C/OpenMP), "
-                         f"{cpu_s:.1f} s"}
+               "sample": f"the full {wl.name} volume ({wl.width}x{wl.height}x{depth}), both "
+                         f"phases, oracle (C/OpenMP restatement), {cpu_s:.1f} s"}
         ok = True
         for g, r in ((res.thickness, ref.thickness), (res.spacing, ref.spacing)):
             ok &= g.count == r.count and g.maximum == r.maximum
@@ -139,15 +184,168 @@ def main():
             if mode == bm.FeatureMode.Standard:
                 ok &= g.median == r.median and g.percentile_90 == r.percentile_90
         parity = bool(ok)
+    return dict(wl=wl, res=res, ms_per_step=ms_per_step, gvox=gvox, roofline=roofline,
+                edt=edt, edt_gbs=edt_gbs, kernels=kernels, cpu=cpu, parity=parity)
 
+
+def batched_form(args, bm, hip, workloads, cpu_threads):
+    t0 = time.time()
+    vols = workloads.batched(threads=cpu_threads)
+    total = sum(v.n for v in vols)
+    log(f"generated 96 volumes ({total / 1e9:.3f} Gvoxel) in {time.time() - t0:.1f}s")
+    bufs = []
+    for v in vols:
+        b, r = hip.DeviceBuffer(v.n), hip.DeviceBuffer(v.n)
+        b.upload(v.bone)
+        r.upload(v.roi)
+        bufs.append((b, r))
+    mode = bm.FeatureMode.Standard
+    items = [(b.value, r.value, v.width, v.height, v.depth, v.spacing, mode)
+             for v, (b, r) in zip(vols, bufs)]
+    stream = hip.stream()
+
+    def plain_all():
+        out = []
+        for it in items:
+            out.append(bm.api.analyze_thickness_spacing_device(*it, stream=stream.value))
+        return out
+
+    ref = plain_all()
+    hip.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        ref = plain_all()
+    hip.synchronize()
+    plain_s = (time.perf_counter() - t1) / args.batch_steps
+    t1 = time.perf_counter()
+    batch = bm.Batch(items, use_graph=True)
+    create_s = time.perf_counter() - t1
+    got = batch.run()
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        got = batch.run()
+    batch_s = (time.perf_counter() - t1) / args.batch_steps
+    equal = all(same_result(a, b) for a, b in zip(got, ref))
+    out = {"workload": "96 volumes, axes U{64..384}, sigma U[2,6], BV/TV U[0.08,0.35], "
+                       "cylinder ROI 70-95% of min(w,h), spacing U[6,20] um, seed 4",
+           "voxels": total, "plain_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
+           "plain_gvox_per_s": total / plain_s / 1e9, "batched_gvox_per_s": total / batch_s / 1e9,
+           "speedup": plain_s / batch_s, "graph": batch.device_bytes() > 0,
+           "create_ms": create_s * 1e3, "device_gb": batch.device_bytes() / 1e9,
+           "tuning_cache": os.environ.get("BM_TUNE_CACHE"),
+           "results_identical_to_plain": bool(equal)}
+    batch.close()
+    log(f"batched: plain {plain_s * 1e3:.1f} ms, batched {batch_s * 1e3:.1f} ms")
+    return out
+
+
+def serving_form(args, bm, workloads, cpu_threads):
+    import numpy as np
+    t0 = time.time()
+    arrivals, pool = workloads.serving_stream(rate=args.serve_rate, duration=args.serve_seconds,
+                                              threads=cpu_threads)
+    log(f"serving: {len(arrivals)} arrivals, pool of {len(pool)} volumes "
+        f"(median {np.median([r.n for r in pool]) ** (1 / 3):.0f}^3) in {time.time() - t0:.1f}s")
+    srv = bm.Server(budget_bytes=int(args.serve_budget_gb * 1e9), max_batch=args.serve_max_batch,
+                    slots=args.serve_slots)
+    # warm-up: one request of each of a few sizes
+    for r in pool[:4]:
+        srv.poll(srv.submit(r.bone, r.roi, r.width, r.height, r.depth, r.spacing), wait=True)
+    tickets = []
+    done = []
+    lock = threading.Condition()
+    finished = [False]
+
+    def poller():
+        i = 0
+        while True:
+            with lock:
+                while i >= len(tickets) and not finished[0]:
+                    lock.wait(0.05)
+                if i >= len(tickets) and finished[0]:
+                    return
+                t, req = tickets[i]
+            res, tim = srv.poll(t, wait=True)
+            done.append((req, res, tim))
+            i += 1
+
+    th = threading.Thread(target=poller)
+    th.start()
+    start = time.perf_counter()
+    max_lag = 0.0
+    last_progress = start
+    for k, a in enumerate(arrivals):
+        now = time.perf_counter() - start
+        if a > now:
+            time.sleep(a - now)
+        lag = time.perf_counter() - start - a
+        max_lag = max(max_lag, lag)
+        r = pool[k % len(pool)]
+        t = srv.submit(r.bone, r.roi, r.width, r.height, r.depth, r.spacing)
+        with lock:
+            tickets.append((t, r))
+            lock.notify()
+        if time.perf_counter() - last_progress > 10:
+            last_progress = time.perf_counter()
+            log(f"serving: {k + 1} submitted, {len(done)} done")
+    with lock:
+        finished[0] = True
+        lock.notify()
+    th.join()
+    stats = srv.stats()
+    srv.close()
+    lat = np.array([tim["done_s"] - tim["submit_s"] for _, _, tim in done]) * 1e3
+    errors = sum(1 for _, res, _ in done if isinstance(res, Exception))
+    vox = sum(req.n for req, res, _ in done if not isinstance(res, Exception))
+    t_first = min(tim["submit_s"] for _, _, tim in done)
+    t_last = max(tim["done_s"] for _, _, tim in done)
+    out = {"workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g} s, seed 5; "
+                       "lognormal voxels (median 192^3, sigma_ln 0.7) clipped to 64^3..512^3, "
+                       "axes jittered +-20%; crops of 4 generated 512^3 volumes, pool of 256",
+           "requests": len(done), "errors": errors,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+           "gvox_per_s": vox / (t_last - t_first) / 1e9,
+           "offered_gvox_per_s": vox / args.serve_seconds / 1e9,
+           "budget_gb": args.serve_budget_gb, "slots": args.serve_slots,
+           "max_batch": args.serve_max_batch, "batches": stats["batches"],
+           "mean_batch": stats["sum_batch"] / max(stats["batches"], 1),
+           "largest_batch": stats["max_batch"], "max_inflight_batches": stats["max_inflight"],
+           "peak_device_gb": stats["peak_device_bytes"] / 1e9,
+           "gpu_busy_frac": stats["busy_s"] / (t_last - t_first),
+           "max_submit_lag_ms": max_lag * 1e3}
+    log(f"serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+        f"{out['gvox_per_s']:.3f} Gvoxel/s")
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (volumes shard "
+              "across GPUs by whole volumes; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+    import bone_morphometry_amd as bm
+    from bone_morphometry_amd import _native, hip, workloads
+
+    if _native.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    p = plain_form(args, bm, _native, hip, workloads, cpu_threads)
+    batched = batched_form(args, bm, hip, workloads, cpu_threads) if "batched" in forms else None
+    serving = serving_form(args, bm, workloads, cpu_threads) if "serving" in forms else None
+    wl = p["wl"]
     line = {
         "metric": "Gvoxel/s EDT+local thickness at 512^3",
-        "value": gvox,
+        "value": p["gvox"],
         "unit": "Gvoxel/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": p["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -155,15 +353,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "shape": [wl.width, wl.height, wl.depth],
                    "roi": "z-cylinder d=460" if wl.name == "plain_512" else "full",
-                   "spacing_um": wl.spacing[0], "mode": args.mode, "phases": phases},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "edt": {"ms_per_phase": edt, "gbs_22B_per_voxel": edt_gbs,
-                "frac": (edt_gbs / HBM_PEAK_GBS) if edt_gbs else None},
-        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in kernels.items()},
-        "parity_vs_oracle": parity,
-        "tb_th_mean_um": res.thickness.mean,
-        "tb_sp_mean_um": res.spacing.mean,
+                   "spacing_um": wl.spacing[0], "mode": args.mode, "phases": 2},
+        "roofline": p["roofline"],
+        "cpu_baseline": p["cpu"],
+        "edt": {"ms_per_launch": p["edt"], "gbs_22B_per_voxel_phase": p["edt_gbs"],
+                "frac": (p["edt_gbs"] / HBM_PEAK_GBS) if p["edt_gbs"] else None},
+        "kernels_ms_per_step": {k: v[1] / args.steps for k, v in p["kernels"].items()},
+        "parity_vs_oracle": p["parity"],
+        "tb_th_mean_um": p["res"].thickness.mean,
+        "tb_sp_mean_um": p["res"].spacing.mean,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0

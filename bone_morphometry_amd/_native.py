@@ -9,6 +9,9 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbm.so")
 SYNTH_PATH = os.path.join(HERE, "libbm_synth.so")
+# The autotuner's cache shipped with the package (per architecture and shape
+# class); BM_TUNE_CACHE, when set, takes precedence.
+TUNE_PATH = os.path.join(HERE, "tuning.json")
 
 BM_OK = 0
 BM_ERR_INVALID = 1
@@ -88,6 +91,8 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError("libbm.so is not built; run __graft_entry__.build() or "
                               "python bone_morphometry_amd/build.py")
+        if os.path.exists(TUNE_PATH):
+            os.environ.setdefault("BM_TUNE_CACHE", TUNE_PATH)
         L = ctypes.CDLL(LIB_PATH)
         P, U64, I, VP = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p
         VOL = ctypes.POINTER(BmVolume)

@@ -513,42 +513,46 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     clk_p3 += c4 - c3;
     // P4: this wave's 4x4x4 sub-brick scans the round in order.
     if (!done) {
-      // Box of the voxels still waiting (lane = x + 4y + 16z of the
-      // sub-brick), from the wave's mask with scalar bit operations; a
-      // candidate that misses it cannot change any result.
       unsigned long long wait = __ballot(target && m == 0);
-      int bx0 = sx, bx1 = sx + 3, by0 = sy, by1 = sy + 3, bz0 = sz, bz1 = sz + 3;
       for (uint32_t c0 = 0; c0 < cnt_round; c0 += 64) {
         ++scanned;
-        if (held == 0xFFFFFFFFu && wait) {
+        const uint32_t ci = c0 + lane;
+        uint32_t cp = 0, cr = 0;
+        if (ci < cnt_round) {
+          cp = s_pos[ci];
+          cr = s_r2[ci];
+        }
+        // Voxels this chunk can still raise: targets whose maximum so far is
+        // below the chunk's largest r^2 (those not covered yet among them).
+        // With shift > 0 a bucket mixes radii in any order, so a voxel
+        // already covered by a ball of the bucket can still gain a larger one
+        // of the same bucket.  Box of those voxels (lane = x + 4y + 16z of the
+        // sub-brick) from the wave's mask with scalar bit operations; a
+        // candidate that misses it cannot change any result.
+        const uint32_t cmax = wave_max_u32(cr);
+        const unsigned long long open = __ballot(target && m < cmax);
+        bool rel = false;
+        if (open) {
           uint32_t zm = 0, rows = 0;
           for (int k = 0; k < 4; ++k) {
-            const uint32_t g = (uint32_t)(wait >> (16 * k)) & 0xFFFFu;
+            const uint32_t g = (uint32_t)(open >> (16 * k)) & 0xFFFFu;
             zm |= (g != 0) << k;
             rows |= g;
           }
           const uint32_t ym = ((rows & 0xF) != 0) | (((rows >> 4) & 0xF) != 0) << 1 |
                               (((rows >> 8) & 0xF) != 0) << 2 | (((rows >> 12) & 0xF) != 0) << 3;
           const uint32_t xm = (rows | rows >> 4 | rows >> 8 | rows >> 12) & 0xF;
-          bx0 = sx + __ffs(xm) - 1;
-          bx1 = sx + 31 - __clz(xm);
-          by0 = sy + __ffs(ym) - 1;
-          by1 = sy + 31 - __clz(ym);
-          bz0 = sz + __ffs(zm) - 1;
-          bz1 = sz + 31 - __clz(zm);
-        }
-        const uint32_t ci = c0 + lane;
-        uint32_t cp = 0, cr = 0;
-        bool rel = false;
-        if (ci < cnt_round) {
-          cp = s_pos[ci];
-          cr = s_r2[ci];
-          const int cx = (int)(cp & 1023) - 512, cy = (int)((cp >> 10) & 1023) - 512,
-                    cz = (int)(cp >> 20) - 512;
-          const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
-                    ez = max(max(bz0 - cz, cz - bz1), 0);
-          rel = (uint32_t)(ex * ex + ey * ey + ez * ez) <= cr;
-          if (held != 0xFFFFFFFFu && (cr >> gshift) != held) rel = false;
+          const int bx0 = sx + __ffs(xm) - 1, bx1 = sx + 31 - __clz(xm);
+          const int by0 = sy + __ffs(ym) - 1, by1 = sy + 31 - __clz(ym);
+          const int bz0 = sz + __ffs(zm) - 1, bz1 = sz + 31 - __clz(zm);
+          if (ci < cnt_round) {
+            const int cx = (int)(cp & 1023) - 512, cy = (int)((cp >> 10) & 1023) - 512,
+                      cz = (int)(cp >> 20) - 512;
+            const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
+                      ez = max(max(bz0 - cz, cz - bz1), 0);
+            rel = (uint32_t)(ex * ex + ey * ey + ez * ez) <= cr;
+            if (held != 0xFFFFFFFFu && (cr >> gshift) != held) rel = false;
+          }
         }
         unsigned long long rm = __ballot(rel);
         while (rm) {

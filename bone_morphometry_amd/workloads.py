@@ -76,3 +76,54 @@ def make_volume(p, threads=0):
 
 def batched(count=96, seed=4, threads=0, lo=64, hi=384):
     return [make_volume(p, threads) for p in batched_params(count, seed, lo, hi)]
+
+
+@dataclasses.dataclass
+class Request:
+    width: int
+    height: int
+    depth: int
+    bone: np.ndarray
+    roi: np.ndarray
+    spacing: tuple
+
+    @property
+    def n(self):
+        return self.width * self.height * self.depth
+
+
+def serving_stream(rate=150.0, duration=60.0, seed=5, pool=256, median_side=192,
+                   sigma_ln=0.7, lo=64, hi=512, jitter=0.2, bases=4, threads=0):
+    """The serving workload: Poisson arrival times at `rate` per second over
+    `duration` seconds, and a pool of request volumes whose voxel counts are
+    lognormal (median median_side^3, log-sigma sigma_ln, clipped to
+    [lo^3, hi^3]) with near-cubic axes (each axis jittered by U[-jitter,
+    jitter], clipped to [lo, hi]).  Arrival i carries pool[i % pool].  Each
+    pool volume is a crop, at a random offset, of one of `bases` generated
+    hi^3 volumes (sigma U[2, 6], BV/TV U[0.08, 0.35]), with a centred
+    z-cylinder ROI of diameter U[0.70, 0.95]*min(w, h) and spacing U[6, 20] um.
+    Returns (arrival times in seconds, pool of Request)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < duration]
+    base = []
+    for _ in range(bases):
+        sigma = float(rng.uniform(2.0, 6.0))
+        bvtv = float(rng.uniform(0.08, 0.35))
+        v = synth.volume(hi, hi, hi, sigma, bvtv, int(rng.integers(1, 2**62)), threads)
+        base.append(v.reshape(hi, hi, hi))
+    reqs = []
+    mu = np.log(float(median_side) ** 3)
+    for _ in range(pool):
+        nvox = float(np.clip(np.exp(rng.normal(mu, sigma_ln)), float(lo) ** 3, float(hi) ** 3))
+        side = nvox ** (1.0 / 3.0)
+        w, h, d = (int(np.clip(round(side * (1.0 + rng.uniform(-jitter, jitter))), lo, hi))
+                   for _ in range(3))
+        b = base[int(rng.integers(0, bases))]
+        z0, y0, x0 = (int(rng.integers(0, hi - s + 1)) for s in (d, h, w))
+        bone = np.ascontiguousarray(b[z0:z0 + d, y0:y0 + h, x0:x0 + w]).reshape(-1)
+        roi = synth.roi_cylinder(w, h, d, float(rng.uniform(0.70, 0.95)) * min(w, h))
+        sp = float(rng.uniform(6.0, 20.0))
+        reqs.append(Request(w, h, d, bone, roi, (sp, sp, sp)))
+    return arrivals, reqs

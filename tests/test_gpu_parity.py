@@ -135,6 +135,36 @@ def test_fill_rounds_match_oracle(cap, monkeypatch):
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
 
+@pytest.mark.parametrize("sites,seed", [(24, 21), (60, 22)])
+def test_fill_mixed_radius_buckets_match_oracle(sites, seed):
+    """Sparse sites give squared distances above 511, so the fill's radius
+    buckets hold several r^2 values each: a voxel covered by a ball of a
+    bucket must still take a larger ball of the same bucket.  Run twice: the
+    order inside a bucket varies between runs, the result may not."""
+    rng = np.random.default_rng(seed)
+    w, h, d = 72, 64, 56
+    a = np.ones(w * h * d, np.uint8)
+    a[rng.choice(a.size, sites, replace=False)] = 0
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    assert ref["squared_edt"].max() >= 512
+    for _ in range(2):
+        got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+        np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+        np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
+def test_analyze_large_radius_volume_matches_oracle():
+    """A volume of the batched workload whose marrow distances exceed 511."""
+    p = workloads.batched_params()[29]
+    wl = workloads.make_volume(p)
+    ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, wl.depth,
+                                        wl.spacing)
+    got = bm.analyze_thickness_spacing(vol(wl.bone, wl.width, wl.height, wl.depth),
+                                       vol(wl.roi, wl.width, wl.height, wl.depth), wl.spacing)
+    _check_stats(got.thickness, ref.thickness)
+    _check_stats(got.spacing, ref.spacing)
+
+
 def test_local_thickness_plain_small_matches_oracle():
     wl = workloads.plain_small()
     for phase in (bm.Phase.Foreground, bm.Phase.Background):
