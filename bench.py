@@ -67,7 +67,26 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (torch), the "
+                         "reference for scaling FETCH_SIZE/WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: src (mib MiB) read once, dst written
+    once, three times."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3,
+                      "equal": ok}))
+    return 0
 
 
 def log(msg):
@@ -321,6 +340,8 @@ def serving_form(args, bm, workloads, cpu_threads):
 
 def main():
     args = parse()
+    if args.calibrate_copy_mib:
+        return calibrate_copy(args.calibrate_copy_mib)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (volumes shard "
               "across GPUs by whole volumes; see DESIGN.md). Run with --gpus 1.",
