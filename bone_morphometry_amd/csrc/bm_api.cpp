@@ -102,12 +102,16 @@ Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
 
 // Kernel tunables for a volume: the autotuner's choice for its shape class,
 // with test overrides from the environment (BM_FILL_CAP: fill candidates per
-// round).
+// round; BM_TILE_ORDER: tile order bits).
 static Tuning tuning_for(const Dims& dims) {
   Tuning t = tuned_for(dims);
   if (const char* env = std::getenv("BM_FILL_CAP")) {
     const long v = std::strtol(env, nullptr, 10);
     if (v >= 1 && v <= 1024) t.fill_cap = (uint32_t)v;
+  }
+  if (const char* env = std::getenv("BM_TILE_ORDER")) {  // tile order bits (see Tuning)
+    const long v = std::strtol(env, nullptr, 10);
+    if (v >= 0 && v <= 7) t.tile_order = (uint32_t)v;
   }
   return t;
 }

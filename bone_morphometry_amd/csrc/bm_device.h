@@ -144,8 +144,31 @@ __device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint3
 // ridge lists overlap, then share an L2.
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t ntiles) {
   const uint32_t per = (ntiles + kXcds - 1) / kXcds;
-  const uint32_t t = (b % kXcds) * per + b / kXcds;
-  return t;  // may be >= ntiles for the last XCD's tail: callers skip those
+  if (b / kXcds >= per) return ntiles;  // more blocks than positions: none
+  return (b % kXcds) * per + b / kXcds;  // >= ntiles in the last XCD's tail: callers skip
+}
+
+// Tile order of the tile kernels: 4x4x4 groups of 8^3 tiles (64 consecutive
+// positions each, x fastest inside), the groups in x-fastest order.  A short
+// run of positions then covers tiles that are neighbours along all three
+// axes, whose halos and ridge lists overlap in L2.  Positions of a partial
+// group beyond the volume give kNoTile.
+// With grouped == false the order is plain x-fastest (position = tile).
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+__host__ __device__ inline uint32_t tile_positions(uint32_t TX, uint32_t TY, uint32_t TZ,
+                                                   bool grouped) {
+  return grouped ? ((TX + 3) / 4) * ((TY + 3) / 4) * ((TZ + 3) / 4) * 64 : TX * TY * TZ;
+}
+__device__ __forceinline__ uint32_t tile_at(uint32_t j, uint32_t TX, uint32_t TY, uint32_t TZ,
+                                            bool grouped) {
+  if (!grouped) return j < TX * TY * TZ ? j : kNoTile;
+  const uint32_t SX = (TX + 3) >> 2, SY = (TY + 3) >> 2;
+  const uint32_t s = j >> 6, l = j & 63;
+  const uint32_t tx = (s % SX) * 4 + (l & 3);
+  const uint32_t ty = ((s / SX) % SY) * 4 + ((l >> 2) & 3);
+  const uint32_t tz = (s / (SX * SY)) * 4 + (l >> 4);
+  if (tx >= TX || ty >= TY || tz >= TZ) return kNoTile;
+  return (tz * TY + ty) * TX + tx;
 }
 
 // Item of block b in a stage: the last item whose start is <= b (items with

@@ -113,6 +113,9 @@ struct Tuning {
   uint32_t z_rows = 512;
   uint32_t z_halo = 64;
   uint32_t fill_cap = 0;  // 0: the kernel's capacity
+  // Tile order of the ridge, fill and clean-up kernels (bits 0, 1, 2): set =
+  // 4x4x4 groups of tiles, clear = x-fastest.
+  uint32_t tile_order = 0;
 };
 
 constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
@@ -165,6 +168,7 @@ struct ItemDesc {
   uint32_t ygx, ygy, ygz, ychunk, yhalo;
   uint32_t zgx, zgy, zgz, zchunk, zhalo;
   uint32_t lblocks, fill_cap, cblocks, qblocks;
+  uint32_t tile_order;  // Tuning::tile_order
   float scale;
   int lsb;
 };

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "bm_device.h"
 #include "bm_engine.h"
 #include "bm_tiles.h"
 
@@ -127,6 +128,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   }
   d.lblocks = lt ? std::min<uint32_t>(std::max<uint32_t>((d.bound + 4) / 4, 1), 128) : 0;
   d.fill_cap = t.fill_cap ? std::min<uint32_t>(t.fill_cap, kFillCap) : kFillCap;
+  d.tile_order = t.tile_order & 7u;
   d.cblocks = lt ? std::min<uint32_t>(std::max<uint32_t>(round8((d.ntiles + 7) / 8), 8),
                                       kStatsBlocks)
                  : 0;
@@ -136,7 +138,10 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
     blocks[kStEdtY] = d.ygx * d.ygy * d.ygz;
     blocks[kStEdtZ] = d.zgx * d.zgy * d.zgz;
     blocks[kStLuts] = d.lblocks;
-    blocks[kStTiles] = lt ? round8(d.ntiles) : 0;
+    // one block per position of the ridge and fill orders (the larger of the two)
+    blocks[kStTiles] = lt ? round8(std::max(tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 1u),
+                                            tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 2u)))
+                          : 0;
     blocks[kStOverflow] = lt ? kOverflowBlocks : 0;
     blocks[kStCleanup] = d.cblocks;
     blocks[kStQHist] = d.qblocks;
@@ -175,7 +180,8 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
       d.qhist = reinterpret_cast<unsigned long long*>(take(kMaxRanks * 256 * 8));
     }
     if (d.flags & kItemDiag)
-      d.diag = reinterpret_cast<unsigned long long*>(take((size_t)round8(d.ntiles) * 8 * 16 * 8));
+      d.diag = reinterpret_cast<unsigned long long*>(
+          take((size_t)round8(tile_positions(d.TX, d.TY, d.TZ, true)) * 8 * 16 * 8));
   }
   return off;
 }

@@ -113,8 +113,12 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   TileMeta* __restrict__ meta = I.meta;
   unsigned long long* __restrict__ masks = I.masks;
   if (!st->has_site) return;
-  const uint32_t t = xcd_tile(vb, TX * TY * TZ);
-  if (t >= TX * TY * TZ) return;
+  const bool grouped = I.tile_order & 1u;
+  const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
+  const uint32_t j = xcd_tile(vb, npos);
+  if (j >= npos) return;
+  const uint32_t t = tile_at(j, TX, TY, TZ, grouped);
+  if (t == kNoTile) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
@@ -287,8 +291,12 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
   unsigned long long clk_p3 = 0, clk_p4 = 0;
-  const uint32_t t = xcd_tile(vb, TX * TY * TZ);
-  if (t >= TX * TY * TZ) return;
+  const bool grouped = I.tile_order & 2u;
+  const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
+  const uint32_t jpos = xcd_tile(vb, npos);
+  if (jpos >= npos) return;
+  const uint32_t t = tile_at(jpos, TX, TY, TZ, grouped);
+  if (t == kNoTile) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   const int hx = min(ox + 7, (int)W - 1), hy = min(oy + 7, (int)H - 1), hz = min(oz + 7, (int)D - 1);
@@ -646,7 +654,8 @@ __global__ void __launch_bounds__(256) k_reduce_diag(const ItemDesc* __restrict_
   const ItemDesc& I = items[blockIdx.x];
   if (!(I.flags & kItemDiag)) return;
   const unsigned long long* __restrict__ diag_buf = I.diag;
-  const uint64_t nslots = (uint64_t)((I.ntiles + kXcds - 1) / kXcds * kXcds) * 8;
+  const uint64_t nslots =
+      (uint64_t)((tile_positions(I.TX, I.TY, I.TZ, true) + kXcds - 1) / kXcds * kXcds) * 8;
   PhaseStatus* __restrict__ st = I.status;
   __shared__ unsigned long long s[16];
   if (threadIdx.x < 16) s[threadIdx.x] = 0;
@@ -841,14 +850,23 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       pre[k] = v;
     }
   };
-  // Block b walks tiles j = b/8, b/8 + G/8, ... of its XCD's contiguous range
-  // (G blocks, a multiple of 8), so neighbouring tiles share that XCD's L2.
-  const uint32_t per = (ntiles + kXcds - 1) / kXcds;
-  const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, ntiles);
+  // Block b walks positions j = b/8, b/8 + G/8, ... of its XCD's contiguous
+  // range of the tile order (G blocks, a multiple of 8), so neighbouring
+  // tiles share that XCD's L2.
+  const bool grouped = I.tile_order & 4u;
+  const uint32_t npos = tile_positions(TX, I.TY, I.TZ, grouped);
+  const uint32_t per = (npos + kXcds - 1) / kXcds;
+  const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, npos);
   const uint32_t jstep = I.cblocks / kXcds;
-  const uint32_t t0 = xbase + vb / kXcds;
-  if (valid_phase && t0 < xend) fetch(t0);
-  for (uint32_t t = t0; t < xend && valid_phase; t += jstep) {
+  // next position at or after j that holds a tile (uniform over the block)
+  auto next_tile = [&](uint32_t j, uint32_t& t) {
+    while (j < xend && (t = tile_at(j, TX, I.TY, I.TZ, grouped)) == kNoTile) j += jstep;
+    return j;
+  };
+  uint32_t t = kNoTile, tn = kNoTile;
+  uint32_t j = valid_phase ? next_tile(xbase + vb / kXcds, t) : xend;
+  if (j < xend) fetch(t);
+  while (j < xend) {
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
     const unsigned long long selw = pre_sel;
     const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
@@ -859,9 +877,12 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       for (int k = 0; k < 4; ++k)
         if (threadIdx.x + 512u * k < 1728) s_m[threadIdx.x + 512u * k] = pre[k];
     }
-    if (t + jstep < xend) fetch(t + jstep);  // next tile, in flight meanwhile
+    const uint32_t jn = next_tile(j + jstep, tn);
+    if (jn < xend) fetch(tn);  // next tile, in flight meanwhile
     if (!any_sel) {  // no selected voxel in this tile
       if (map_out && valid) map_out[gidx] = 0.0f;
+      j = jn;
+      t = tn;
       continue;
     }
     __syncthreads();
@@ -924,6 +945,8 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     }
     if (map_out && valid) map_out[gidx] = tv;
     __syncthreads();  // s_m, s_mark are rewritten by the next tile
+    j = jn;
+    t = tn;
   }
   if (top16) {
     __syncthreads();

@@ -146,6 +146,7 @@ void parse_cache(const std::string& text) {
         get("z_rows", e.t.z_rows);
         get("z_halo", e.t.z_halo);
         get("fill_cap", e.t.fill_cap);
+        get("tile_order", e.t.tile_order);
         for (const auto& kv : m)
           if (kv.first.find("_ms") != std::string::npos) e.extra[kv.first] = kv.second;
         classes[cls] = e;
@@ -167,7 +168,8 @@ std::string dump_cache() {
       const Tuning& t = c.second.t;
       o << (first ? "" : ",\n") << "    \"" << c.first << "\": {\"y_rows\": " << t.y_rows
         << ", \"y_halo\": " << t.y_halo << ", \"z_rows\": " << t.z_rows
-        << ", \"z_halo\": " << t.z_halo << ", \"fill_cap\": " << t.fill_cap;
+        << ", \"z_halo\": " << t.z_halo << ", \"fill_cap\": " << t.fill_cap
+        << ", \"tile_order\": " << t.tile_order;
       for (const auto& kv : c.second.extra) {
         char buf[64];
         snprintf(buf, sizeof buf, "%.4f", kv.second);
