@@ -175,6 +175,16 @@ def plain_form(args, bm, native, hip, workloads, cpu_threads):
     edt_ms = sum(edt.values())
     edt_gbs = 22.0 * 2 * wl.n / (edt_ms * 1e-3) / 1e9 if edt_ms else None
 
+    # The host-pointer entry point (bm_analyze_thickness_spacing): inputs copied
+    # over PCIe each call, for the PCIe-inclusive rate.
+    bvol = bm.BinaryVolume(wl.bone, wl.width, wl.height, wl.depth)
+    rvol = bm.BinaryVolume(wl.roi, wl.width, wl.height, wl.depth)
+    bm.analyze_thickness_spacing(bvol, rvol, wl.spacing, mode)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        bm.analyze_thickness_spacing(bvol, rvol, wl.spacing, mode)
+    host_ms = (time.perf_counter() - t1) / 3 * 1e3
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -204,7 +214,8 @@ def plain_form(args, bm, native, hip, workloads, cpu_threads):
                 ok &= g.median == r.median and g.percentile_90 == r.percentile_90
         parity = bool(ok)
     return dict(wl=wl, res=res, ms_per_step=ms_per_step, gvox=gvox, roofline=roofline,
-                edt=edt, edt_gbs=edt_gbs, kernels=kernels, cpu=cpu, parity=parity)
+                edt=edt, edt_gbs=edt_gbs, kernels=kernels, cpu=cpu, parity=parity,
+                host_ms=host_ms)
 
 
 def batched_form(args, bm, hip, workloads, cpu_threads):
@@ -381,6 +392,8 @@ def main():
                 "frac": (p["edt_gbs"] / HBM_PEAK_GBS) if p["edt_gbs"] else None},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in p["kernels"].items()},
         "parity_vs_oracle": p["parity"],
+        "host_pointer_call": {"ms": p["host_ms"],
+                              "gvox_per_s_pcie_inclusive": wl.n / (p["host_ms"] * 1e-3) / 1e9},
         "tb_th_mean_um": p["res"].thickness.mean,
         "tb_sp_mean_um": p["res"].spacing.mean,
         "batched": batched,
