@@ -125,7 +125,7 @@ constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
 constexpr uint32_t kTopBase = 0x4000u;
 constexpr int kTopBins = 2048;
 constexpr int kEdtXBlocks = 4096;
-constexpr uint32_t kMinZChunk = 64;  // smallest z extent of a Z-pass tile
+constexpr uint32_t kMinZChunk = 16;  // smallest z extent of a split Z-pass tile
 constexpr uint32_t kXcds = 8;        // XCDs of the device (round-robin block dispatch)
 
 // Item flags.

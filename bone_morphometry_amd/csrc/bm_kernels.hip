@@ -180,6 +180,48 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
 }
 
 // ---------------------------------------------------------------------------
+// The 1-D step of the Y and Z passes for one lane: d(q) = min_p (q-p)^2 + f(p)
+// for ROWS consecutive q per group, groups q0 = first, first + step, ... below
+// end.  At offset o the candidates of row k are q0+k-o and q0+k+o, a window
+// that slides by one row per offset, so two reads of f serve 2*ROWS
+// candidates.  The search stops once o^2 reaches every row's best, which is
+// exact: every later candidate adds at least o^2.  f(p) is kInf32 outside
+// [0, n); n bounds the offsets when no finite candidate exists.
+constexpr int kEdtRows = 8;
+template <int ROWS, class F, class Out>
+__device__ __forceinline__ void window_search(uint32_t first, uint32_t end, uint32_t step,
+                                              uint32_t n, F&& f, Out&& out) {
+  for (uint32_t q0 = first; q0 < end; q0 += step) {
+    uint32_t best[ROWS], lo[ROWS], up[ROWS];
+    uint32_t bmax = 0;
+#pragma unroll
+    for (int k = 0; k < ROWS; ++k) {
+      best[k] = lo[k] = up[k] = f((int)(q0 + k));
+      if (q0 + k < end) bmax = max(bmax, best[k]);
+    }
+    for (uint32_t o = 1; o * o < bmax; ++o) {
+      if ((int)(q0 + ROWS - 1) < (int)o && q0 + o >= n) break;  // nothing left on either side
+#pragma unroll
+      for (int k = ROWS - 1; k > 0; --k) lo[k] = lo[k - 1];
+      lo[0] = f((int)q0 - (int)o);
+#pragma unroll
+      for (int k = 0; k < ROWS - 1; ++k) up[k] = up[k + 1];
+      up[ROWS - 1] = f((int)(q0 + ROWS - 1 + o));
+      const uint32_t oo = o * o;
+      bmax = 0;
+#pragma unroll
+      for (int k = 0; k < ROWS; ++k) {
+        best[k] = min(best[k], min(lo[k], up[k]) + oo);
+        if (q0 + k < end) bmax = max(bmax, best[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < ROWS; ++k)
+      if (q0 + k < end) out(q0 + k, best[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Y pass: tile of 64 x-columns x rows [ya, yb) of one z-slice in LDS (u16).
 __global__ void __launch_bounds__(256)
 k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
@@ -215,26 +257,15 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t x = x0 + lane;
   if (x >= W) return;
   const uint16_t* col = in + sbase + x;
-  for (uint32_t q = y0 + wv; q < y1; q += nwv) {
-    uint32_t best = sq16(ytile[(q - ya) * 64 + lane]);
-    for (uint32_t o = 1; o * o < best; ++o) {
-      bool any = false;
-      if (q >= o) {
-        const uint32_t p = q - o;
-        const uint32_t f = p >= ya ? sq16(ytile[(p - ya) * 64 + lane]) : sq16(col[(size_t)p * W]);
-        best = min(best, f + o * o);
-        any = true;
-      }
-      if (q + o < H) {
-        const uint32_t p = q + o;
-        const uint32_t f = p < yb ? sq16(ytile[(p - ya) * 64 + lane]) : sq16(col[(size_t)p * W]);
-        best = min(best, f + o * o);
-        any = true;
-      }
-      if (!any) break;
-    }
-    out[sbase + (size_t)q * W + x] = best;
-  }
+  // f(p) = dx(p)^2 of this column, kInf32 outside it; global beyond the halo.
+  auto fy = [&](int p) -> uint32_t {
+    const uint32_t r = (uint32_t)(p - (int)ya);
+    if (r < nrows) return sq16(ytile[r * 64 + lane]);
+    if (p < 0 || p >= (int)H) return kInf32;
+    return sq16(col[(size_t)p * W]);
+  };
+  window_search<kEdtRows>(y0 + kEdtRows * wv, y1, kEdtRows * nwv, H, fy,
+                          [&](uint32_t q, uint32_t v) { out[sbase + (size_t)q * W + x] = v; });
 }
 
 // ---------------------------------------------------------------------------
@@ -275,27 +306,18 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   uint32_t mx = 0;
   if (x < W) {
     const uint32_t* col = in + base + xl;
-    for (uint32_t q = z0 + 2 * wv + zs; q < z1; q += 2 * nwv) {
-      uint32_t best = ztile[(q - za) * 32 + xl];
-      for (uint32_t o = 1; o * o < best; ++o) {
-        bool any = false;
-        if (q >= o) {
-          const uint32_t p = q - o;
-          const uint32_t f = p >= za ? ztile[(p - za) * 32 + xl] : col[(size_t)p * slice];
-          best = min(best, f + o * o);
-          any = true;
-        }
-        if (q + o < D) {
-          const uint32_t p = q + o;
-          const uint32_t f = p < zb ? ztile[(p - za) * 32 + xl] : col[(size_t)p * slice];
-          best = min(best, f + o * o);
-          any = true;
-        }
-        if (!any) break;
-      }
-      out[(size_t)q * slice + base + xl] = best;
-      mx = max(mx, best);
-    }
+    auto fz = [&](int p) -> uint32_t {  // kInf32 outside the line; global beyond the halo
+      const uint32_t r = (uint32_t)(p - (int)za);
+      if (r < nrows) return ztile[r * 32 + xl];
+      if (p < 0 || p >= (int)D) return kInf32;
+      return col[(size_t)p * slice];
+    };
+    // the two half-waves take adjacent groups of rows
+    window_search<kEdtRows>(z0 + 2 * kEdtRows * wv + kEdtRows * zs, z1, 2 * kEdtRows * nwv, D, fz,
+                            [&](uint32_t q, uint32_t v) {
+                              out[(size_t)q * slice + base + xl] = v;
+                              mx = max(mx, v);
+                            });
   }
   mx = wave_max_u32(mx);
   __shared__ uint32_t s_mx[4];

@@ -257,7 +257,8 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
   Engine& e = engine_for_device(current_device());
   std::lock_guard<std::mutex> g(e.mutex());
   const std::string arch = device_arch();
-  const uint32_t rows_c[] = {128, 256, 512};
+  // (rows, halo) of the EDT tiles; a split line needs a chunk of at least kMinZChunk
+  const uint32_t tile_c[][2] = {{64, 8}, {128, 16}, {128, 32}, {256, 32}, {512, 64}};
   const uint32_t cap_c[] = {256, 512, 1024};
   std::map<std::string, TuneEntry> found;
   for (const auto& smp : samples) {
@@ -270,15 +271,15 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
     base.keep_d2 = false;
     TuneEntry best;
     double by = 1e30, bz = 1e30, bf = 1e30;
-    for (uint32_t rows : rows_c) {
-      const uint32_t halo = rows / 8;
+    for (const auto& tc : tile_c) {
+      const uint32_t rows = tc[0], halo = tc[1];
       // a split column needs a chunk; the whole column is one candidate
       if (rows < smp.dims.h || rows == 512) {
         ItemSpec sp = base;
         sp.tune.y_rows = rows;
         sp.tune.y_halo = halo;
         const double ms = time_kernel(e, sp, "edt_y", 3);
-        if (verbose) fprintf(stderr, "[tune] %s y_rows %u: %.3f ms\n", shape_class(smp.dims).c_str(), rows, ms);
+        if (verbose) fprintf(stderr, "[tune] %s y_rows %u halo %u: %.3f ms\n", shape_class(smp.dims).c_str(), rows, halo, ms);
         if (ms < by) {
           by = ms;
           best.t.y_rows = rows;
@@ -290,7 +291,7 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
         sp.tune.z_rows = rows;
         sp.tune.z_halo = halo;
         const double ms = time_kernel(e, sp, "edt_z", 3);
-        if (verbose) fprintf(stderr, "[tune] %s z_rows %u: %.3f ms\n", shape_class(smp.dims).c_str(), rows, ms);
+        if (verbose) fprintf(stderr, "[tune] %s z_rows %u halo %u: %.3f ms\n", shape_class(smp.dims).c_str(), rows, halo, ms);
         if (ms < bz) {
           bz = ms;
           best.t.z_rows = rows;
