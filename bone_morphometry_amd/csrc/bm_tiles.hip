@@ -638,7 +638,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   }
 }
 
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, 8)
 k_fill_gather(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
               uint32_t nitems) {
   k_fill_gather_t<false>(items, starts, nitems);
