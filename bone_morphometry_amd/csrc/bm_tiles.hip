@@ -800,7 +800,7 @@ k_lists_to_flags(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, u
 // Clean-up + phase mask + statistics over 8x8x8 tiles (persistent blocks).
 constexpr uint32_t kOut = 0xFFFFFFFFu;  // outside the array: never zero, never summed
 
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512, 6)
 k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                 uint32_t nitems) {
   __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
