@@ -9,6 +9,19 @@ namespace bm {
 
 typedef unsigned __int128 u128;
 
+// Device pointers taken from an ItemDesc in memory are generic to the
+// compiler, which then emits flat loads and stores; flat accesses also count
+// against the LDS wait counter, so every LDS wait would wait for them too.
+// gbl() states that a pointer addresses global memory.
+#define BM_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ BM_GLOBAL T* gbl(T* p) {
+  return (BM_GLOBAL T*)p;
+}
+// Vector types usable through global-address-space pointers.
+typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t sq16(uint32_t d) { return d == kInf16 ? kInf32 : d * d; }
 
 __device__ __forceinline__ bool selected_voxel(uint8_t v, uint8_t r, uint32_t want) {

@@ -28,16 +28,21 @@ namespace bm {
 // ---------------------------------------------------------------------------
 // X pass.  Sites bits of 8 consecutive voxels x..x+7 of a row; bit j set when
 // voxel x+j exists and is not selected.  Counts selected voxels.
-__device__ __forceinline__ uint32_t site_bits8(const uint8_t* __restrict__ data,
-                                               const uint8_t* __restrict__ roi, uint32_t want,
+__device__ __forceinline__ uint32_t site_bits8(BM_GLOBAL const uint8_t* __restrict__ data,
+                                               BM_GLOBAL const uint8_t* __restrict__ roi, uint32_t want,
                                                size_t rb, uint32_t x, uint32_t W,
                                                uint32_t& selcnt) {
   uint32_t bits = 0;
   const bool aligned = ((reinterpret_cast<uintptr_t>(data + rb + x) |
                          (roi ? reinterpret_cast<uintptr_t>(roi + rb + x) : 0)) & 7) == 0;
   if (x + 8 <= W && aligned) {
-    uint2 v = *reinterpret_cast<const uint2*>(data + rb + x);
-    uint2 r = roi ? *reinterpret_cast<const uint2*>(roi + rb + x) : make_uint2(0x01010101u, 0x01010101u);
+    const v2u32 va = *(BM_GLOBAL const v2u32*)(data + rb + x);
+    const uint2 v = make_uint2(va.x, va.y);
+    uint2 r = make_uint2(0x01010101u, 0x01010101u);
+    if (roi) {
+      const v2u32 ra = *(BM_GLOBAL const v2u32*)(roi + rb + x);
+      r = make_uint2(ra.x, ra.y);
+    }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       uint8_t vb = (uint8_t)((j < 4 ? v.x : v.y) >> (8 * (j & 3)));
@@ -69,9 +74,9 @@ __global__ void __launch_bounds__(256) k_prepare(const ItemDesc* __restrict__ it
   uint32_t* st = reinterpret_cast<uint32_t*>(I.status);
   for (uint32_t i = threadIdx.x; i < sizeof(PhaseStatus) / 4; i += blockDim.x) st[i] = 0;
   if (I.flags & kItemThickness) {
-    if (threadIdx.x == 0) I.overflow[0] = 0;
+    if (threadIdx.x == 0) gbl(I.overflow)[0] = 0;
     if (I.flags & kItemQuantiles)
-      for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) I.top16[i] = 0;
+      for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) gbl(I.top16)[i] = 0;
   }
 }
 
@@ -79,10 +84,10 @@ __global__ void __launch_bounds__(256)
 k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint8_t* __restrict__ data = I.data;
-  const uint8_t* __restrict__ roi = I.roi;
+  BM_GLOBAL const uint8_t* __restrict__ data = gbl(I.data);
+  BM_GLOBAL const uint8_t* __restrict__ roi = gbl(I.roi);
   const uint32_t want = I.want_nonzero, W = I.W, rows = I.H * I.D;
-  uint16_t* __restrict__ out = I.dx;
+  BM_GLOBAL uint16_t* __restrict__ out = gbl(I.dx);
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gw = (vb * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (I.xblocks * blockDim.x) >> 6;
@@ -117,14 +122,19 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
           const int dist = min(x - L, R - x);
           o[j] = dist > 65534 ? (uint16_t)kInf16 : (uint16_t)dist;
         }
-        uint16_t* p = out + rb + x0;
+        auto p = out + rb + x0;
         if (x0 + 8 <= W && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
           uint4 v;
           v.x = o[0] | ((uint32_t)o[1] << 16);
           v.y = o[2] | ((uint32_t)o[3] << 16);
           v.z = o[4] | ((uint32_t)o[5] << 16);
           v.w = o[6] | ((uint32_t)o[7] << 16);
-          *reinterpret_cast<uint4*>(p) = v;
+          v4u32 w;
+          w.x = v.x;
+          w.y = v.y;
+          w.z = v.z;
+          w.w = v.w;
+          *(BM_GLOBAL v4u32*)(p) = w;
         } else {
           for (int j = 0; j < 8; ++j)
             if (x0 + j < W) p[j] = o[j];
@@ -145,7 +155,7 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
         const int Rc = higher ? fromR : carryR;
         if (has) carryR = __shfl(first, __ffsll(has) - 1);
         if (x0 < W) {
-          uint16_t* p = out + rb + x0;
+          auto p = out + rb + x0;
           for (int j = 0; j < 8; ++j) {
             if (x0 + j >= W) break;
             const uint32_t mR = bits & (~0u << j);
@@ -174,8 +184,8 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
       a += s_sel[i];
       b |= s_any[i];
     }
-    I.xpart[vb].selected = a;
-    I.xpart[vb].flag = b;
+    gbl(I.xpart)[vb].selected = a;
+    gbl(I.xpart)[vb].flag = b;
   }
 }
 
@@ -228,8 +238,8 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   extern __shared__ uint16_t ytile[];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint16_t* __restrict__ in = I.dx;
-  uint32_t* __restrict__ out = I.h;
+  BM_GLOBAL const uint16_t* __restrict__ in = gbl(I.dx);
+  BM_GLOBAL uint32_t* __restrict__ out = gbl(I.h);
   const uint32_t W = I.W, H = I.H, chunk = I.ychunk, halo = I.yhalo;
   const uint32_t bx = vb % I.ygx, bz = vb / I.ygx / I.ygy;
   const uint32_t x0 = bx * 64, z = (vb / I.ygx) % I.ygy;
@@ -240,10 +250,11 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const bool vec = (x0 + 64 <= W) && ((W & 7) == 0);
   for (uint32_t i = threadIdx.x; i < nrows * 8; i += blockDim.x) {
     const uint32_t rr = i >> 3, c = (i & 7) * 8;
-    const uint16_t* p = in + sbase + (size_t)(ya + rr) * W + x0 + c;
+    auto p = in + sbase + (size_t)(ya + rr) * W + x0 + c;
     uint4 v;
     if (vec) {
-      v = *reinterpret_cast<const uint4*>(p);
+      const v4u32 w = *(BM_GLOBAL const v4u32*)(p);
+      v = make_uint4(w.x, w.y, w.z, w.w);
     } else {
       uint32_t t[8];
       for (int k = 0; k < 8; ++k) t[k] = (x0 + c + k < W) ? p[k] : kInf16;
@@ -256,7 +267,7 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const uint32_t x = x0 + lane;
   if (x >= W) return;
-  const uint16_t* col = in + sbase + x;
+  auto col = in + sbase + x;
   // f(p) = dx(p)^2 of this column, kInf32 outside it; global beyond the halo.
   auto fy = [&](int p) -> uint32_t {
     const uint32_t r = (uint32_t)(p - (int)ya);
@@ -275,8 +286,8 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   extern __shared__ uint32_t ztile[];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint32_t* __restrict__ in = I.h;
-  uint32_t* __restrict__ out = I.d2;
+  BM_GLOBAL const uint32_t* __restrict__ in = gbl(I.h);
+  BM_GLOBAL uint32_t* __restrict__ out = gbl(I.d2);
   const uint32_t W = I.W, H = I.H, D = I.D, chunk = I.zchunk, halo = I.zhalo;
   const uint32_t x0 = (vb % I.zgx) * 32, y = (vb / I.zgx) % I.zgy;
   const uint32_t z0 = (vb / I.zgx / I.zgy) * chunk, z1 = min(D, z0 + chunk);
@@ -287,10 +298,11 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const bool vec = (x0 + 32 <= W) && ((W & 3) == 0);
   for (uint32_t i = threadIdx.x; i < nrows * 8; i += blockDim.x) {
     const uint32_t rr = i >> 3, c = (i & 7) * 4;
-    const uint32_t* p = in + (size_t)(za + rr) * slice + base + c;
+    auto p = in + (size_t)(za + rr) * slice + base + c;
     uint4 v;
     if (vec) {
-      v = *reinterpret_cast<const uint4*>(p);
+      const v4u32 w = *(BM_GLOBAL const v4u32*)(p);
+      v = make_uint4(w.x, w.y, w.z, w.w);
     } else {
       v.x = (x0 + c + 0 < W) ? p[0] : kInf32;
       v.y = (x0 + c + 1 < W) ? p[1] : kInf32;
@@ -305,7 +317,7 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t x = x0 + xl;
   uint32_t mx = 0;
   if (x < W) {
-    const uint32_t* col = in + base + xl;
+    auto col = in + base + xl;
     auto fz = [&](int p) -> uint32_t {  // kInf32 outside the line; global beyond the halo
       const uint32_t r = (uint32_t)(p - (int)za);
       if (r < nrows) return ztile[r * 32 + xl];
@@ -326,17 +338,17 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   if (threadIdx.x == 0) {
     uint32_t r = 0;
     for (uint32_t i = 0; i < nwv; ++i) r = max(r, s_mx[i]);
-    I.zmax[vb] = r;
+    gbl(I.zmax)[vb] = r;
   }
 }
 
 // Status of a phase from the EDT passes' per-block partials (one block).
 __global__ void __launch_bounds__(1024) k_reduce_status(const ItemDesc* __restrict__ items) {
   const ItemDesc& I = items[blockIdx.x];
-  const BlockPartial* __restrict__ xpart = I.xpart;
-  const uint32_t* __restrict__ zmax = I.zmax;
+  BM_GLOBAL const BlockPartial* __restrict__ xpart = gbl(I.xpart);
+  BM_GLOBAL const uint32_t* __restrict__ zmax = gbl(I.zmax);
   const uint32_t nx = I.xblocks, nz = I.zgx * I.zgy * I.zgz;
-  PhaseStatus* __restrict__ st = I.status;
+  BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
   unsigned long long sel = 0;
   uint32_t any = 0, mx = 0;
   for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -376,13 +388,13 @@ __global__ void __launch_bounds__(256)
 k_luts(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const PhaseStatus* __restrict__ st = I.status;
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
   if (!st->has_site) return;
   const uint32_t mx = st->max_d2;
   const uint32_t lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
   const uint32_t nw = I.lblocks * wpb, L = I.bound + 1;
   for (uint32_t m = vb * wpb + (threadIdx.x >> 6); m <= mx; m += nw) {
-    if (lane == 0) I.diam[m] = diameter_of(m);
+    if (lane == 0) gbl(I.diam)[m] = diameter_of(m);
     if (m == 0) continue;
     const uint32_t radius = isqrt_u32(m) + 1;
     uint32_t t1 = 0, t2 = 0, t3 = 0;
@@ -404,9 +416,9 @@ k_luts(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, 
     t2 = wave_max_u32(t2);
     t3 = wave_max_u32(t3);
     if (lane == 0) {
-      I.tmpl[m] = t1;
-      I.tmpl[L + m] = t2;
-      I.tmpl[2 * L + m] = t3;
+      gbl(I.tmpl)[m] = t1;
+      gbl(I.tmpl)[L + m] = t2;
+      gbl(I.tmpl)[2 * L + m] = t3;
     }
   }
 }

@@ -30,8 +30,8 @@ static const double h_qs[5] = {0.5, 0.1, 0.25, 0.75, 0.9};
 __global__ void __launch_bounds__(256) k_select_init(const ItemDesc* __restrict__ items) {
   const ItemDesc& I = items[blockIdx.x];
   if (!(I.flags & kItemQuantiles)) return;
-  SelectState* __restrict__ st = I.sel;
-  const unsigned long long count = I.stats->count;
+  BM_GLOBAL SelectState* __restrict__ st = gbl(I.sel);
+  const unsigned long long count = gbl(I.stats)->count;
   if (threadIdx.x < 5) {
     const uint32_t q = threadIdx.x;
     const double position = c_qs[q] * (double)(count > 0 ? count - 1 : 0);
@@ -69,9 +69,9 @@ k_top16_pick(const ItemDesc* __restrict__ items) {
   __shared__ unsigned long long s_w[4];
   const ItemDesc& I = items[blockIdx.x];
   if (!(I.flags & kItemQuantiles)) return;
-  SelectState* __restrict__ st = I.sel;
-  const uint32_t* __restrict__ top16 = I.top16;
-  for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) I.qhist[i] = 0;
+  BM_GLOBAL SelectState* __restrict__ st = gbl(I.sel);
+  BM_GLOBAL const uint32_t* __restrict__ top16 = gbl(I.top16);
+  for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) gbl(I.qhist)[i] = 0;
   constexpr int per = kTopBins / 256;
   uint32_t c[per];
   unsigned long long own = 0;
@@ -109,10 +109,10 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
   __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng;
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const SelectState* __restrict__ st = I.sel;
-  const float* __restrict__ map = I.map;
+  BM_GLOBAL const SelectState* __restrict__ st = gbl(I.sel);
+  BM_GLOBAL const float* __restrict__ map = gbl(I.map);
   const uint64_t n = I.n;
-  unsigned long long* __restrict__ hist = I.qhist;
+  BM_GLOBAL unsigned long long* __restrict__ hist = gbl(I.qhist);
   const uint32_t nr = st->nranks;
   if (nr == 0) return;
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) (&h[0][0])[i] = 0;
@@ -151,7 +151,7 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
     // expand the shared row to every rank of the group
     const uint32_t gi = i / 256, d = i % 256;
     for (uint32_t r = 0; r < nr; ++r)
-      if (grow[r] == gi) atomicAdd(&hist[r * 256 + d], (unsigned long long)v);
+      if (grow[r] == gi) atomicAdd((unsigned long long*)&hist[r * 256 + d], (unsigned long long)v);
   }
 }
 
@@ -161,8 +161,8 @@ __global__ void __launch_bounds__(256) k_digit_pick(const ItemDesc* __restrict__
   __shared__ unsigned long long s_w[4];
   const ItemDesc& I = items[blockIdx.x];
   if (!(I.flags & kItemQuantiles)) return;
-  SelectState* __restrict__ st = I.sel;
-  unsigned long long* __restrict__ hist = I.qhist;
+  BM_GLOBAL SelectState* __restrict__ st = gbl(I.sel);
+  BM_GLOBAL unsigned long long* __restrict__ hist = gbl(I.qhist);
   const uint32_t nr = st->nranks;
   const uint32_t d = threadIdx.x;
   for (uint32_t r = 0; r < nr; ++r) {

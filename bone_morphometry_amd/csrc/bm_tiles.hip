@@ -105,13 +105,13 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   __shared__ uint32_t s_tmp[16];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint32_t* __restrict__ d2 = I.d2;
+  BM_GLOBAL const uint32_t* __restrict__ d2 = gbl(I.d2);
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ, bound = I.bound;
-  const uint32_t* __restrict__ tmpl = I.tmpl;
-  PhaseStatus* __restrict__ st = I.status;
-  uint32_t* __restrict__ lists = I.lists;
-  TileMeta* __restrict__ meta = I.meta;
-  unsigned long long* __restrict__ masks = I.masks;
+  BM_GLOBAL const uint32_t* __restrict__ tmpl = gbl(I.tmpl);
+  BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
+  BM_GLOBAL uint32_t* __restrict__ lists = gbl(I.lists);
+  BM_GLOBAL TileMeta* __restrict__ meta = gbl(I.meta);
+  BM_GLOBAL unsigned long long* __restrict__ masks = gbl(I.masks);
   if (!st->has_site) return;
   const bool grouped = I.tile_order & 1u;
   const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
@@ -193,8 +193,8 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
     m.count = count;
     m.maxr2 = maxr2;
     for (int j = 0; j < 16; ++j) m.pref[j] = (uint16_t)s_tmp[j];
-    meta[t] = m;
-    if (maxr2 >= (1u << 23)) atomicOr(&st->error, 1u);
+    *(TileMeta*)&meta[t] = m;
+    if (maxr2 >= (1u << 23)) atomicOr((uint32_t*)&st->error, 1u);
   }
 }
 
@@ -211,7 +211,7 @@ __device__ __forceinline__ uint32_t box_gap(int lo, int hi, int v) {
 // Number of entries of a tile list that can reach a box at squared distance
 // bd2: those whose bucket key is at least that of kPrefR2(j), j the largest
 // with kPrefR2(j) <= bd2 (an upper bound read from pref[]).
-__device__ __forceinline__ uint32_t reach_len(const TileMeta* __restrict__ m, uint32_t count,
+__device__ __forceinline__ uint32_t reach_len(BM_GLOBAL const TileMeta* __restrict__ m, uint32_t count,
                                               uint32_t bd2) {
   if (bd2 < 1) return count;
   // (2j+1)^2 <= bd2  <=>  j <= (isqrt(bd2) - 1) / 2
@@ -279,15 +279,15 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ;
-  const uint32_t* __restrict__ lists = I.lists;
-  const TileMeta* __restrict__ meta = I.meta;
-  const unsigned long long* __restrict__ masks = I.masks;
-  PhaseStatus* __restrict__ st = I.status;
-  uint32_t* __restrict__ M = I.h;  // the Y-pass buffer is free once the Z pass has run
-  uint32_t* __restrict__ overflow = I.overflow;
+  BM_GLOBAL const uint32_t* __restrict__ lists = gbl(I.lists);
+  BM_GLOBAL const TileMeta* __restrict__ meta = gbl(I.meta);
+  BM_GLOBAL const unsigned long long* __restrict__ masks = gbl(I.masks);
+  BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
+  BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);  // the Y-pass buffer is free once the Z pass has run
+  BM_GLOBAL uint32_t* __restrict__ overflow = gbl(I.overflow);
   const uint32_t cap = I.fill_cap;
   constexpr bool diag = kDiag;
-  unsigned long long* __restrict__ diag_buf = I.diag;
+  BM_GLOBAL unsigned long long* __restrict__ diag_buf = gbl(I.diag);
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
   unsigned long long clk_p3 = 0, clk_p4 = 0;
@@ -371,8 +371,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const unsigned long long tb = diag ? wall_clock64() : 0ull;
   if (s_over) {  // too many tiles in reach: exact slow path
     if (threadIdx.x == 0) {
-      overflow[1 + atomicAdd(&overflow[0], 1u)] = t;
-      atomicAdd(&st->overflow_tiles, 1u);
+      overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t;
+      atomicAdd((uint32_t*)&st->overflow_tiles, 1u);
     }
     return;
   }
@@ -479,8 +479,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     uint32_t b1 = b0 + 1;
     if (s_start[b1] - s_start[b0] > cap) {
       if (threadIdx.x == 0) {  // one bucket alone exceeds LDS: exact slow path
-        overflow[1 + atomicAdd(&overflow[0], 1u)] = t;
-        atomicAdd(&st->overflow_tiles, 1u);
+        overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t;
+        atomicAdd((uint32_t*)&st->overflow_tiles, 1u);
       }
       return;
     }
@@ -534,10 +534,14 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         // below the chunk's largest r^2 (those not covered yet among them).
         // With shift > 0 a bucket mixes radii in any order, so a voxel
         // already covered by a ball of the bucket can still gain a larger one
-        // of the same bucket.  Box of those voxels (lane = x + 4y + 16z of the
-        // sub-brick) from the wave's mask with scalar bit operations; a
-        // candidate that misses it cannot change any result.
-        const uint32_t cmax = wave_max_u32(cr);
+        // of the same bucket.  The chunk is sorted by bucket, so its largest
+        // r^2 is at most the top of its first entry's bucket (one broadcast
+        // read instead of a wave reduction).  Box of those voxels (lane =
+        // x + 4y + 16z of the sub-brick) from the wave's mask with scalar bit
+        // operations; a candidate that misses it cannot change any result.
+        const uint32_t first_key =
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(s_r2[c0] >> gshift));
+        const uint32_t cmax = ((first_key + 1u) << gshift) - 1u;
         const unsigned long long open = __ballot(target && m < cmax);
         bool rel = false;
         if (open) {
@@ -618,7 +622,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // Work counters and phase clocks (diagnostics only): plain stores into this
   // block's slot, summed afterwards by k_reduce_diag.
   if (diag && lane == 0) {
-    unsigned long long* slot = diag_buf + ((size_t)vb * 8 + wv) * 16;
+    auto slot = diag_buf + ((size_t)vb * 8 + wv) * 16;
     slot[0] = 1;  // waves that reached the end
     slot[1] = tests;
     slot[2] = scanned;
@@ -653,10 +657,10 @@ k_fill_gather_diag(const ItemDesc* __restrict__ items, const uint32_t* __restric
 __global__ void __launch_bounds__(256) k_reduce_diag(const ItemDesc* __restrict__ items) {
   const ItemDesc& I = items[blockIdx.x];
   if (!(I.flags & kItemDiag)) return;
-  const unsigned long long* __restrict__ diag_buf = I.diag;
+  BM_GLOBAL const unsigned long long* __restrict__ diag_buf = gbl(I.diag);
   const uint64_t nslots =
       (uint64_t)((tile_positions(I.TX, I.TY, I.TZ, true) + kXcds - 1) / kXcds * kXcds) * 8;
-  PhaseStatus* __restrict__ st = I.status;
+  BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
   __shared__ unsigned long long s[16];
   if (threadIdx.x < 16) s[threadIdx.x] = 0;
   __syncthreads();
@@ -675,11 +679,11 @@ k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ;
-  const uint32_t* __restrict__ lists = I.lists;
-  const TileMeta* __restrict__ meta = I.meta;
-  const PhaseStatus* __restrict__ st = I.status;
-  uint32_t* __restrict__ M = I.h;
-  const uint32_t* __restrict__ overflow = I.overflow;
+  BM_GLOBAL const uint32_t* __restrict__ lists = gbl(I.lists);
+  BM_GLOBAL const TileMeta* __restrict__ meta = gbl(I.meta);
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
+  BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);
+  BM_GLOBAL const uint32_t* __restrict__ overflow = gbl(I.overflow);
   if (!st->has_site) return;
   const uint32_t count = overflow[0];
   const uint32_t max_r2 = st->max_d2;
@@ -697,7 +701,7 @@ k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       for (int uy = max(0, (int)ty - K); uy <= min((int)TY - 1, (int)ty + K); ++uy)
         for (int ux = max(0, (int)tx - K); ux <= min((int)TX - 1, (int)tx + K); ++ux) {
           const uint32_t u = ((uint32_t)uz * TY + (uint32_t)uy) * TX + (uint32_t)ux;
-          const TileMeta mt = meta[u];
+          const TileMeta mt = *(const TileMeta*)&meta[u];
           if (mt.maxr2 <= m) continue;  // cannot raise the maximum
           for (uint32_t e = 0; e < mt.count; ++e) {
             const uint32_t ent = lists[(size_t)u * 512 + e];
@@ -724,16 +728,16 @@ k_fill_scatter_lists(const ItemDesc* __restrict__ items, const uint32_t* __restr
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
-  const uint32_t* __restrict__ lists = I.lists;
-  const TileMeta* __restrict__ meta = I.meta;
-  const PhaseStatus* __restrict__ st = I.status;
-  uint32_t* __restrict__ M = I.h;
+  BM_GLOBAL const uint32_t* __restrict__ lists = gbl(I.lists);
+  BM_GLOBAL const TileMeta* __restrict__ meta = gbl(I.meta);
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
+  BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);
   if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gw = (vb * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nw = (kOverflowBlocks * blockDim.x) >> 6;
   for (uint32_t t = gw; t < ntiles; t += nw) {
-    const TileMeta mt = meta[t];
+    const TileMeta mt = *(const TileMeta*)&meta[t];
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
     for (uint32_t e = 0; e < mt.count; ++e) {
       const uint32_t ent = lists[(size_t)t * 512 + e];
@@ -750,7 +754,7 @@ k_fill_scatter_lists(const ItemDesc* __restrict__ items, const uint32_t* __restr
           const int ny = y + dy, nx = x + dx;
           if (ny < 0 || ny >= (int)H || nx < 0 || nx >= (int)W) continue;
           if (dz2 + (uint32_t)(dy * dy) + (uint32_t)(dx * dx) > r2) continue;
-          atomicMax(&M[((size_t)nz * H + ny) * W + nx], r2);
+          atomicMax((uint32_t*)&M[((size_t)nz * H + ny) * W + nx], r2);
         }
       }
     }
@@ -762,9 +766,9 @@ k_zero_if_scatter(const ItemDesc* __restrict__ items, const uint32_t* __restrict
                   uint32_t nitems) {
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const PhaseStatus* __restrict__ st = I.status;
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
   if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
-  uint32_t* __restrict__ M = I.h;
+  BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);
   const uint64_t n = I.n;
   const uint64_t stride = (uint64_t)kOverflowBlocks * blockDim.x;
   for (uint64_t i = (uint64_t)vb * blockDim.x + threadIdx.x; i < n; i += stride) M[i] = 0;
@@ -786,7 +790,7 @@ k_lists_to_flags(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, u
                  const uint32_t* __restrict__ lists, const TileMeta* __restrict__ meta,
                  uint8_t* __restrict__ flags) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const TileMeta mt = meta[t];
+    const TileMeta mt = *(const TileMeta*)&meta[t];
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
     for (uint32_t e = threadIdx.x; e < mt.count; e += blockDim.x) {
       const uint32_t l = lists[(size_t)t * 512 + e] & 511;
@@ -810,10 +814,10 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint32_t* __restrict__ M = I.h;
-  const unsigned long long* __restrict__ masks = I.masks;
-  const float* __restrict__ diam = I.diam;
-  const PhaseStatus* __restrict__ st = I.status;
+  BM_GLOBAL const uint32_t* __restrict__ M = gbl(I.h);
+  BM_GLOBAL const unsigned long long* __restrict__ masks = gbl(I.masks);
+  BM_GLOBAL const float* __restrict__ diam = gbl(I.diam);
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
   const float scale_f32 = I.scale;
   const int lsb_exp = I.lsb;
