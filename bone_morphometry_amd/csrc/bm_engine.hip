@@ -132,7 +132,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   d.cblocks = lt ? std::min<uint32_t>(std::max<uint32_t>(round8((d.ntiles + 7) / 8), 8),
                                       kStatsBlocks)
                  : 0;
-  d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 16384), 8), 2048) : 0;
+  d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 32768), 8), 1024) : 0;
   if (blocks) {
     blocks[kStEdtX] = d.xblocks;
     blocks[kStEdtY] = d.ygx * d.ygy * d.ygz;
