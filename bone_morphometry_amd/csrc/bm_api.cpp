@@ -109,6 +109,10 @@ static Tuning tuning_for(const Dims& dims) {
     const long v = std::strtol(env, nullptr, 10);
     if (v >= 1 && v <= 1024) t.fill_cap = (uint32_t)v;
   }
+  if (const char* env = std::getenv("BM_FILL_TILES")) {  // tiles per fill block (1 or 2)
+    const long v = std::strtol(env, nullptr, 10);
+    if (v == 1 || v == 2) t.fill_tiles = (uint32_t)v;
+  }
   if (const char* env = std::getenv("BM_TILE_ORDER")) {  // tile order bits (see Tuning)
     const long v = std::strtol(env, nullptr, 10);
     if (v >= 0 && v <= 7) t.tile_order = (uint32_t)v;

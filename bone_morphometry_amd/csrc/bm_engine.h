@@ -116,6 +116,10 @@ struct Tuning {
   // Tile order of the ridge, fill and clean-up kernels (bits 0, 1, 2): set =
   // 4x4x4 groups of tiles, clear = x-fastest.
   uint32_t tile_order = 0;
+  // Tiles per fill block along x (1: 512 threads per 8^3 tile; 2: 1024
+  // threads per 16x8x8 pair sharing one candidate gathering).  One value per
+  // plan: the first item's.
+  uint32_t fill_tiles = 1;
 };
 
 constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
@@ -179,7 +183,8 @@ enum Stage : int {
   kStEdtY,
   kStEdtZ,
   kStLuts,
-  kStTiles,     // ridges, fill gather (one block per tile, padded to a multiple of 8)
+  kStRidges,    // ridges (one block per tile, padded to a multiple of 8)
+  kStTiles,     // fill gather (one block per 1 or 2 tiles, padded to a multiple of 8)
   kStOverflow,  // fill overflow and scatter fallback
   kStCleanup,
   kStQHist,
@@ -258,6 +263,7 @@ class Plan {
   std::vector<uint32_t> starts_;  // kNumStages x (nitems + 1)
   uint32_t total_[kNumStages] = {};
   uint32_t y_lds_ = 0, z_lds_ = 0;
+  uint32_t fill_tiles_ = 1;  // fill variant of the whole plan
   uint32_t any_ = 0;  // union of item flags
   size_t bytes_ = 0;
   Arena mem_;                         // item buffers

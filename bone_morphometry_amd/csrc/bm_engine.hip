@@ -33,6 +33,7 @@ __global__ void k_collect(const ItemDesc*, ItemResult*);
 __global__ void k_ridges_tiled(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather_diag(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_gather2(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_reduce_diag(const ItemDesc*);
 __global__ void k_fill_overflow(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_zero_if_scatter(const ItemDesc*, const uint32_t*, uint32_t);
@@ -84,7 +85,8 @@ static uint32_t round8(uint64_t v) { return (uint32_t)((v + kXcds - 1) / kXcds *
 // ---------------------------------------------------------------------------
 // Item layout: geometry, block counts and buffer offsets (relative to the
 // item's base when base is null).  Returns the bytes the item needs.
-static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* blocks) {
+static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* blocks,
+                      uint32_t fill_tiles = 1) {
   const Dims& dims = s.dims;
   std::memset(&d, 0, sizeof d);
   d.data = s.src.data;
@@ -139,9 +141,11 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
     blocks[kStEdtZ] = d.zgx * d.zgy * d.zgz;
     blocks[kStLuts] = d.lblocks;
     // one block per position of the ridge and fill orders (the larger of the two)
-    blocks[kStTiles] = lt ? round8(std::max(tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 1u),
-                                            tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 2u)))
-                          : 0;
+    blocks[kStRidges] = lt ? round8(tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 1u)) : 0;
+    blocks[kStTiles] = !lt ? 0
+                       : fill_tiles == 2
+                           ? round8((uint64_t)((d.TX + 1) / 2) * d.TY * d.TZ)
+                           : round8(tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 2u));
     blocks[kStOverflow] = lt ? kOverflowBlocks : 0;
     blocks[kStCleanup] = d.cblocks;
     blocks[kStQHist] = d.qblocks;
@@ -208,9 +212,12 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
   size_t total = 0;
   any_ = 0;
   y_lds_ = z_lds_ = 0;
+  fill_tiles_ = specs[0].tune.fill_tiles == 2 ? 2u : 1u;
+  for (const auto& s : specs)
+    if (s.thickness && s.diag) fill_tiles_ = 1;  // the diagnostic fill has one tile per block
   for (size_t i = 0; i < nit; ++i) {
     offs[i] = total;
-    total += lay_out(specs[i], desc_[i], nullptr, &blocks[i * kNumStages]);
+    total += lay_out(specs[i], desc_[i], nullptr, &blocks[i * kNumStages], fill_tiles_);
     any_ |= desc_[i].flags;
     y_lds_ = std::max(y_lds_, std::min(desc_[i].H, desc_[i].ychunk + 2 * desc_[i].yhalo) * 64u * 2u);
     z_lds_ = std::max(z_lds_, std::min(desc_[i].D, desc_[i].zchunk + 2 * desc_[i].zhalo) * 32u * 4u);
@@ -225,7 +232,8 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
     base = static_cast<char*>(mem_.base());
   }
   bytes_ = total;
-  for (size_t i = 0; i < nit; ++i) lay_out(specs[i], desc_[i], base + offs[i], nullptr);
+  for (size_t i = 0; i < nit; ++i)
+    lay_out(specs[i], desc_[i], base + offs[i], nullptr, fill_tiles_);
   // Start tables: stage-major, nit + 1 entries each.
   starts_.assign((size_t)kNumStages * (nit + 1), 0);
   for (int st = 0; st < kNumStages; ++st) {
@@ -275,9 +283,13 @@ void Plan::launch(hipStream_t s) const {
   BM_LAUNCH("reduce_status", k_reduce_status, nit, 1024, 0, s, d_desc_);
   if (any_ & kItemThickness) {
     BM_LAUNCH("luts", k_luts, total_[kStLuts], 256, 0, s, d_desc_, st(kStLuts), nit);
-    BM_LAUNCH("ridges", k_ridges_tiled, total_[kStTiles], 512, 0, s, d_desc_, st(kStTiles), nit);
+    BM_LAUNCH("ridges", k_ridges_tiled, total_[kStRidges], 512, 0, s, d_desc_, st(kStRidges),
+              nit);
     if (any_ & kItemDiag)
       BM_LAUNCH("fill_gather", k_fill_gather_diag, total_[kStTiles], 512, 0, s, d_desc_,
+                st(kStTiles), nit);
+    else if (fill_tiles_ == 2)
+      BM_LAUNCH("fill_gather", k_fill_gather2, total_[kStTiles], 1024, 0, s, d_desc_,
                 st(kStTiles), nit);
     else
       BM_LAUNCH("fill_gather", k_fill_gather, total_[kStTiles], 512, 0, s, d_desc_,

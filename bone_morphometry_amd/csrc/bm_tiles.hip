@@ -259,23 +259,54 @@ __device__ __forceinline__ bool entry_candidate(uint32_t ent, int ux, int uy, in
   return ex * ex + ey * ey + ez * ez <= r2;
 }
 
-// kDiag: work counters and phase clocks into the item's diag buffer (a
-// separate instantiation, so the counters cost nothing otherwise).
-template <bool kDiag>
+// Exclusive scan of a[0..n) in place by the block, n <= blockDim.x (one entry
+// per thread); returns the total.
+__device__ uint32_t block_exclusive_scan_n(uint32_t* a, uint32_t n, uint32_t* s_tmp) {
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
+  const uint32_t v = t < n ? a[t] : 0u;
+  uint32_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) s_tmp[wv] = x;
+  __syncthreads();
+  uint32_t wave_off = 0, total = 0;
+  for (uint32_t i = 0; i < nw; ++i) {
+    if (i < wv) wave_off += s_tmp[i];
+    total += s_tmp[i];
+  }
+  if (t < n) a[t] = wave_off + x - v;
+  __syncthreads();
+  return total;
+}
+
+// One block per TXN x-adjacent tiles (a 8*TXN x 8 x 8 target box, 512*TXN
+// threads, one wavefront per 4x4x4 sub-brick): the tiles share the gathering
+// and sorting of their candidates.  kDiag: work counters and phase clocks into
+// the item's diag buffer (a separate instantiation, so the counters cost
+// nothing otherwise).
+template <bool kDiag, int TXN>
 __device__ __forceinline__ void
 k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                 uint32_t nitems) {
-  __shared__ uint32_t s_pos[kFillCap], s_r2[kFillCap];  // one round, sorted by bucket
-  __shared__ uint32_t s_cnt[kBuckets];                  // histogram, then cursors
-  __shared__ uint32_t s_start[kBuckets + 1];            // bucket starts over all rounds
-  __shared__ uint16_t s_len[kMaxNeighbours];            // useful list prefix of a tile in reach
-  __shared__ uint32_t s_org[kMaxNeighbours];            // its tile coordinates, packed
-  __shared__ uint16_t s_cmap[kChunkMap];                // chunk -> tile in reach
-  __shared__ uint32_t s_off[kBuckets];                  // chunk (or entry) offsets
-  __shared__ uint32_t s_spos[kStageCap], s_sr2[kStageCap];  // candidates in list order
-  __shared__ uint32_t s_tmp[16];
+  constexpr int NW = 8 * TXN;                // wavefronts
+  constexpr int kCap = kFillCap * TXN;       // sorted candidates per round in LDS
+  constexpr int kStage = kStageCap * TXN;    // staged candidates
+  constexpr int kNbr = kMaxNeighbours * TXN;  // tiles in reach
+  constexpr int kCmap = kChunkMap * TXN;     // chunks mapped to their tile
+  __shared__ uint32_t s_pos[kCap], s_r2[kCap];  // one round, sorted by bucket
+  __shared__ uint32_t s_cnt[kBuckets];          // histogram, then cursors
+  __shared__ uint32_t s_start[kBuckets + 1];    // bucket starts over all rounds
+  __shared__ uint16_t s_len[kNbr];              // useful list prefix of a tile in reach
+  __shared__ uint32_t s_org[kNbr];              // its tile coordinates, packed
+  __shared__ uint16_t s_cmap[kCmap];            // chunk -> tile in reach
+  __shared__ uint32_t s_off[kNbr];              // chunk offsets
+  __shared__ uint32_t s_spos[kStage], s_sr2[kStage];  // candidates in list order
+  __shared__ uint32_t s_tmp[NW];
   __shared__ uint32_t s_nn, s_over, s_pending, s_nst;
-  __shared__ unsigned long long s_tgt[8];
+  __shared__ unsigned long long s_tgt[8 * TXN];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ;
@@ -285,42 +316,60 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
   BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);  // the Y-pass buffer is free once the Z pass has run
   BM_GLOBAL uint32_t* __restrict__ overflow = gbl(I.overflow);
-  const uint32_t cap = I.fill_cap;
+  const uint32_t cap = min(I.fill_cap * TXN, (uint32_t)kCap);
   constexpr bool diag = kDiag;
   BM_GLOBAL unsigned long long* __restrict__ diag_buf = gbl(I.diag);
   if (!st->has_site) return;
   const unsigned long long clk0 = diag ? wall_clock64() : 0ull;
   unsigned long long clk_p3 = 0, clk_p4 = 0;
-  const bool grouped = I.tile_order & 2u;
-  const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
-  const uint32_t jpos = xcd_tile(vb, npos);
-  if (jpos >= npos) return;
-  const uint32_t t = tile_at(jpos, TX, TY, TZ, grouped);
-  if (t == kNoTile) return;
-  const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+  uint32_t t, tx, ty, tz;  // first tile of the block
+  if (TXN == 1) {
+    const bool grouped = I.tile_order & 2u;
+    const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
+    const uint32_t jpos = xcd_tile(vb, npos);
+    if (jpos >= npos) return;
+    t = tile_at(jpos, TX, TY, TZ, grouped);
+    if (t == kNoTile) return;
+    tx = t % TX;
+    ty = (t / TX) % TY;
+    tz = t / (TX * TY);
+  } else {
+    const uint32_t TG = (TX + TXN - 1) / TXN;  // groups of TXN tiles along x
+    const uint32_t npos = TG * TY * TZ;
+    const uint32_t jpos = xcd_tile(vb, npos);
+    if (jpos >= npos) return;
+    tx = (jpos % TG) * TXN;
+    ty = (jpos / TG) % TY;
+    tz = jpos / (TG * TY);
+    t = (tz * TY + ty) * TX + tx;
+  }
+  const uint32_t ntx = min((uint32_t)TXN, TX - tx);  // tiles of the block
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
-  const int hx = min(ox + 7, (int)W - 1), hy = min(oy + 7, (int)H - 1), hz = min(oz + 7, (int)D - 1);
+  const int hx = min(ox + 8 * TXN - 1, (int)W - 1), hy = min(oy + 7, (int)H - 1),
+            hz = min(oz + 7, (int)D - 1);
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t max_r2 = st->max_d2;
   const int R = (int)ceil_sqrt_u32(max_r2);
   if (R > (int)kGatherMaxRadius) return;  // the scatter fill handles this volume
-  if (threadIdx.x < 8) s_tgt[threadIdx.x] = masks[(size_t)t * 16 + 8 + threadIdx.x];
+  if (threadIdx.x < 8 * TXN)
+    s_tgt[threadIdx.x] = (threadIdx.x >> 3) < ntx
+                             ? masks[(size_t)(t + (threadIdx.x >> 3)) * 16 + 8 + (threadIdx.x & 7)]
+                             : 0ull;
   if (threadIdx.x == 0) {
     s_nn = 0;
     s_over = 0;
     s_nst = 0;
   }
-  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) {
-    s_cnt[i] = 0;
-    s_off[i] = 0;
-  }
+  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_cnt[i] = 0;
   __syncthreads();
   const unsigned long long ta = diag ? wall_clock64() : 0ull;
   bool any_target = false;
-  for (int i = 0; i < 8; ++i) any_target |= s_tgt[i] != 0;
-  const int sx = (int)(wv & 1) * 4, sy = (int)((wv >> 1) & 1) * 4, sz = (int)(wv >> 2) * 4;
+  for (int i = 0; i < 8 * TXN; ++i) any_target |= s_tgt[i] != 0;
+  const int sx = (int)(wv % (2 * TXN)) * 4, sy = (int)((wv / (2 * TXN)) & 1) * 4,
+            sz = (int)(wv / (4 * TXN)) * 4;
   const int vx = sx + (int)(lane & 3), vy = sy + (int)((lane >> 2) & 3), vz = sz + (int)(lane >> 4);
-  const uint32_t li = (uint32_t)(vx + 8 * vy + 64 * vz);
+  const uint32_t li = (uint32_t)((vx & 7) + 8 * vy + 64 * vz);  // index within its tile
+  const uint32_t tgt_word = 8 * (uint32_t)(vx >> 3) + (li >> 6);
   const bool valid = ox + vx < (int)W && oy + vy < (int)H && oz + vz < (int)D;
   const size_t gidx = ((size_t)(oz + vz) * H + (oy + vy)) * W + (ox + vx);
   if (!any_target) {
@@ -329,20 +378,21 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   }
   // P1: tiles in reach and the length of their useful list prefix.
   const int K = (R + 7) / 8;
-  const uint32_t side = 2 * K + 1;
-  // side <= 8 (K <= 3, the common case): 3-bit fields of i instead of divisions
-  const uint32_t nnb = side <= 8 ? 512u : side * side * side;
+  const uint32_t side = 2 * K + 1, sidex = 2 * K + TXN;
+  // sides <= 8 (the common case): 3-bit fields of i instead of divisions
+  const bool small = sidex <= 8;
+  const uint32_t nnb = small ? 512u : sidex * side * side;
   for (uint32_t i = threadIdx.x; i < nnb; i += blockDim.x) {
     int ix, iy, iz;
-    if (side <= 8) {
+    if (small) {
       ix = (int)(i & 7);
       iy = (int)((i >> 3) & 7);
       iz = (int)(i >> 6);
-      if (ix >= (int)side || iy >= (int)side || iz >= (int)side) continue;
+      if (ix >= (int)sidex || iy >= (int)side || iz >= (int)side) continue;
     } else {
-      ix = (int)(i % side);
-      iy = (int)((i / side) % side);
-      iz = (int)(i / (side * side));
+      ix = (int)(i % sidex);
+      iy = (int)((i / sidex) % side);
+      iz = (int)(i / (sidex * side));
     }
     const int ux = (int)tx + ix - K, uy = (int)ty + iy - K, uz = (int)tz + iz - K;
     if (ux < 0 || uy < 0 || uz < 0 || ux >= (int)TX || uy >= (int)TY || uz >= (int)TZ) continue;
@@ -360,7 +410,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const uint32_t len = reach_len(meta + u, mcount, bd2);
     if (len == 0) continue;
     const uint32_t slot = atomicAdd(&s_nn, 1u);
-    if (slot < kMaxNeighbours) {
+    if (slot < (uint32_t)kNbr) {
       s_len[slot] = (uint16_t)len;
       s_org[slot] = (uint32_t)ux | ((uint32_t)uy << 10) | ((uint32_t)uz << 20);
     } else {
@@ -370,8 +420,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __syncthreads();
   const unsigned long long tb = diag ? wall_clock64() : 0ull;
   if (s_over) {  // too many tiles in reach: exact slow path
-    if (threadIdx.x == 0) {
-      overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t;
+    if (threadIdx.x < ntx) {
+      overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t + threadIdx.x;
       atomicAdd((uint32_t*)&st->overflow_tiles, 1u);
     }
     return;
@@ -379,9 +429,9 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const uint32_t nn = s_nn;
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = (s_len[j] + 63) >> 6;
   __syncthreads();
-  const uint32_t nchunks = block_exclusive_scan_buckets(s_off, s_tmp);  // s_off: chunk offsets
+  const uint32_t nchunks = block_exclusive_scan_n(s_off, nn, s_tmp);  // s_off: chunk offsets
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
-    const uint32_t c0 = s_off[j], c1 = min(c0 + ((s_len[j] + 63u) >> 6), (uint32_t)kChunkMap);
+    const uint32_t c0 = s_off[j], c1 = min(c0 + ((s_len[j] + 63u) >> 6), (uint32_t)kCmap);
     for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = (uint16_t)j;
   }
   __syncthreads();
@@ -389,19 +439,19 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   unsigned long long td = tc;
   const uint32_t gshift = bucket_shift(max_r2);
   const uint32_t gtop = max_r2 >> gshift;
-  // Chunks q = wv, wv + 8, ... of the flattened (neighbour, 64 entries) range;
-  // one chunk is a sorted run of one list.  Read four chunks per trip.
+  // Chunks q = wv, wv + NW, ... of the flattened (neighbour, 64 entries)
+  // range; one chunk is a sorted run of one list.  Read four chunks per trip.
   auto chunk_pass = [&](auto&& use) {
-    for (uint32_t q0 = wv; q0 < nchunks; q0 += 32) {
+    for (uint32_t q0 = wv; q0 < nchunks; q0 += 4 * NW) {
       uint32_t ent[4], org[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const uint32_t q = q0 + 8u * k;
+        const uint32_t q = q0 + (uint32_t)NW * k;
         ent[k] = 0;
         org[k] = 0;
         if (q < nchunks) {
           uint32_t cur;
-          if (q < (uint32_t)kChunkMap) {
+          if (q < (uint32_t)kCmap) {
             cur = s_cmap[q];
           } else {  // last slot with s_off <= q (offsets increase strictly)
             uint32_t lo = 0, hi = nn - 1;
@@ -421,7 +471,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (q0 + 8u * k >= nchunks) break;  // uniform over the wave
+        if (q0 + (uint32_t)NW * k >= nchunks) break;  // uniform over the wave
         uint32_t pos = 0, r2 = 0;
         const bool keep = ent[k] != 0 &&
                           entry_candidate(ent[k], (int)(org[k] & 1023), (int)((org[k] >> 10) & 1023),
@@ -445,7 +495,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       if (lane == 0) k0 = atomicAdd(&s_nst, (uint32_t)__popcll(km));
       k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k0);
       const uint32_t k = k0 + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
-      if (keep && k < (uint32_t)kStageCap) {
+      if (keep && k < (uint32_t)kStage) {
         s_spos[k] = pos;
         s_sr2[k] = r2;
       }
@@ -453,7 +503,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   });
   __syncthreads();
   const uint32_t nst = s_nst;
-  const bool staged = nst <= (uint32_t)kStageCap;  // else the placement re-reads the lists
+  const bool staged = nst <= (uint32_t)kStage;  // else the placement re-reads the lists
   // Staged candidates, 64 per wave and trip (the trip count is uniform over a wave).
   auto staged_pass = [&](auto&& use) {
     for (uint32_t i0 = wv * 64; i0 < nst; i0 += blockDim.x) {
@@ -462,13 +512,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       use(keep, keep ? s_spos[i] : 0u, keep ? s_sr2[i] : 0u);
     }
   };
-  const uint32_t n = block_exclusive_scan_buckets(s_cnt, s_tmp);
+  const uint32_t n = block_exclusive_scan_n(s_cnt, kBuckets, s_tmp);
   for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_start[i] = s_cnt[i];
   if (threadIdx.x == 0) s_start[kBuckets] = n;
   __syncthreads();
   const unsigned long long clk1 = diag ? wall_clock64() : 0ull;
   // Rounds over bucket ranges that fit in LDS, largest radii first.
-  const bool target = (s_tgt[li >> 6] >> (li & 63)) & 1ull;
+  const bool target = (s_tgt[tgt_word] >> (li & 63)) & 1ull;
   uint32_t m = 0;
   bool done = __ballot(target) == 0;
   uint32_t held = 0xFFFFFFFFu;  // bucket key that must be finished before leaving
@@ -478,8 +528,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     ++rounds;
     uint32_t b1 = b0 + 1;
     if (s_start[b1] - s_start[b0] > cap) {
-      if (threadIdx.x == 0) {  // one bucket alone exceeds LDS: exact slow path
-        overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t;
+      if (threadIdx.x < ntx) {  // one bucket alone exceeds LDS: exact slow path
+        overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t + threadIdx.x;
         atomicAdd((uint32_t*)&st->overflow_tiles, 1u);
       }
       return;
@@ -622,7 +672,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // Work counters and phase clocks (diagnostics only): plain stores into this
   // block's slot, summed afterwards by k_reduce_diag.
   if (diag && lane == 0) {
-    auto slot = diag_buf + ((size_t)vb * 8 + wv) * 16;
+    auto slot = diag_buf + ((size_t)vb * NW + wv) * 16;
     slot[0] = 1;  // waves that reached the end
     slot[1] = tests;
     slot[2] = scanned;
@@ -645,12 +695,17 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
 __global__ void __launch_bounds__(512, 8)
 k_fill_gather(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
               uint32_t nitems) {
-  k_fill_gather_t<false>(items, starts, nitems);
+  k_fill_gather_t<false, 1>(items, starts, nitems);
+}
+__global__ void __launch_bounds__(1024, 8)
+k_fill_gather2(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+               uint32_t nitems) {
+  k_fill_gather_t<false, 2>(items, starts, nitems);
 }
 __global__ void __launch_bounds__(512)
 k_fill_gather_diag(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                    uint32_t nitems) {
-  k_fill_gather_t<true>(items, starts, nitems);
+  k_fill_gather_t<true, 1>(items, starts, nitems);
 }
 
 // Sum of the per-wave diagnostic slots (one block).

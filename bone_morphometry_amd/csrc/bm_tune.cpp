@@ -147,6 +147,7 @@ void parse_cache(const std::string& text) {
         get("z_halo", e.t.z_halo);
         get("fill_cap", e.t.fill_cap);
         get("tile_order", e.t.tile_order);
+        get("fill_tiles", e.t.fill_tiles);
         for (const auto& kv : m)
           if (kv.first.find("_ms") != std::string::npos) e.extra[kv.first] = kv.second;
         classes[cls] = e;
@@ -169,7 +170,7 @@ std::string dump_cache() {
       o << (first ? "" : ",\n") << "    \"" << c.first << "\": {\"y_rows\": " << t.y_rows
         << ", \"y_halo\": " << t.y_halo << ", \"z_rows\": " << t.z_rows
         << ", \"z_halo\": " << t.z_halo << ", \"fill_cap\": " << t.fill_cap
-        << ", \"tile_order\": " << t.tile_order;
+        << ", \"tile_order\": " << t.tile_order << ", \"fill_tiles\": " << t.fill_tiles;
       for (const auto& kv : c.second.extra) {
         char buf[64];
         snprintf(buf, sizeof buf, "%.4f", kv.second);
@@ -259,7 +260,7 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
   const std::string arch = device_arch();
   // (rows, halo) of the EDT tiles; a split line needs a chunk of at least kMinZChunk
   const uint32_t tile_c[][2] = {{64, 8}, {128, 16}, {128, 32}, {256, 32}, {512, 64}};
-  const uint32_t cap_c[] = {256, 512, 1024};
+  const uint32_t cap_c[] = {512, 1024};
   std::map<std::string, TuneEntry> found;
   for (const auto& smp : samples) {
     ItemSpec base;
@@ -309,6 +310,18 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
       if (ms < bf) {
         bf = ms;
         best.t.fill_cap = cap;
+      }
+    }
+    for (uint32_t tiles : {2u}) {  // fill blocks of two tiles against the one-tile default
+      ItemSpec sp = base;
+      sp.thickness = true;
+      sp.tune = best.t;
+      sp.tune.fill_tiles = tiles;
+      const double ms = time_kernel(e, sp, "fill_gather", 2);
+      if (verbose) fprintf(stderr, "[tune] %s fill_tiles %u: %.3f ms\n", shape_class(smp.dims).c_str(), tiles, ms);
+      if (ms < bf) {
+        bf = ms;
+        best.t.fill_tiles = tiles;
       }
     }
     best.extra["edt_y_ms"] = by;

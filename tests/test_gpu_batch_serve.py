@@ -151,7 +151,7 @@ def test_autotune_writes_cache_and_keeps_results(tmp_path):
     assert len(classes) >= 1
     for entry in classes.values():
         assert entry["y_rows"] in (64, 128, 256, 512) and entry["z_rows"] in (64, 128, 256, 512)
-        assert entry["fill_cap"] in (256, 512, 1024)
+        assert entry["fill_cap"] in (512, 1024) and entry["fill_tiles"] in (1, 2)
     bm.tuning_load(path)
     after = [_plain(it) for it in items]
     assert all(_same(a, b) for a, b in zip(after, before))
