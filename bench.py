@@ -53,11 +53,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain_512", choices=["plain_512", "plain_small"])
+    ap.add_argument("--workload", default="plain_512", choices=["plain_512", "plain_small", "large"])
     ap.add_argument("--mode", default="standard", choices=["standard", "classic"])
     ap.add_argument("--forms", default="plain,batched,serving",
                     help="comma list of plain, batched, serving")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-call", action="store_true",
+                    help="skip timing the host-pointer entry point")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample-s", type=float, default=15.0,
                     help="target seconds of CPU work for the baseline sample")
@@ -177,13 +179,15 @@ def plain_form(args, bm, native, hip, workloads, cpu_threads):
 
     # The host-pointer entry point (bm_analyze_thickness_spacing): inputs copied
     # over PCIe each call, for the PCIe-inclusive rate.
-    bvol = bm.BinaryVolume(wl.bone, wl.width, wl.height, wl.depth)
-    rvol = bm.BinaryVolume(wl.roi, wl.width, wl.height, wl.depth)
-    bm.analyze_thickness_spacing(bvol, rvol, wl.spacing, mode)
-    t1 = time.perf_counter()
-    for _ in range(3):
+    host_ms = None
+    if not args.no_host_call:
+        bvol = bm.BinaryVolume(wl.bone, wl.width, wl.height, wl.depth)
+        rvol = bm.BinaryVolume(wl.roi, wl.width, wl.height, wl.depth)
         bm.analyze_thickness_spacing(bvol, rvol, wl.spacing, mode)
-    host_ms = (time.perf_counter() - t1) / 3 * 1e3
+        t1 = time.perf_counter()
+        for _ in range(3):
+            bm.analyze_thickness_spacing(bvol, rvol, wl.spacing, mode)
+        host_ms = (time.perf_counter() - t1) / 3 * 1e3
 
     cpu = None
     parity = None
@@ -384,7 +388,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": wl.name, "shape": [wl.width, wl.height, wl.depth],
-                   "roi": "z-cylinder d=460" if wl.name == "plain_512" else "full",
+                   "roi": {"plain_512": "z-cylinder d=460",
+                           "large_1024x1024x1536": "ellipsoid 500x480x740"}.get(wl.name, "full"),
                    "spacing_um": wl.spacing[0], "mode": args.mode, "phases": 2},
         "roofline": p["roofline"],
         "cpu_baseline": p["cpu"],
@@ -392,8 +397,8 @@ def main():
                 "frac": (p["edt_gbs"] / HBM_PEAK_GBS) if p["edt_gbs"] else None},
         "kernels_ms_per_step": {k: v[1] / args.steps for k, v in p["kernels"].items()},
         "parity_vs_oracle": p["parity"],
-        "host_pointer_call": {"ms": p["host_ms"],
-                              "gvox_per_s_pcie_inclusive": wl.n / (p["host_ms"] * 1e-3) / 1e9},
+        "host_pointer_call": None if p["host_ms"] is None else {
+            "ms": p["host_ms"], "gvox_per_s_pcie_inclusive": wl.n / (p["host_ms"] * 1e-3) / 1e9},
         "tb_th_mean_um": p["res"].thickness.mean,
         "tb_sp_mean_um": p["res"].spacing.mean,
         "batched": batched,
