@@ -9,6 +9,13 @@ interface for this path (bone-morphometry 0.7.0-rc.3):
   local_thickness                     src/local_thickness.rs:16-44
   analyze_thickness_spacing           src/metrics.rs:64-132 (thickness/spacing part)
   FeatureMode                         src/metrics.rs:12-24
+  require_isotropic_spacing           src/sdf_curvature.rs:147-160
+  signed_distance_field_um            src/sdf_curvature.rs:162-180
+  gaussian_kernel                     src/sdf_curvature.rs:182-198
+  gaussian_smooth_nearest_in_place    src/sdf_curvature.rs:200-269
+  smoothed_signed_distance_field_um   the two above in sequence, as
+                                      src/sdf_curvature.rs:98-105 and
+                                      src/minkowski.rs:64-71 call them
 
 and two forms with no counterpart in the reference:
 
@@ -23,8 +30,12 @@ reference's message.
 """
 from .api import (Batch, BinaryVolume, BmError, DistributionStats, FeatureMode,  # noqa: F401
                   Phase, Server, ThicknessSpacing, analyze_thickness_spacing,
-                  autotune, euclidean_distance_transform_squared, local_thickness,
-                  local_thickness_stages, tuning_load)
+                  autotune, euclidean_distance_transform_squared, gaussian_kernel,
+                  gaussian_smooth_nearest_in_place, gaussian_smooth_nearest_in_place_device,
+                  local_thickness, local_thickness_stages, require_isotropic_spacing,
+                  signed_distance_field_um, signed_distance_field_um_device,
+                  smoothed_signed_distance_field_um, smoothed_signed_distance_field_um_device,
+                  tuning_load)
 from . import synth  # noqa: F401
 
 __version__ = "0.1.0"

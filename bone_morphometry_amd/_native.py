@@ -29,6 +29,10 @@ EXPORTS = (
     "bm_autotune", "bm_tuning_load",
     "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_stats_get",
     "bm_server_destroy",
+    "bm_require_isotropic_spacing", "bm_gaussian_kernel", "bm_signed_distance_field_um",
+    "bm_gaussian_smooth_nearest_in_place", "bm_smoothed_signed_distance_field_um",
+    "bm_signed_distance_field_um_device", "bm_gaussian_smooth_nearest_in_place_device",
+    "bm_smoothed_signed_distance_field_um_device",
 )
 
 
@@ -127,6 +131,15 @@ def lib():
                                      ctypes.POINTER(BmRequestTiming), ctypes.POINTER(I)]
         L.bm_server_stats_get.argtypes = [VP, ctypes.POINTER(BmServerStats)]
         L.bm_server_destroy.argtypes = [VP]
+        D = ctypes.c_double
+        L.bm_require_isotropic_spacing.argtypes = [ctypes.POINTER(D), ctypes.POINTER(D)]
+        L.bm_gaussian_kernel.argtypes = [D, P, U64, ctypes.POINTER(U64)]
+        L.bm_signed_distance_field_um.argtypes = [VOL, D, P, U64]
+        L.bm_gaussian_smooth_nearest_in_place.argtypes = [P, U64, U64, U64, U64, D]
+        L.bm_smoothed_signed_distance_field_um.argtypes = [VOL, D, D, P, U64]
+        L.bm_signed_distance_field_um_device.argtypes = [P, U64, U64, U64, D, P, VP]
+        L.bm_gaussian_smooth_nearest_in_place_device.argtypes = [P, U64, U64, U64, D, VP]
+        L.bm_smoothed_signed_distance_field_um_device.argtypes = [P, U64, U64, U64, D, D, P, VP]
         L.bm_profile_enable.argtypes = [I]
         L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
         for name in EXPORTS:

@@ -107,6 +107,76 @@ def local_thickness_stages(volume, phase):
                 ridge_count=rc.value, fill_visits=fv.value)
 
 
+def require_isotropic_spacing(spacing):
+    """src/sdf_curvature.rs:147-160 -> h (the common spacing) or BmError."""
+    sp = (ctypes.c_double * 3)(*[float(v) for v in spacing])
+    h = ctypes.c_double(0.0)
+    _check(_native.lib().bm_require_isotropic_spacing(sp, ctypes.byref(h)))
+    return h.value
+
+
+def gaussian_kernel(sigma):
+    """src/sdf_curvature.rs:182-198 -> np.ndarray[float64] of 2r+1 normalised weights."""
+    n = ctypes.c_uint64(0)
+    _check(_native.lib().bm_gaussian_kernel(ctypes.c_double(sigma), None, 0, ctypes.byref(n)))
+    out = np.empty(n.value, np.float64)
+    _check(_native.lib().bm_gaussian_kernel(ctypes.c_double(sigma), out.ctypes.data, out.size,
+                                            ctypes.byref(n)))
+    return out
+
+
+def signed_distance_field_um(bone, h):
+    """src/sdf_curvature.rs:162-180 -> np.ndarray[float64]: +sqrt(d2) h inside, -sqrt(d2') h out."""
+    out = np.empty(bone.data.size, np.float64)
+    v = bone._c()
+    _check(_native.lib().bm_signed_distance_field_um(ctypes.byref(v), ctypes.c_double(h),
+                                                     out.ctypes.data, out.size))
+    return out
+
+
+def gaussian_smooth_nearest_in_place(data, width, height, depth, sigma_vox):
+    """src/sdf_curvature.rs:200-269 on a C-contiguous float64 array, in place."""
+    if not (isinstance(data, np.ndarray) and data.dtype == np.float64
+            and data.flags.c_contiguous and data.flags.writeable):
+        raise TypeError("data must be a writeable C-contiguous float64 array")
+    _check(_native.lib().bm_gaussian_smooth_nearest_in_place(
+        data.ctypes.data, data.size, int(width), int(height), int(depth),
+        ctypes.c_double(sigma_vox)))
+    return data
+
+
+def smoothed_signed_distance_field_um(bone, h, sigma_vox):
+    """signed_distance_field_um then gaussian_smooth_nearest_in_place, as the callers at
+    src/sdf_curvature.rs:98-105 and src/minkowski.rs:64-71 make them."""
+    out = np.empty(bone.data.size, np.float64)
+    v = bone._c()
+    _check(_native.lib().bm_smoothed_signed_distance_field_um(
+        ctypes.byref(v), ctypes.c_double(h), ctypes.c_double(sigma_vox), out.ctypes.data,
+        out.size))
+    return out
+
+
+def signed_distance_field_um_device(d_bone_ptr, width, height, depth, h, d_out_ptr, stream=None):
+    """Device-resident form (device pointers, e.g. torch tensor.data_ptr())."""
+    _check(_native.lib().bm_signed_distance_field_um_device(
+        d_bone_ptr, width, height, depth, ctypes.c_double(h), d_out_ptr, stream))
+
+
+def gaussian_smooth_nearest_in_place_device(d_data_ptr, width, height, depth, sigma_vox,
+                                            stream=None):
+    """Device-resident form: smooths width*height*depth float64 values in place."""
+    _check(_native.lib().bm_gaussian_smooth_nearest_in_place_device(
+        d_data_ptr, width, height, depth, ctypes.c_double(sigma_vox), stream))
+
+
+def smoothed_signed_distance_field_um_device(d_bone_ptr, width, height, depth, h, sigma_vox,
+                                             d_out_ptr, stream=None):
+    """Device-resident form of smoothed_signed_distance_field_um."""
+    _check(_native.lib().bm_smoothed_signed_distance_field_um_device(
+        d_bone_ptr, width, height, depth, ctypes.c_double(h), ctypes.c_double(sigma_vox),
+        d_out_ptr, stream))
+
+
 @dataclasses.dataclass
 class DistributionStats:
     """src/metrics.rs:26-38 (+ count)."""

@@ -26,6 +26,7 @@
 #include "bm_analyze.h"
 #include "bm_engine.h"
 #include "bm_serve.h"
+#include "bm_smooth.h"
 
 namespace bm {
 
@@ -579,6 +580,113 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
               d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[13],
               d[14], d[15], so.status.overflow_tiles);
     }
+  });
+}
+
+// ---- signed distance field and Gaussian smoothing ---------------------------
+int bm_require_isotropic_spacing(const double spacing[3], double* h) {
+  return guarded([&] {
+    if (!spacing || !h) throw Error("null argument");
+    *h = require_isotropic_spacing(spacing);
+  });
+}
+
+int bm_gaussian_kernel(double sigma, double* out, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    if (!len || (!out && cap)) throw Error("null argument");
+    const std::vector<double> k = gaussian_kernel(sigma);
+    *len = k.size();
+    std::memcpy(out, k.data(), std::min<uint64_t>(cap, k.size()) * sizeof(double));
+  });
+}
+
+int bm_signed_distance_field_um(const bm_volume* bone, double h, double* out, uint64_t out_len) {
+  return guarded([&] {
+    BinaryVolume v = from_c(bone);
+    const Dims dims = device_dims(v.width, v.height, v.depth);
+    check_out_len(out_len, v.len());
+    const uint64_t n = dims.n();
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    uint8_t* d_in = static_cast<uint8_t*>(e.scratch_bytes(n, 0));
+    double* d_out = static_cast<double*>(e.scratch_bytes(n * sizeof(double), 2));
+    BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
+    signed_distance_field_device(e, d_in, dims, h, d_out, s);
+    BM_HIP(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int bm_gaussian_smooth_nearest_in_place(double* data, uint64_t len, uint64_t width,
+                                        uint64_t height, uint64_t depth, double sigma_vox) {
+  return guarded([&] {
+    const Dims dims = smooth_dims(width, height, depth);
+    if (len != dims.n())
+      throw Error(fmt("data holds %llu values, the volume has %llu voxels",
+                      (unsigned long long)len, (unsigned long long)dims.n()));
+    if (!data && len) throw Error("null data");
+    if (gaussian_kernel(sigma_vox).size() <= 1 || len == 0) return;
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    double* d = static_cast<double*>(e.scratch_bytes(len * sizeof(double), 0));
+    BM_HIP(hipMemcpyAsync(d, data, len * sizeof(double), hipMemcpyHostToDevice, s));
+    gaussian_smooth_device(e, d, dims, sigma_vox, s);
+    BM_HIP(hipMemcpyAsync(data, d, len * sizeof(double), hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int bm_smoothed_signed_distance_field_um(const bm_volume* bone, double h, double sigma_vox,
+                                         double* out, uint64_t out_len) {
+  return guarded([&] {
+    BinaryVolume v = from_c(bone);
+    const Dims dims = device_dims(v.width, v.height, v.depth);
+    check_out_len(out_len, v.len());
+    const uint64_t n = dims.n();
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    uint8_t* d_in = static_cast<uint8_t*>(e.scratch_bytes(n, 0));
+    BM_HIP(hipMemcpyAsync(d_in, v.data, n, hipMemcpyHostToDevice, s));
+    const double* d_out =
+        smoothed_signed_distance_field_device(e, d_in, dims, h, sigma_vox, nullptr, s);
+    BM_HIP(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int bm_signed_distance_field_um_device(const uint8_t* d_bone, uint64_t width, uint64_t height,
+                                       uint64_t depth, double h, double* d_out, void* stream) {
+  return guarded([&] {
+    const Dims dims = device_dims(width, height, depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    signed_distance_field_device(e, d_bone, dims, h, d_out, stream_or_default(e, stream));
+  });
+}
+
+int bm_gaussian_smooth_nearest_in_place_device(double* d_data, uint64_t width, uint64_t height,
+                                               uint64_t depth, double sigma_vox, void* stream) {
+  return guarded([&] {
+    const Dims dims = smooth_dims(width, height, depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    gaussian_smooth_device(e, d_data, dims, sigma_vox, stream_or_default(e, stream));
+  });
+}
+
+int bm_smoothed_signed_distance_field_um_device(const uint8_t* d_bone, uint64_t width,
+                                                uint64_t height, uint64_t depth, double h,
+                                                double sigma_vox, double* d_out, void* stream) {
+  return guarded([&] {
+    const Dims dims = device_dims(width, height, depth);
+    if (!d_out) throw Error("null output");
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    smoothed_signed_distance_field_device(e, d_bone, dims, h, sigma_vox, d_out,
+                                          stream_or_default(e, stream));
   });
 }
 

@@ -298,7 +298,7 @@ class Engine {
   hipStream_t stream_ = nullptr;
   std::mutex mu_;
   Plan plain_;
-  Arena stage_[3];
+  Arena stage_[4];  // slots 0-2: staging and scratch volumes; 3: small tables
 };
 
 // Per-kernel timing with HIP events recorded on the launching stream (off by

@@ -138,6 +138,48 @@ int bm_local_thickness_stages(const bm_volume *volume, int phase, uint32_t *squa
                               uint8_t *ridge_flags, uint32_t *propagated, float *thickness,
                               uint64_t *ridge_count, uint64_t *fill_visits);
 
+/* ---- The signed distance field and its Gaussian smoothing, the EDT consumers
+ * of the curvature and Minkowski measures (src/sdf_curvature.rs:75-105,
+ * src/minkowski.rs:54-71).  Outputs are f64, z-major like the volumes. */
+
+/* Replaces require_isotropic_spacing(spacing) -> Result<f64>,
+ * src/sdf_curvature.rs:147-160: *h = spacing[0] or BM_ERR_INVALID with the
+ * reference's message. */
+int bm_require_isotropic_spacing(const double spacing[3], double *h);
+
+/* Replaces gaussian_kernel(sigma) -> Vec<f64>, src/sdf_curvature.rs:182-198:
+ * writes min(len, cap) weights and sets *len = 2r+1 (r = floor(4 sigma + 0.5);
+ * 1 for sigma <= 0 or NaN).  BM_ERR_UNSUPPORTED beyond r = 2^20. */
+int bm_gaussian_kernel(double sigma, double *out, uint64_t cap, uint64_t *len);
+
+/* Replaces signed_distance_field_um(&BinaryVolume, h) -> Result<Vec<f64>>,
+ * src/sdf_curvature.rs:162-180: +sqrt(d2) h on bone voxels, -sqrt(d2') h
+ * elsewhere (d2, d2' the squared EDTs of the two phases).  The errors are the
+ * EDT's: the foreground transform's first. */
+int bm_signed_distance_field_um(const bm_volume *bone, double h, double *out, uint64_t out_len);
+
+/* Replaces gaussian_smooth_nearest_in_place(&mut Vec<f64>, width, height,
+ * depth, sigma_vox), src/sdf_curvature.rs:200-269: x, y then z passes with
+ * clamped indices, in place.  data holds len == width*height*depth values. */
+int bm_gaussian_smooth_nearest_in_place(double *data, uint64_t len, uint64_t width,
+                                        uint64_t height, uint64_t depth, double sigma_vox);
+
+/* The two calls above in sequence, as src/sdf_curvature.rs:98-105,
+ * src/minkowski.rs:64-71 and src/sdf_curvature.rs:719-726 make them; the
+ * unsmoothed field never leaves the device. */
+int bm_smoothed_signed_distance_field_um(const bm_volume *bone, double h, double sigma_vox,
+                                         double *out, uint64_t out_len);
+
+/* Device-resident forms (device pointers; stream may be NULL).  They return
+ * when the output is complete. */
+int bm_signed_distance_field_um_device(const uint8_t *d_bone, uint64_t width, uint64_t height,
+                                       uint64_t depth, double h, double *d_out, void *stream);
+int bm_gaussian_smooth_nearest_in_place_device(double *d_data, uint64_t width, uint64_t height,
+                                               uint64_t depth, double sigma_vox, void *stream);
+int bm_smoothed_signed_distance_field_um_device(const uint8_t *d_bone, uint64_t width,
+                                                uint64_t height, uint64_t depth, double h,
+                                                double sigma_vox, double *d_out, void *stream);
+
 /* ---- Batched form (no counterpart in the reference: many analyze() calls,
  * src/metrics.rs:64-132, in one launch sequence).  The volumes are
  * device-resident, of any shapes; each is analyzed exactly as by

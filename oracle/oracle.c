@@ -34,6 +34,16 @@
  *   oracle_nonzero_stats     known answer metrics.rs:315-326 + closed forms
  *   oracle_quantile          known answer metrics.rs:304-313 + numpy linear
  *                            quantiles
+ *   oracle_gaussian_kernel   known answer sdf_curvature.rs:783-789 (sums to
+ *                            1), scipy's truncate-4 Gaussian weights
+ *   oracle_gaussian_smooth_nearest
+ *                            known answer sdf_curvature.rs:791-796 (constant
+ *                            field kept), impulse response = w[kz](w[ky]w[kx])
+ *                            exactly, scipy.ndimage.gaussian_filter with
+ *                            mode "nearest" (edges), brute-force order check
+ *   oracle_signed_distance_field_um
+ *                            brute-force distances on small volumes (sign,
+ *                            phase and scale), the EDT's error precedence
  */
 #include <math.h>
 #include <stdint.h>
@@ -580,4 +590,120 @@ done:
   free(ab);
   free(am);
   return rc;
+}
+
+/* ------------------------------------------------------------------------
+ * The signed distance field and its Gaussian smoothing (the EDT consumers of
+ * the curvature and Minkowski measures).
+ */
+
+/* gaussian_kernel, src/sdf_curvature.rs:182-198.  Writes min(len, cap)
+ * weights; returns len = 2r+1 (0 with the error set when r exceeds 2^20).
+ * `(4.0 * sigma + 0.5).floor() as isize` saturates in Rust and maps NaN to 0;
+ * exp is the C library's, the one Rust's f64::exp calls on this platform. */
+size_t oracle_gaussian_kernel(double sigma, double *out, size_t cap) {
+  if (sigma <= 0.0) {
+    if (cap) out[0] = 1.0;
+    return 1;
+  }
+  double rf = floor(4.0 * sigma + 0.5);
+  long radius = 0;
+  if (!isnan(rf)) {
+    if (rf > (double)(1 << 20)) {
+      fail("oracle: gaussian radius too large");
+      return 0;
+    }
+    radius = (long)rf;
+  }
+  size_t len = (size_t)(2 * radius + 1);
+  double *k = (double *)malloc(len * sizeof(double));
+  double sum = 0.0;
+  for (long i = -radius; i <= radius; ++i) {
+    double x = (double)i;
+    double w = exp(-0.5 * x * x / (sigma * sigma));
+    k[i + radius] = w;
+    sum += w;
+  }
+  for (size_t i = 0; i < len; ++i) k[i] /= sum;
+  if (cap) memcpy(out, k, (len < cap ? len : cap) * sizeof(double));
+  free(k);
+  return len;
+}
+
+static inline size_t clamp_index(long v, size_t n) {
+  return v < 0 ? 0 : (v >= (long)n ? n - 1 : (size_t)v);
+}
+
+/* gaussian_smooth_nearest_in_place, src/sdf_curvature.rs:200-269: x, y, z
+ * passes, each output acc = 0; acc += w[k] * data[clamp(p + k - r)] in tap
+ * order.  The reference runs serially; the outputs are independent, so the
+ * z-slices are spread over threads without changing any value. */
+int oracle_gaussian_smooth_nearest(double *data, size_t width, size_t height, size_t depth,
+                                   double sigma_vox) {
+  size_t len = oracle_gaussian_kernel(sigma_vox, NULL, 0);
+  if (len == 0) return 1;
+  if (len == 1) return 0;
+  double *kernel = (double *)malloc(len * sizeof(double));
+  oracle_gaussian_kernel(sigma_vox, kernel, len);
+  long radius = (long)(len / 2);
+  size_t n = width * height * depth;
+  double *tmp = (double *)malloc((n ? n : 1) * sizeof(double));
+  if (!tmp || !kernel) {
+    free(tmp);
+    free(kernel);
+    return fail("oracle: out of host memory");
+  }
+#define IDX(x, y, z) (((z) * height + (y)) * width + (x))
+  for (int pass = 0; pass < 3; ++pass) {
+    const double *src = pass == 1 ? tmp : data;
+    double *dst = pass == 1 ? data : tmp;
+#pragma omp parallel for schedule(static)
+    for (long zl = 0; zl < (long)depth; ++zl) {
+      size_t z = (size_t)zl;
+      for (size_t y = 0; y < height; ++y)
+        for (size_t x = 0; x < width; ++x) {
+          double acc = 0.0;
+          for (size_t ki = 0; ki < len; ++ki) {
+            long dk = (long)ki - radius;
+            size_t i;
+            if (pass == 0) i = IDX(clamp_index((long)x + dk, width), y, z);
+            else if (pass == 1) i = IDX(x, clamp_index((long)y + dk, height), z);
+            else i = IDX(x, y, clamp_index((long)z + dk, depth));
+            acc += kernel[ki] * src[i];
+          }
+          dst[IDX(x, y, z)] = acc;
+        }
+    }
+  }
+#undef IDX
+  /* three swaps: the z pass's output (tmp) is the result */
+  memcpy(data, tmp, n * sizeof(double));
+  free(tmp);
+  free(kernel);
+  return 0;
+}
+
+/* signed_distance_field_um, src/sdf_curvature.rs:162-180: the foreground EDT
+ * (its error first), +sqrt(d2) h on bone voxels; then the background EDT,
+ * -sqrt(d2) h on the others. */
+int oracle_signed_distance_field_um(const uint8_t *bone, size_t w, size_t h, size_t d,
+                                    double hs, double *phi) {
+  size_t n = w * h * d;
+  uint32_t *sq = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  if (!sq) return fail("oracle: out of host memory");
+  for (size_t i = 0; i < n; ++i) phi[i] = 0.0;
+  if (oracle_edt_squared(bone, w, h, d, 0, sq)) {
+    free(sq);
+    return 1;
+  }
+  for (size_t i = 0; i < n; ++i)
+    if (bone[i] != 0) phi[i] = sqrt((double)sq[i]) * hs;
+  if (oracle_edt_squared(bone, w, h, d, 1, sq)) {
+    free(sq);
+    return 1;
+  }
+  for (size_t i = 0; i < n; ++i)
+    if (bone[i] == 0) phi[i] = -sqrt((double)sq[i]) * hs;
+  free(sq);
+  return 0;
 }

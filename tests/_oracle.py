@@ -140,3 +140,44 @@ def set_threads(n):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def _bind_smoothing():
+    L = lib()
+    if getattr(L, "_smoothing_bound", False):
+        return L
+    L.oracle_gaussian_kernel.argtypes = [D, P, Z]
+    L.oracle_gaussian_kernel.restype = Z
+    L.oracle_gaussian_smooth_nearest.argtypes = [P, Z, Z, Z, D]
+    L.oracle_signed_distance_field_um.argtypes = [P, Z, Z, Z, D, P]
+    L._smoothing_bound = True
+    return L
+
+
+def gaussian_kernel(sigma):
+    """gaussian_kernel, src/sdf_curvature.rs:182-198."""
+    L = _bind_smoothing()
+    n = L.oracle_gaussian_kernel(float(sigma), None, 0)
+    if n == 0:
+        raise OracleError(L.oracle_last_error().decode())
+    out = np.empty(n, np.float64)
+    L.oracle_gaussian_kernel(float(sigma), out.ctypes.data, n)
+    return out
+
+
+def gaussian_smooth_nearest(data, w, h, d, sigma_vox):
+    """gaussian_smooth_nearest_in_place, src/sdf_curvature.rs:200-269, on a copy."""
+    L = _bind_smoothing()
+    out = np.array(data, dtype=np.float64, copy=True).reshape(-1)
+    _chk(L.oracle_gaussian_smooth_nearest(out.ctypes.data, w, h, d, float(sigma_vox)))
+    return out
+
+
+def signed_distance_field_um(bone, w, h, d, hs):
+    """signed_distance_field_um, src/sdf_curvature.rs:162-180."""
+    L = _bind_smoothing()
+    bone = np.ascontiguousarray(bone, np.uint8).reshape(-1)
+    out = np.empty(bone.size, np.float64)
+    _chk(L.oracle_signed_distance_field_um(bone.ctypes.data, w, h, d, float(hs),
+                                           out.ctypes.data))
+    return out

@@ -85,3 +85,37 @@ def test_tuning_cache_is_parsed_without_a_device(tmp_path):
         bm.tuning_load(bad)
     assert e.value.code == _native.BM_ERR_INVALID and "tuning cache" in str(e.value)
     bm.tuning_load(tmp_path / "absent.json")  # no file: the defaults
+
+
+# require_isotropic_spacing, src/sdf_curvature.rs:147-160 (host side, no device)
+def test_require_isotropic_spacing_messages():
+    assert bm.require_isotropic_spacing((10.5, 10.5, 10.5)) == 10.5
+    assert bm.require_isotropic_spacing((8.0, 8.0 + 5e-9, 8.0 - 5e-9)) == 8.0  # within 1e-9*h
+    with pytest.raises(bm.BmError, match="^voxel spacing must be finite and positive$"):
+        bm.require_isotropic_spacing((1.0, math.nan, 1.0))
+    with pytest.raises(bm.BmError, match="voxel spacing must be finite and positive"):
+        bm.require_isotropic_spacing((0.0, 1.0, 1.0))
+    # Rust's Display of f64: no exponent, shortest digits, no trailing ".0"
+    with pytest.raises(bm.BmError) as e:
+        bm.require_isotropic_spacing((1.0, 1.5, 1.0))
+    assert str(e.value) == ("advanced SDF curvature currently requires isotropic spacing; "
+                            "got [1, 1.5, 1] µm")
+    with pytest.raises(bm.BmError) as e:
+        bm.require_isotropic_spacing((1e-7, 2.5e-7, 1e21))
+    assert str(e.value).endswith("got [0.0000001, 0.00000025, 1000000000000000000000] µm")
+
+
+# gaussian_kernel, src/sdf_curvature.rs:182-198 (host side): the library's
+# weights are the oracle's to the bit
+def test_gaussian_kernel_matches_oracle():
+    import _oracle as orc
+    for sigma in (0.125, 0.3, 0.5, 1.0, 1.5, 2.0, 2.857142857142857, 3.75, 7.1, 31.0, 120.5):
+        got = bm.gaussian_kernel(sigma)
+        want = orc.gaussian_kernel(sigma)
+        assert got.size == want.size and np.array_equal(got.view(np.uint64),
+                                                         want.view(np.uint64)), sigma
+    assert bm.gaussian_kernel(0.0).tolist() == [1.0]
+    assert math.isnan(bm.gaussian_kernel(math.nan)[0])
+    with pytest.raises(bm.BmError) as e:
+        bm.gaussian_kernel(1e7)
+    assert e.value.code == _native.BM_ERR_UNSUPPORTED
