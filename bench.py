@@ -14,6 +14,9 @@ Beside it:
   serving  a seeded Poisson stream (150 requests/s for 60 s, lognormal sizes
            of median 192^3) through the request loop with a 200 GB HBM budget:
            p50/p99 latency (completion - submission) and Gvoxel/s.
+  sdf      the signed distance field of the same bone volume smoothed with the
+           30 um Gaussian (signed_distance_field_um + gaussian_smooth_nearest_in_place),
+           per-pass HBM fractions, the oracle's time and bit-exact parity.
 
 The roofline object describes the kernel with the largest share of the plain
 step, from per-kernel HIP events recorded on the launching stream; the
@@ -55,8 +58,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_512", choices=["plain_512", "plain_small", "large"])
     ap.add_argument("--mode", default="standard", choices=["standard", "classic"])
-    ap.add_argument("--forms", default="plain,batched,serving",
-                    help="comma list of plain, batched, serving")
+    ap.add_argument("--forms", default="plain,batched,serving,sdf",
+                    help="comma list of plain, batched, serving, sdf")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host-call", action="store_true",
                     help="skip timing the host-pointer entry point")
@@ -222,6 +225,77 @@ def plain_form(args, bm, native, hip, workloads, cpu_threads):
                 host_ms=host_ms)
 
 
+# Algorithmic HBM bytes per voxel of the smoothing passes (f64): the x pass
+# reads the bone byte and the squared EDT of the voxel's phase and writes the
+# x-smoothed field; the y and z passes read and write one f64 each.
+SMOOTH_BYTES = {"smooth_x": 1.0 + 4.0 + 8.0, "smooth_y": 16.0, "smooth_z": 16.0}
+
+
+def sdf_form(args, bm, native, hip, wl, cpu_threads):
+    """signed_distance_field_um + gaussian_smooth_nearest_in_place on the bone
+    of the plain workload, 30 um bandwidth (the callers' default) at its
+    spacing: one bm_smoothed_signed_distance_field_um_device call per step."""
+    import numpy as np
+    hs = wl.spacing[0]
+    sigma_vox = 30.0 / hs
+    d_bone = hip.DeviceBuffer(wl.n)
+    d_bone.upload(wl.bone)
+    d_out = hip.DeviceBuffer(wl.n * 8)
+    stream = hip.stream()
+
+    def step():
+        bm.api.smoothed_signed_distance_field_um_device(
+            d_bone.value, wl.width, wl.height, wl.depth, hs, sigma_vox, d_out.value,
+            stream.value)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    hip.synchronize()
+    native.profile_read()
+    native.profile_enable(True)
+    e0, e1 = hip.Event(), hip.Event()
+    steps = max(1, args.steps // 2)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    hip.synchronize()
+    total_ms = e0.elapsed_ms(e1)
+    native.profile_enable(False)
+    kernels = native.profile_read()
+    ms = total_ms / steps
+    passes = {}
+    for k, b in SMOOTH_BYTES.items():
+        if k in kernels:
+            avg = kernels[k][1] / kernels[k][0]
+            gbs = b * wl.n / (avg * 1e-3) / 1e9
+            passes[k] = {"avg_launch_ms": avg, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_voxel": b}
+    got = np.empty(wl.n, np.float64)
+    d_out.download(got)
+    log(f"sdf: {steps} calls in {total_ms:.2f} ms")
+    cpu = parity = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as orc
+        orc.set_threads(cpu_threads)
+        t1 = time.time()
+        ref = orc.gaussian_smooth_nearest(
+            orc.signed_distance_field_um(wl.bone, wl.width, wl.height, wl.depth, hs),
+            wl.width, wl.height, wl.depth, sigma_vox)
+        cpu_s = time.time() - t1
+        cpu = {"value": wl.n / cpu_s / 1e9, "unit": "Gvoxel/s", "cores": cpu_threads,
+               "kind": "port",
+               "sample": f"the full {wl.name} bone volume, oracle (C/OpenMP restatement), "
+                         f"{cpu_s:.1f} s"}
+        parity = bool(np.array_equal(got.view(np.uint64), ref.view(np.uint64)))
+    return {"workload": f"{wl.name} bone, sigma 30 um at {hs} um (sigma_vox {sigma_vox:.4f}, "
+                        f"radius {len(bm.gaussian_kernel(sigma_vox)) // 2})",
+            "ms_per_call": ms, "gvox_per_s": wl.n / (ms * 1e-3) / 1e9,
+            "kernels_ms_per_call": {k: v[1] / steps for k, v in kernels.items()},
+            "smoothing_passes": passes, "cpu_baseline": cpu, "bit_exact_vs_oracle": parity}
+
+
 def batched_form(args, bm, hip, workloads, cpu_threads):
     t0 = time.time()
     vols = workloads.batched(threads=cpu_threads)
@@ -370,10 +444,23 @@ def main():
         return 1
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if "plain" not in forms:  # the smoothed field alone (profiling runs)
+        wl = getattr(workloads, args.workload)(threads=cpu_threads)
+        sdf = sdf_form(args, bm, _native, hip, wl, cpu_threads)
+        print(json.dumps({
+            "metric": "Gvoxel/s smoothed signed distance field", "value": sdf["gvox_per_s"],
+            "unit": "Gvoxel/s", "n_gpus": 1, "steps": max(1, args.steps // 2),
+            "warmup": max(1, args.warmup), "ms_per_step": sdf["ms_per_call"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": wl.name,
+                                            "shape": [wl.width, wl.height, wl.depth]},
+            "smoothed_sdf": sdf}))
+        return 0
     p = plain_form(args, bm, _native, hip, workloads, cpu_threads)
     batched = batched_form(args, bm, hip, workloads, cpu_threads) if "batched" in forms else None
     serving = serving_form(args, bm, workloads, cpu_threads) if "serving" in forms else None
     wl = p["wl"]
+    sdf = sdf_form(args, bm, _native, hip, wl, cpu_threads) if "sdf" in forms else None
     line = {
         "metric": "Gvoxel/s EDT+local thickness at 512^3",
         "value": p["gvox"],
@@ -403,6 +490,7 @@ def main():
         "tb_sp_mean_um": p["res"].spacing.mean,
         "batched": batched,
         "serving": serving,
+        "smoothed_sdf": sdf,
     }
     print(json.dumps(line))
     return 0

@@ -53,11 +53,15 @@ struct SmoothSrc {
 
 enum SrcKind { kSrcField = 0, kSrcSdf = 1 };
 
-// src/sdf_curvature.rs:162-180: `(d2 as f64).sqrt() * h` and
-// `-(d2 as f64).sqrt() * h`; the square root is correctly rounded.
+// src/sdf_curvature.rs:162-180: `(d2 as f64).sqrt() * h` on bone voxels and
+// `-(d2 as f64).sqrt() * h` elsewhere, d2 the transform of the voxel's own
+// phase; the other phase's transform is 0 there, so the sum is that d2 and
+// no load depends on the bone byte.  The square root is correctly rounded,
+// and -(a * h) == (-a) * h exactly.
 __device__ __forceinline__ double sdf_value(uint8_t bone, uint32_t d2_bone, uint32_t d2_back,
                                             double h) {
-  return bone ? sqrt((double)d2_bone) * h : -sqrt((double)d2_back) * h;
+  const double s = sqrt((double)(d2_bone + d2_back)) * h;
+  return bone ? s : -s;
 }
 
 template <int kKind>
@@ -65,6 +69,27 @@ __device__ __forceinline__ double src_at(const SmoothSrc& s, uint64_t i) {
   if (kKind == kSrcField) return gbl(s.data)[i];
   return sdf_value(gbl(s.bone)[i], gbl(s.d2_bone)[i], gbl(s.d2_back)[i], s.h);
 }
+
+// A staged input held in registers between its loads and its use, so a lane
+// issues all of a batch's loads before it waits for the first.
+template <int kKind>
+struct Staged {
+  double v;
+  __device__ __forceinline__ void load(const SmoothSrc& s, uint64_t i) { v = gbl(s.data)[i]; }
+  __device__ __forceinline__ double value(const SmoothSrc&) const { return v; }
+};
+template <>
+struct Staged<kSrcSdf> {
+  uint32_t bone, d2_bone, d2_back;
+  __device__ __forceinline__ void load(const SmoothSrc& s, uint64_t i) {
+    bone = gbl(s.bone)[i];
+    d2_bone = gbl(s.d2_bone)[i];
+    d2_back = gbl(s.d2_back)[i];
+  }
+  __device__ __forceinline__ double value(const SmoothSrc& s) const {
+    return sdf_value((uint8_t)bone, d2_bone, d2_back, s.h);
+  }
+};
 
 __device__ __forceinline__ uint32_t clamp_line(int64_t v, uint32_t n) {
   return v < 0 ? 0u : (v >= (int64_t)n ? n - 1 : (uint32_t)v);
@@ -139,48 +164,53 @@ k_smooth_tiled(SmoothSrc src, double* __restrict__ out, const double* __restrict
   const uint32_t Pn = kAxis == 0 ? H : W;
   const uint32_t a0 = at * kSmTile, p0 = pt * kSmLanes;
   const uint32_t K = 2 * r + 1, L = kSmTile + 2 * r;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index is uniform; saying so keeps row tests scalar branches
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   BM_GLOBAL double* __restrict__ dst = gbl(out);
 
-  // stage positions a0 - r .. a0 + 63 + r (clamped) of the 64 lines, kStage
-  // loads in flight per lane before their LDS stores
+  // Stage positions a0 - r .. a0 + 63 + r (clamped) of the 64 lines: kStage
+  // loads per lane in flight, then their values into LDS.  Nothing here is
+  // conditional, so the compiler never waits for one batch before the next:
+  // lines beyond the volume load its last line (their outputs are never
+  // stored), and positions beyond the tile (j >= L) load a clamped element
+  // and go to the spare row L.
   constexpr uint32_t kStage = 8;
-  if (kAxis == 0) {  // lines are rows: contiguous positions across lanes
+  static_assert(kSmLanes % (kSmWaves * kStage) == 0, "rows per staging batch");
+  if (kAxis == 0) {  // lines are rows: a wave takes whole rows, positions across
+                     // lanes, so a row's base index is uniform and a lane only
+                     // adds its clamped position
     for (uint32_t jb = 0; jb < L; jb += 64) {
       const uint32_t j = jb + lane;
       const uint32_t a = clamp_line((int64_t)a0 - r + j, An);
+      const uint32_t row = j < L ? j : L;
       for (uint32_t pb = wave; pb < kSmLanes; pb += kSmWaves * kStage) {
-        double v[kStage];
+        Staged<kKind> v[kStage];
 #pragma unroll
         for (uint32_t u = 0; u < kStage; ++u) {
-          const uint32_t p = pb + u * kSmWaves;
-          if (j < L && p < kSmLanes && p0 + p < Pn)
-            v[u] = src_at<kKind>(src, smooth_index<0>(a, p0 + p, o, W, H));
+          const uint32_t pl = min(p0 + pb + u * kSmWaves, Pn - 1);
+          v[u].load(src, ((uint64_t)o * H + pl) * W + a);
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kStage; ++u) {
-          const uint32_t p = pb + u * kSmWaves;
-          if (j < L && p < kSmLanes && p0 + p < Pn) s_tile[j * kSmPitch + p] = v[u];
-        }
+        for (uint32_t u = 0; u < kStage; ++u)
+          s_tile[row * kSmPitch + pb + u * kSmWaves] = v[u].value(src);
       }
     }
-  } else {  // lines are x-columns: one line per lane
-    if (p0 + lane < Pn)
-      for (uint32_t jb = wave; jb < L; jb += kSmWaves * kStage) {
-        double v[kStage];
+  } else {  // lines are x-columns: one line per lane, positions uniform per wave
+    const uint32_t pl = min(p0 + lane, Pn - 1);
+    for (uint32_t jb = wave; jb < L; jb += kSmWaves * kStage) {
+      Staged<kKind> v[kStage];
 #pragma unroll
-        for (uint32_t u = 0; u < kStage; ++u) {
-          const uint32_t j = jb + u * kSmWaves;
-          if (j < L)
-            v[u] = src_at<kKind>(src, smooth_index<kAxis>(clamp_line((int64_t)a0 - r + j, An),
-                                                          p0 + lane, o, W, H));
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kStage; ++u) {
-          const uint32_t j = jb + u * kSmWaves;
-          if (j < L) s_tile[j * kSmPitch + lane] = v[u];
-        }
+      for (uint32_t u = 0; u < kStage; ++u) {
+        const uint32_t j = jb + u * kSmWaves;
+        v[u].load(src, smooth_index<kAxis>(clamp_line((int64_t)a0 - r + j, An), pl, o, W, H));
       }
+#pragma unroll
+      for (uint32_t u = 0; u < kStage; ++u) {
+        const uint32_t j = jb + u * kSmWaves;
+        s_tile[(j < L ? j : L) * kSmPitch + lane] = v[u].value(src);
+      }
+    }
   }
   __syncthreads();
 
@@ -331,8 +361,9 @@ std::vector<double> gaussian_kernel(double sigma) {
   return kernel;
 }
 
+// The staged tile (64 + 2r positions) and a spare row.
 static size_t tiled_lds_bytes(uint32_t r) {
-  return (size_t)(kSmTile + 2 * r) * kSmPitch * sizeof(double);
+  return (size_t)(kSmTile + 2 * r + 1) * kSmPitch * sizeof(double);
 }
 
 template <int kAxis, int kKind>
