@@ -8,6 +8,9 @@ interface for this path (bone-morphometry 0.7.0-rc.3):
   euclidean_distance_transform_squared  src/edt.rs:57-156
   local_thickness                     src/local_thickness.rs:16-44
   analyze_thickness_spacing           src/metrics.rs:64-132 (thickness/spacing part)
+  analyze                             src/metrics.rs:64-164 (every Metrics field)
+  cubical_measures, analyze_topology, bonej_connectivity,
+  bone_marrow_interface_faces         src/topology.rs:29-490
   FeatureMode                         src/metrics.rs:12-24
   require_isotropic_spacing           src/sdf_curvature.rs:147-160
   signed_distance_field_um            src/sdf_curvature.rs:162-180
@@ -29,7 +32,9 @@ Every call runs on the GPU through libbm.so; errors raise BmError carrying the
 reference's message.
 """
 from .api import (Batch, BinaryVolume, BmError, DistributionStats, FeatureMode,  # noqa: F401
-                  Phase, Server, ThicknessSpacing, analyze_thickness_spacing,
+                  Metrics, Phase, Server, ThicknessSpacing, analyze, analyze_device,
+                  analyze_thickness_spacing, analyze_topology, bone_marrow_interface_faces,
+                  bonej_connectivity, cubical_measures,
                   autotune, euclidean_distance_transform_squared, gaussian_kernel,
                   gaussian_smooth_nearest_in_place, gaussian_smooth_nearest_in_place_device,
                   local_thickness, local_thickness_stages, require_isotropic_spacing,

@@ -33,6 +33,8 @@ EXPORTS = (
     "bm_gaussian_smooth_nearest_in_place", "bm_smoothed_signed_distance_field_um",
     "bm_signed_distance_field_um_device", "bm_gaussian_smooth_nearest_in_place_device",
     "bm_smoothed_signed_distance_field_um_device",
+    "bm_cubical_measures", "bm_analyze_topology", "bm_bonej_connectivity",
+    "bm_bone_marrow_interface_faces", "bm_analyze", "bm_analyze_device",
 )
 
 
@@ -52,6 +54,34 @@ class BmThicknessSpacing(ctypes.Structure):
     _fields_ = [("thickness", BmDistributionStats), ("spacing", BmDistributionStats),
                 ("bone_volume", ctypes.c_double), ("total_volume", ctypes.c_double),
                 ("bone_volume_fraction", ctypes.c_double)]
+
+
+class BmCubical(ctypes.Structure):
+    _fields_ = [("raw_euler", ctypes.c_int64), ("mean_breadth_lattice_units", ctypes.c_double)]
+
+
+class BmTopology(ctypes.Structure):
+    _fields_ = [("raw_euler", ctypes.c_int64), ("beta0", ctypes.c_uint64),
+                ("beta1", ctypes.c_uint64), ("beta2", ctypes.c_uint64)]
+
+
+class BmBonej(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("edge_correction", "delta_chi", "connectivity")]
+
+
+class BmGeometry(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "bone_surface", "bone_surface_to_bone_volume", "bone_surface_to_total_volume",
+        "mean_breadth")]
+
+
+class BmMetrics(ctypes.Structure):
+    _fields_ = [("thickness", BmDistributionStats), ("spacing", BmDistributionStats),
+                ("bone_volume", ctypes.c_double), ("total_volume", ctypes.c_double),
+                ("bone_volume_fraction", ctypes.c_double), ("topology", BmTopology),
+                ("bonej_connectivity", BmBonej)] + [(n, ctypes.c_double) for n in (
+                    "bonej_connectivity_density", "connectivity", "connectivity_density",
+                    "roi_boundary_bone_fraction")] + [("geometry", BmGeometry)]
 
 
 class BmBatchItem(ctypes.Structure):
@@ -140,6 +170,13 @@ def lib():
         L.bm_signed_distance_field_um_device.argtypes = [P, U64, U64, U64, D, P, VP]
         L.bm_gaussian_smooth_nearest_in_place_device.argtypes = [P, U64, U64, U64, D, VP]
         L.bm_smoothed_signed_distance_field_um_device.argtypes = [P, U64, U64, U64, D, D, P, VP]
+        L.bm_cubical_measures.argtypes = [VOL, ctypes.POINTER(BmCubical)]
+        L.bm_analyze_topology.argtypes = [VOL, ctypes.c_int64, ctypes.POINTER(BmTopology)]
+        L.bm_bonej_connectivity.argtypes = [VOL, ctypes.c_int64, ctypes.POINTER(BmBonej)]
+        L.bm_bone_marrow_interface_faces.argtypes = [VOL, VOL, ctypes.POINTER(ctypes.c_int64)]
+        L.bm_analyze.argtypes = [VOL, VOL, ctypes.POINTER(D), I, ctypes.POINTER(BmMetrics)]
+        L.bm_analyze_device.argtypes = [P, P, U64, U64, U64, ctypes.POINTER(D), I,
+                                        ctypes.POINTER(BmMetrics), VP]
         L.bm_profile_enable.argtypes = [I]
         L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
         for name in EXPORTS:

@@ -233,6 +233,125 @@ def analyze_thickness_spacing_device(d_bone_ptr, d_roi_ptr, width, height, depth
 
 
 # ---------------------------------------------------------------------------
+# The rest of analyze (src/topology.rs, src/metrics.rs:134-202).
+
+def _struct(cls, c):
+    return cls(**{f.name: getattr(c, f.name) for f in dataclasses.fields(cls)})
+
+
+@dataclasses.dataclass
+class CubicalMeasures:
+    """src/topology.rs:13-17."""
+    raw_euler: int
+    mean_breadth_lattice_units: float
+
+
+@dataclasses.dataclass
+class Topology:
+    """src/topology.rs:5-11."""
+    raw_euler: int
+    beta0: int
+    beta1: int
+    beta2: int
+
+
+@dataclasses.dataclass
+class BonejConnectivity:
+    """src/topology.rs:19-27."""
+    edge_correction: float
+    delta_chi: float
+    connectivity: float
+
+
+@dataclasses.dataclass
+class GeometryMetrics:
+    """src/metrics.rs:40-46."""
+    bone_surface: float
+    bone_surface_to_bone_volume: float
+    bone_surface_to_total_volume: float
+    mean_breadth: float
+
+
+@dataclasses.dataclass
+class Metrics:
+    """src/metrics.rs:48-62."""
+    thickness: DistributionStats
+    spacing: DistributionStats
+    bone_volume: float
+    total_volume: float
+    bone_volume_fraction: float
+    topology: Topology
+    bonej_connectivity: BonejConnectivity
+    bonej_connectivity_density: float
+    connectivity: float
+    connectivity_density: float
+    roi_boundary_bone_fraction: float
+    geometry: GeometryMetrics
+
+    @classmethod
+    def _from_c(cls, m):
+        return cls(DistributionStats._from_c(m.thickness), DistributionStats._from_c(m.spacing),
+                   m.bone_volume, m.total_volume, m.bone_volume_fraction,
+                   _struct(Topology, m.topology), _struct(BonejConnectivity, m.bonej_connectivity),
+                   m.bonej_connectivity_density, m.connectivity, m.connectivity_density,
+                   m.roi_boundary_bone_fraction, _struct(GeometryMetrics, m.geometry))
+
+
+def cubical_measures(volume):
+    """src/topology.rs:53-171 on the volume's nonzero voxels."""
+    out = _native.BmCubical()
+    v = volume._c()
+    _check(_native.lib().bm_cubical_measures(ctypes.byref(v), ctypes.byref(out)))
+    return _struct(CubicalMeasures, out)
+
+
+def analyze_topology(volume, raw_euler):
+    """src/topology.rs:29-51: Betti numbers of the volume's nonzero voxels."""
+    out = _native.BmTopology()
+    v = volume._c()
+    _check(_native.lib().bm_analyze_topology(ctypes.byref(v), int(raw_euler), ctypes.byref(out)))
+    return _struct(Topology, out)
+
+
+def bonej_connectivity(volume, raw_euler):
+    """src/topology.rs:175-201: BoneJ's stack edge correction."""
+    out = _native.BmBonej()
+    v = volume._c()
+    _check(_native.lib().bm_bonej_connectivity(ctypes.byref(v), int(raw_euler),
+                                               ctypes.byref(out)))
+    return _struct(BonejConnectivity, out)
+
+
+def bone_marrow_interface_faces(bone, roi):
+    """src/topology.rs:442-490."""
+    out = ctypes.c_int64(0)
+    b, r = bone._c(), roi._c()
+    _check(_native.lib().bm_bone_marrow_interface_faces(ctypes.byref(b), ctypes.byref(r),
+                                                        ctypes.byref(out)))
+    return out.value
+
+
+def analyze(bone, roi, voxel_spacing, feature_mode=FeatureMode.Standard):
+    """analyze, src/metrics.rs:64-164: every field of Metrics."""
+    sp = (ctypes.c_double * 3)(*[float(v) for v in voxel_spacing])
+    out = _native.BmMetrics()
+    b, r = bone._c(), roi._c()
+    _check(_native.lib().bm_analyze(ctypes.byref(b), ctypes.byref(r), sp, int(feature_mode),
+                                    ctypes.byref(out)))
+    return Metrics._from_c(out)
+
+
+def analyze_device(d_bone_ptr, d_roi_ptr, width, height, depth, voxel_spacing,
+                   feature_mode=FeatureMode.Standard, stream=None):
+    """Device-resident analyze (device pointers, e.g. from hip.DeviceBuffer)."""
+    sp = (ctypes.c_double * 3)(*[float(v) for v in voxel_spacing])
+    out = _native.BmMetrics()
+    _check(_native.lib().bm_analyze_device(d_bone_ptr, d_roi_ptr, width, height, depth, sp,
+                                           int(feature_mode), ctypes.byref(out), stream))
+    return Metrics._from_c(out)
+
+
+# ---------------------------------------------------------------------------
 # Batched and serving forms (include/bm.h: bm_batch_*, bm_server_*).
 
 def _batch_items(items):

@@ -27,6 +27,7 @@
 #include "bm_engine.h"
 #include "bm_serve.h"
 #include "bm_smooth.h"
+#include "bm_topology.h"
 
 namespace bm {
 
@@ -368,6 +369,99 @@ void analyze_thickness_spacing(const BinaryVolume& bone, const BinaryVolume& roi
 }
 
 // ---------------------------------------------------------------------------
+// The rest of analyze (src/metrics.rs:134-164) from the device's integer counts.
+
+// cubical_measures' two outputs, src/topology.rs:167-170.
+static bm_cubical cubical_from(const TopologyCounts& t) {
+  bm_cubical c;
+  c.raw_euler = t.vertices - t.edges + t.faces - t.cubes;
+  c.mean_breadth_lattice_units = 0.5 * (double)(t.edges - 2 * t.faces + 3 * t.cubes);
+  return c;
+}
+
+// analyze_topology, src/topology.rs:29-51.
+static bm_topology topology_from(const TopologyCounts& t, int64_t raw_euler) {
+  const int64_t beta1 = (int64_t)t.beta0 + (int64_t)t.beta2 - raw_euler;
+  if (beta1 < 0)
+    throw Error(fmt("computed a negative beta1 (%lld); check the 26/6 topology convention",
+                    (long long)beta1));
+  bm_topology r;
+  r.raw_euler = raw_euler;
+  r.beta0 = t.beta0;
+  r.beta1 = (uint64_t)beta1;
+  r.beta2 = t.beta2;
+  return r;
+}
+
+// bonej_connectivity, src/topology.rs:181-201.
+static bm_bonej bonej_from(const TopologyCounts& t, int64_t raw_euler) {
+  const int64_t f = t.stack_vertices;
+  const int64_t e = t.stack_edges + 3 * f;
+  const int64_t c = t.stack_faces + 2 * e - 3 * f;
+  const int64_t d = t.edge_vertices + f;
+  const int64_t a = t.face_vertices;
+  const int64_t b = t.face_edges;
+  const double chi_zero = (double)f;
+  const double chi_one = (double)(d - e);
+  const double chi_two = (double)(a - b + c);
+  bm_bonej r;
+  r.edge_correction = chi_two / 2.0 + chi_one / 4.0 + chi_zero / 8.0;
+  r.delta_chi = (double)raw_euler - r.edge_correction;
+  r.connectivity = 1.0 - r.delta_chi;
+  return r;
+}
+
+// analyze, src/metrics.rs:64-164, on device-resident bone and ROI: the
+// thickness plan and the topology passes run back to back on one stream;
+// errors come in the reference's order (the thickness part's first).
+static void analyze_full_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_roi,
+                                const Dims& dims, const double spacing[3], int mode,
+                                bm_metrics* out, hipStream_t s) {
+  check_spacing(spacing);
+  check_mode(mode);
+  std::vector<ItemSpec> specs;
+  analyze_items(d_bone, d_roi, dims, spacing, mode, specs);
+  Plan& pl = e.plain_plan();
+  pl.build(specs, s);
+  pl.launch(s);
+  TopologyCounts t;
+  topology_device(e, d_bone, d_roi, dims, kTopoCubical | kTopoBonej | kTopoBetti, &t, s);
+  pl.fetch(s);
+  bm_thickness_spacing ts;
+  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, &ts);
+  out->thickness = ts.thickness;
+  out->spacing = ts.spacing;
+  out->bone_volume = ts.bone_volume;
+  out->total_volume = ts.total_volume;
+  out->bone_volume_fraction = ts.bone_volume_fraction;
+  const bm_cubical cubical = cubical_from(t);
+  out->topology = topology_from(t, cubical.raw_euler);
+  out->bonej_connectivity = bonej_from(t, cubical.raw_euler);
+  out->bonej_connectivity_density = out->bonej_connectivity.connectivity / out->total_volume;
+  out->connectivity = (double)out->topology.beta1;
+  out->connectivity_density = out->connectivity / out->total_volume;
+  const double bone_surface = (double)t.interface_faces * spacing[0] * spacing[1];
+  out->geometry.bone_surface = bone_surface;
+  out->geometry.bone_surface_to_bone_volume = bone_surface / out->bone_volume;
+  out->geometry.bone_surface_to_total_volume = bone_surface / out->total_volume;
+  out->geometry.mean_breadth = cubical.mean_breadth_lattice_units * spacing[0];
+  out->roi_boundary_bone_fraction = (double)t.boundary_bone_voxels / (double)t.bone_voxels;
+}
+
+// One volume's counts (the nonzero voxels, no ROI), uploaded from the host.
+static TopologyCounts volume_counts(const BinaryVolume& v, unsigned parts) {
+  const Dims dims = device_dims(v.width, v.height, v.depth);
+  Engine& e = engine_for_device(current_device());
+  std::lock_guard<std::mutex> g(e.mutex());
+  hipStream_t s = e.stream();
+  uint8_t* d = static_cast<uint8_t*>(e.scratch_bytes(dims.n(), 0));
+  BM_HIP(hipMemcpyAsync(d, v.data, dims.n(), hipMemcpyHostToDevice, s));
+  TopologyCounts t;
+  topology_device(e, d, nullptr, dims, parts, &t, s);
+  return t;
+}
+
+// ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
 template <class F>
@@ -580,6 +674,89 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
               d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[13],
               d[14], d[15], so.status.overflow_tiles);
     }
+  });
+}
+
+// ---- the rest of analyze -----------------------------------------------------
+int bm_cubical_measures(const bm_volume* volume, bm_cubical* out) {
+  return guarded([&] {
+    if (!out) throw Error("null output");
+    *out = cubical_from(volume_counts(from_c(volume), kTopoCubical));
+  });
+}
+
+int bm_analyze_topology(const bm_volume* volume, int64_t raw_euler, bm_topology* out) {
+  return guarded([&] {
+    if (!out) throw Error("null output");
+    *out = topology_from(volume_counts(from_c(volume), kTopoBetti), raw_euler);
+  });
+}
+
+int bm_bonej_connectivity(const bm_volume* volume, int64_t raw_euler, bm_bonej* out) {
+  return guarded([&] {
+    if (!out) throw Error("null output");
+    *out = bonej_from(volume_counts(from_c(volume), kTopoBonej), raw_euler);
+  });
+}
+
+int bm_bone_marrow_interface_faces(const bm_volume* bone, const bm_volume* roi, int64_t* faces) {
+  return guarded([&] {
+    if (!faces) throw Error("null output");
+    BinaryVolume b = from_c(bone);
+    BinaryVolume r = from_c(roi);
+    if (!b.same_shape(r))  // src/topology.rs:447-456
+      throw Error(fmt("bone dimensions %llux%llux%llu do not match ROI dimensions %llux%llux%llu",
+                      (unsigned long long)b.width, (unsigned long long)b.height,
+                      (unsigned long long)b.depth, (unsigned long long)r.width,
+                      (unsigned long long)r.height, (unsigned long long)r.depth));
+    const Dims dims = device_dims(b.width, b.height, b.depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    const uint64_t n = dims.n();
+    uint8_t* d = static_cast<uint8_t*>(e.scratch_bytes(2 * n + 256, 0));
+    uint8_t* d_roi = d + ((n + 255) / 256) * 256;
+    BM_HIP(hipMemcpyAsync(d, b.data, n, hipMemcpyHostToDevice, s));
+    BM_HIP(hipMemcpyAsync(d_roi, r.data, n, hipMemcpyHostToDevice, s));
+    // Only pairs with both voxels inside the ROI count, and there the bone
+    // clipped to the ROI (what the lattice pass sees) is the bone as given.
+    TopologyCounts t;
+    topology_device(e, d, d_roi, dims, kTopoCubical, &t, s);
+    *faces = t.interface_faces;
+  });
+}
+
+int bm_analyze(const bm_volume* bone, const bm_volume* roi, const double voxel_spacing[3],
+               int feature_mode, bm_metrics* out) {
+  return guarded([&] {
+    if (!out || !voxel_spacing) throw Error("null argument");
+    BinaryVolume b = from_c(bone);
+    BinaryVolume r = from_c(roi);
+    check_spacing(voxel_spacing);
+    if (!b.same_shape(r)) throw Error(shape_mismatch(b, r));
+    const Dims dims = device_dims(b.width, b.height, b.depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    hipStream_t s = e.stream();
+    const uint64_t n = dims.n();
+    uint8_t* d = static_cast<uint8_t*>(e.scratch_bytes(2 * n + 256, 0));
+    uint8_t* d_roi = d + ((n + 255) / 256) * 256;
+    BM_HIP(hipMemcpyAsync(d, b.data, n, hipMemcpyHostToDevice, s));
+    BM_HIP(hipMemcpyAsync(d_roi, r.data, n, hipMemcpyHostToDevice, s));
+    analyze_full_device(e, d, d_roi, dims, voxel_spacing, feature_mode, out, s);
+  });
+}
+
+int bm_analyze_device(const uint8_t* d_bone, const uint8_t* d_roi, uint64_t width,
+                      uint64_t height, uint64_t depth, const double voxel_spacing[3],
+                      int feature_mode, bm_metrics* out, void* stream) {
+  return guarded([&] {
+    if (!out || !voxel_spacing) throw Error("null argument");
+    const Dims dims = device_dims(width, height, depth);
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    analyze_full_device(e, d_bone, d_roi, dims, voxel_spacing, feature_mode, out,
+                        stream_or_default(e, stream));
   });
 }
 

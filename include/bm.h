@@ -138,6 +138,71 @@ int bm_local_thickness_stages(const bm_volume *volume, int phase, uint32_t *squa
                               uint8_t *ridge_flags, uint32_t *propagated, float *thickness,
                               uint64_t *ridge_count, uint64_t *fill_visits);
 
+/* ---- The rest of analyze: cubical measures, topology, BoneJ connectivity,
+ * interface surface and the ROI-boundary fraction (src/topology.rs,
+ * src/metrics.rs:134-202). */
+
+/* CubicalMeasures, src/topology.rs:13-17. */
+typedef struct bm_cubical {
+  int64_t raw_euler;
+  double mean_breadth_lattice_units;
+} bm_cubical;
+
+/* Topology, src/topology.rs:5-11. */
+typedef struct bm_topology {
+  int64_t raw_euler;
+  uint64_t beta0, beta1, beta2;
+} bm_topology;
+
+/* BonejConnectivity, src/topology.rs:19-27. */
+typedef struct bm_bonej {
+  double edge_correction, delta_chi, connectivity;
+} bm_bonej;
+
+/* GeometryMetrics, src/metrics.rs:40-46. */
+typedef struct bm_geometry {
+  double bone_surface, bone_surface_to_bone_volume, bone_surface_to_total_volume, mean_breadth;
+} bm_geometry;
+
+/* Metrics, src/metrics.rs:48-62 (field order kept). */
+typedef struct bm_metrics {
+  bm_distribution_stats thickness, spacing;
+  double bone_volume, total_volume, bone_volume_fraction;
+  bm_topology topology;
+  bm_bonej bonej_connectivity;
+  double bonej_connectivity_density, connectivity, connectivity_density,
+      roi_boundary_bone_fraction;
+  bm_geometry geometry;
+} bm_metrics;
+
+/* Replaces cubical_measures(&BinaryVolume) -> CubicalMeasures,
+ * src/topology.rs:53-171 (the volume's nonzero voxels are the complex). */
+int bm_cubical_measures(const bm_volume *volume, bm_cubical *out);
+
+/* Replaces analyze_topology(&BinaryVolume, raw_euler) -> Result<Topology>,
+ * src/topology.rs:29-51: beta0 (26-connected components), beta2 (enclosed
+ * 6-connected background components), beta1 = beta0 + beta2 - raw_euler, with
+ * the reference's error when that is negative. */
+int bm_analyze_topology(const bm_volume *volume, int64_t raw_euler, bm_topology *out);
+
+/* Replaces bonej_connectivity(&BinaryVolume, raw_euler) -> BonejConnectivity,
+ * src/topology.rs:175-201 (stack edge correction). */
+int bm_bonej_connectivity(const bm_volume *volume, int64_t raw_euler, bm_bonej *out);
+
+/* Replaces bone_marrow_interface_faces(&bone, &roi) -> Result<i64>,
+ * src/topology.rs:442-490. */
+int bm_bone_marrow_interface_faces(const bm_volume *bone, const bm_volume *roi, int64_t *faces);
+
+/* Replaces analyze(bone, roi, voxel_spacing, feature_mode) -> Result<Metrics>,
+ * src/metrics.rs:64-164, whole: thickness/spacing as
+ * bm_analyze_thickness_spacing, then the measures above on the analysis
+ * bone, in the reference's order of checks and errors. */
+int bm_analyze(const bm_volume *bone, const bm_volume *roi, const double voxel_spacing[3],
+               int feature_mode, bm_metrics *out);
+int bm_analyze_device(const uint8_t *d_bone, const uint8_t *d_roi, uint64_t width,
+                      uint64_t height, uint64_t depth, const double voxel_spacing[3],
+                      int feature_mode, bm_metrics *out, void *stream);
+
 /* ---- The signed distance field and its Gaussian smoothing, the EDT consumers
  * of the curvature and Minkowski measures (src/sdf_curvature.rs:75-105,
  * src/minkowski.rs:54-71).  Outputs are f64, z-major like the volumes. */

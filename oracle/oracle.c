@@ -707,3 +707,353 @@ int oracle_signed_distance_field_um(const uint8_t *bone, size_t w, size_t h, siz
   free(sq);
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * The rest of analyze (src/metrics.rs:134-164): cubical measures, topology,
+ * BoneJ connectivity, bone-marrow interface faces and the ROI-boundary bone
+ * fraction, all on the analysis bone (bone inside the ROI).
+ */
+static inline int fg_at(const uint8_t *v, long x, long y, long z, long w, long h, long d) {
+  if (x < 0 || y < 0 || z < 0 || x >= w || y >= h || z >= d) return 0;
+  return v[((size_t)z * (size_t)h + (size_t)y) * (size_t)w + (size_t)x] != 0;
+}
+
+typedef struct {
+  int64_t raw_euler;
+  double mean_breadth_lattice_units;
+  int64_t vertices, edges, faces, cubes; /* the cell counts behind them */
+} oracle_cubical;
+
+/* cubical_measures, src/topology.rs:53-171 (integer counts; the loops are
+ * spread over threads, which changes no count). */
+void oracle_cubical_measures(const uint8_t *v, size_t w_, size_t h_, size_t d_, oracle_cubical *out) {
+  long w = (long)w_, h = (long)h_, d = (long)d_;
+  int64_t cubes = 0, shared = 0, edges = 0, vertices = 0;
+#pragma omp parallel for reduction(+ : cubes, shared) schedule(static)
+  for (long z = 0; z < d; ++z)
+    for (long y = 0; y < h; ++y)
+      for (long x = 0; x < w; ++x) {
+        if (!fg_at(v, x, y, z, w, h, d)) continue;
+        cubes += 1;
+        shared += fg_at(v, x + 1, y, z, w, h, d) + fg_at(v, x, y + 1, z, w, h, d) +
+                  fg_at(v, x, y, z + 1, w, h, d);
+      }
+  int64_t faces = 6 * cubes - shared;
+#pragma omp parallel for reduction(+ : edges) schedule(static)
+  for (long zv = 0; zv <= d; ++zv) {
+    for (long yv = 0; yv <= h; ++yv) /* x-directed */
+      for (long x = 0; x < w; ++x)
+        edges += fg_at(v, x, yv - 1, zv - 1, w, h, d) || fg_at(v, x, yv, zv - 1, w, h, d) ||
+                 fg_at(v, x, yv - 1, zv, w, h, d) || fg_at(v, x, yv, zv, w, h, d);
+    for (long y = 0; y < h; ++y) /* y-directed */
+      for (long xv = 0; xv <= w; ++xv)
+        edges += fg_at(v, xv - 1, y, zv - 1, w, h, d) || fg_at(v, xv, y, zv - 1, w, h, d) ||
+                 fg_at(v, xv - 1, y, zv, w, h, d) || fg_at(v, xv, y, zv, w, h, d);
+  }
+#pragma omp parallel for reduction(+ : edges) schedule(static)
+  for (long z = 0; z < d; ++z) /* z-directed */
+    for (long yv = 0; yv <= h; ++yv)
+      for (long xv = 0; xv <= w; ++xv)
+        edges += fg_at(v, xv - 1, yv - 1, z, w, h, d) || fg_at(v, xv, yv - 1, z, w, h, d) ||
+                 fg_at(v, xv - 1, yv, z, w, h, d) || fg_at(v, xv, yv, z, w, h, d);
+#pragma omp parallel for reduction(+ : vertices) schedule(static)
+  for (long zv = 0; zv <= d; ++zv)
+    for (long yv = 0; yv <= h; ++yv)
+      for (long xv = 0; xv <= w; ++xv) {
+        int occ = 0;
+        for (long dz = -1; dz <= 0 && !occ; ++dz)
+          for (long dy = -1; dy <= 0 && !occ; ++dy)
+            for (long dx = -1; dx <= 0 && !occ; ++dx) occ = fg_at(v, xv + dx, yv + dy, zv + dz, w, h, d);
+        vertices += occ;
+      }
+  out->vertices = vertices;
+  out->edges = edges;
+  out->faces = faces;
+  out->cubes = cubes;
+  out->raw_euler = vertices - edges + faces - cubes;
+  out->mean_breadth_lattice_units = 0.5 * (double)(edges - 2 * faces + 3 * cubes);
+}
+
+/* bonej_connectivity and its stack counts, src/topology.rs:175-427. */
+typedef struct {
+  double edge_correction, delta_chi, connectivity;
+  int64_t f, e, c, d, a, b; /* stack_vertices, ... as combined at src/topology.rs:182-187 */
+} oracle_bonej;
+
+void oracle_bonej_connectivity(const uint8_t *v, size_t w_, size_t h_, size_t d_, int64_t raw_euler,
+                               oracle_bonej *out) {
+  long w = (long)w_, h = (long)h_, d = (long)d_;
+#define FG(x, y, z) fg_at(v, (x), (y), (z), w, h, d)
+  long zs[2] = {0, d - 1}, ys[2] = {0, h - 1}, xs[2] = {0, w - 1};
+  int64_t sv = 0, se = 0, sf = 0, fv = 0, fe = 0, ev = 0;
+  for (int i = 0; i < 2; ++i) /* stack_vertices */
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) sv += FG(xs[k], ys[j], zs[i]);
+  for (int i = 0; i < 2; ++i) /* stack_edges */
+    for (int j = 0; j < 2; ++j)
+      for (long x = 1; x < w - 1; ++x) se += FG(x, ys[j], zs[i]);
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 2; ++k)
+      for (long y = 1; y < h - 1; ++y) se += FG(xs[k], y, zs[i]);
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 2; ++k)
+      for (long z = 1; z < d - 1; ++z) se += FG(xs[k], ys[j], z);
+  for (int i = 0; i < 2; ++i) /* stack_faces */
+    for (long y = 1; y < h - 1; ++y)
+      for (long x = 1; x < w - 1; ++x) sf += FG(x, y, zs[i]);
+  for (int j = 0; j < 2; ++j)
+    for (long z = 1; z < d - 1; ++z)
+      for (long x = 1; x < w - 1; ++x) sf += FG(x, ys[j], z);
+  for (int k = 0; k < 2; ++k)
+    for (long y = 1; y < h - 1; ++y)
+      for (long z = 1; z < d - 1; ++z) sf += FG(xs[k], y, z);
+  for (int i = 0; i < 2; ++i) /* face_vertices */
+    for (long y = 0; y <= h; ++y)
+      for (long x = 0; x <= w; ++x) {
+        long z = zs[i];
+        fv += FG(x, y, z) || FG(x, y - 1, z) || FG(x - 1, y - 1, z) || FG(x - 1, y, z);
+      }
+  for (int k = 0; k < 2; ++k)
+    for (long y = 0; y <= h; ++y)
+      for (long z = 1; z < d; ++z) {
+        long x = xs[k];
+        fv += FG(x, y, z) || FG(x, y - 1, z) || FG(x, y - 1, z - 1) || FG(x, y, z - 1);
+      }
+  for (int j = 0; j < 2; ++j)
+    for (long x = 1; x < w; ++x)
+      for (long z = 1; z < d; ++z) {
+        long y = ys[j];
+        fv += FG(x, y, z) || FG(x, y, z - 1) || FG(x - 1, y, z - 1) || FG(x - 1, y, z);
+      }
+  for (int i = 0; i < 2; ++i) /* face_edges */
+    for (long y = 0; y <= h; ++y)
+      for (long x = 0; x <= w; ++x) {
+        long z = zs[i];
+        if (FG(x, y, z)) fe += 2;
+        else fe += FG(x, y - 1, z) + FG(x - 1, y, z);
+      }
+  for (int j = 0; j < 2; ++j) {
+    long y = ys[j];
+    for (long z = 1; z < d; ++z)
+      for (long x = 0; x < w; ++x) fe += FG(x, y, z) || FG(x, y, z - 1);
+    for (long z = 0; z < d; ++z)
+      for (long x = 0; x <= w; ++x) fe += FG(x, y, z) || FG(x - 1, y, z);
+  }
+  for (int k = 0; k < 2; ++k) {
+    long x = xs[k];
+    for (long z = 1; z < d; ++z)
+      for (long y = 0; y < h; ++y) fe += FG(x, y, z) || FG(x, y, z - 1);
+    for (long z = 0; z < d; ++z)
+      for (long y = 1; y < h; ++y) fe += FG(x, y, z) || FG(x, y - 1, z);
+  }
+  for (int i = 0; i < 2; ++i) /* edge_vertices */
+    for (int j = 0; j < 2; ++j)
+      for (long x = 1; x < w; ++x) ev += FG(x, ys[j], zs[i]) || FG(x - 1, ys[j], zs[i]);
+  for (int i = 0; i < 2; ++i)
+    for (int k = 0; k < 2; ++k)
+      for (long y = 1; y < h; ++y) ev += FG(xs[k], y, zs[i]) || FG(xs[k], y - 1, zs[i]);
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (long z = 1; z < d; ++z) ev += FG(xs[k], ys[j], z) || FG(xs[k], ys[j], z - 1);
+#undef FG
+  int64_t f = sv, e = se + 3 * f, c = sf + 2 * e - 3 * f, dd = ev + f, a = fv, b = fe;
+  out->f = f;
+  out->e = e;
+  out->c = c;
+  out->d = dd;
+  out->a = a;
+  out->b = b;
+  double chi_zero = (double)f, chi_one = (double)(dd - e), chi_two = (double)(a - b + c);
+  out->edge_correction = chi_two / 2.0 + chi_one / 4.0 + chi_zero / 8.0;
+  out->delta_chi = (double)raw_euler - out->edge_correction;
+  out->connectivity = 1.0 - out->delta_chi;
+}
+
+/* flood, src/topology.rs:614-701: depth-first with an explicit stack */
+static int flood_fill(const uint8_t *v, size_t w, size_t h, size_t d, size_t seed, int foreground,
+                      int conn26, uint8_t *visited, uint32_t **stack, size_t *cap) {
+  size_t top = 0, slice = w * h;
+  (*stack)[top++] = (uint32_t)seed;
+  visited[seed] = 1;
+  while (top) {
+    size_t index = (*stack)[--top];
+    long z = (long)(index / slice), rem = (long)(index % slice);
+    long y = rem / (long)w, x = rem % (long)w;
+    for (long dz = -1; dz <= 1; ++dz)
+      for (long dy = -1; dy <= 1; ++dy)
+        for (long dx = -1; dx <= 1; ++dx) {
+          int n6 = (dx != 0) + (dy != 0) + (dz != 0);
+          if (n6 == 0 || (!conn26 && n6 != 1)) continue;
+          long nx = x + dx, ny = y + dy, nz = z + dz;
+          if (nx < 0 || ny < 0 || nz < 0 || nx >= (long)w || ny >= (long)h || nz >= (long)d) continue;
+          size_t nb = ((size_t)nz * h + (size_t)ny) * w + (size_t)nx;
+          if (!visited[nb] && ((v[nb] != 0) == foreground)) {
+            visited[nb] = 1;
+            if (top == *cap) {
+              *cap *= 2;
+              uint32_t *s = (uint32_t *)realloc(*stack, *cap * sizeof(uint32_t));
+              if (!s) return 1;
+              *stack = s;
+            }
+            (*stack)[top++] = (uint32_t)nb;
+          }
+        }
+  }
+  return 0;
+}
+
+typedef struct {
+  int64_t raw_euler;
+  uint64_t beta0, beta1, beta2;
+} oracle_topology;
+
+/* analyze_topology, src/topology.rs:29-51 with count_components (26,
+ * foreground) and count_enclosed_background_components_6, :504-612. */
+int oracle_analyze_topology(const uint8_t *v, size_t w, size_t h, size_t d, int64_t raw_euler,
+                            oracle_topology *out) {
+  size_t n = w * h * d;
+  uint8_t *visited = (uint8_t *)calloc(n, 1);
+  size_t cap = 1 << 16;
+  uint32_t *stack = (uint32_t *)malloc(cap * sizeof(uint32_t));
+  if (!visited || !stack) return fail("oracle: out of host memory");
+  uint64_t beta0 = 0, beta2 = 0;
+  int rc = 0;
+  for (size_t seed = 0; seed < n && !rc; ++seed)
+    if (v[seed] != 0 && !visited[seed]) {
+      beta0 += 1;
+      rc = flood_fill(v, w, h, d, seed, 1, 1, visited, &stack, &cap);
+    }
+  memset(visited, 0, n);
+#define BOUNDARY(i)                                                              \
+  do {                                                                           \
+    size_t s_ = (i);                                                             \
+    if (!rc && v[s_] == 0 && !visited[s_])                                       \
+      rc = flood_fill(v, w, h, d, s_, 0, 0, visited, &stack, &cap);             \
+  } while (0)
+  for (size_t y = 0; y < h; ++y)
+    for (size_t x = 0; x < w; ++x) {
+      BOUNDARY((0 * h + y) * w + x);
+      BOUNDARY(((d - 1) * h + y) * w + x);
+    }
+  for (size_t z = 1; z + 1 < d; ++z)
+    for (size_t x = 0; x < w; ++x) {
+      BOUNDARY((z * h + 0) * w + x);
+      BOUNDARY((z * h + (h - 1)) * w + x);
+    }
+  for (size_t z = 1; z + 1 < d; ++z)
+    for (size_t y = 1; y + 1 < h; ++y) {
+      BOUNDARY((z * h + y) * w + 0);
+      BOUNDARY((z * h + y) * w + (w - 1));
+    }
+#undef BOUNDARY
+  for (size_t seed = 0; seed < n && !rc; ++seed)
+    if (v[seed] == 0 && !visited[seed]) {
+      beta2 += 1;
+      rc = flood_fill(v, w, h, d, seed, 0, 0, visited, &stack, &cap);
+    }
+  free(visited);
+  free(stack);
+  if (rc) return fail("oracle: out of host memory");
+  int64_t beta1 = (int64_t)beta0 + (int64_t)beta2 - raw_euler;
+  if (beta1 < 0) {
+    snprintf(g_err, sizeof g_err,
+             "computed a negative beta1 (%lld); check the 26/6 topology convention",
+             (long long)beta1);
+    return 1;
+  }
+  out->raw_euler = raw_euler;
+  out->beta0 = beta0;
+  out->beta1 = (uint64_t)beta1;
+  out->beta2 = beta2;
+  return 0;
+}
+
+/* bone_marrow_interface_faces, src/topology.rs:442-490 */
+int64_t oracle_interface_faces(const uint8_t *bone, const uint8_t *roi, size_t w_, size_t h_,
+                               size_t d_) {
+  long w = (long)w_, h = (long)h_, d = (long)d_;
+  int64_t faces = 0;
+#pragma omp parallel for reduction(+ : faces) schedule(static)
+  for (long z = 0; z < d; ++z)
+    for (long y = 0; y < h; ++y)
+      for (long x = 0; x < w; ++x) {
+        if (!fg_at(roi, x, y, z, w, h, d)) continue;
+        int c = fg_at(bone, x, y, z, w, h, d);
+        if (x + 1 < w && fg_at(roi, x + 1, y, z, w, h, d) && c != fg_at(bone, x + 1, y, z, w, h, d))
+          faces += 1;
+        if (y + 1 < h && fg_at(roi, x, y + 1, z, w, h, d) && c != fg_at(bone, x, y + 1, z, w, h, d))
+          faces += 1;
+        if (z + 1 < d && fg_at(roi, x, y, z + 1, w, h, d) && c != fg_at(bone, x, y, z + 1, w, h, d))
+          faces += 1;
+      }
+  return faces;
+}
+
+/* roi_boundary_bone_fraction, src/metrics.rs:166-202: the two counts */
+void oracle_roi_boundary_counts(const uint8_t *abone, const uint8_t *roi, size_t w_, size_t h_,
+                                size_t d_, uint64_t *bone_voxels, uint64_t *boundary_voxels) {
+  long w = (long)w_, h = (long)h_, d = (long)d_;
+  uint64_t b = 0, t = 0;
+  static const long off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma omp parallel for reduction(+ : b, t) schedule(static)
+  for (long z = 0; z < d; ++z)
+    for (long y = 0; y < h; ++y)
+      for (long x = 0; x < w; ++x) {
+        if (!fg_at(abone, x, y, z, w, h, d)) continue;
+        b += 1;
+        int touches = 0;
+        for (int k = 0; k < 6 && !touches; ++k)
+          touches = !fg_at(roi, x + off[k][0], y + off[k][1], z + off[k][2], w, h, d);
+        t += touches;
+      }
+  *bone_voxels = b;
+  *boundary_voxels = t;
+}
+
+/* The fields of Metrics beyond thickness/spacing, src/metrics.rs:123-164. */
+typedef struct {
+  double bone_volume, total_volume, bone_volume_fraction;
+  oracle_topology topology;
+  oracle_bonej bonej;
+  double bonej_connectivity_density, connectivity, connectivity_density,
+      roi_boundary_bone_fraction;
+  double bone_surface, bone_surface_to_bone_volume, bone_surface_to_total_volume, mean_breadth;
+  int64_t interface_faces;
+  oracle_cubical cubical;
+} oracle_rest;
+
+int oracle_analyze_rest(const uint8_t *bone, const uint8_t *roi, size_t w, size_t h, size_t d,
+                        const double spacing[3], oracle_rest *m) {
+  size_t n = w * h * d;
+  uint8_t *ab = (uint8_t *)malloc(n);
+  if (!ab) return fail("oracle: out of host memory");
+  oracle_phase_inside(bone, roi, n, 1, ab);
+  uint64_t bone_voxels = 0, roi_voxels = 0;
+  for (size_t i = 0; i < n; ++i) {
+    bone_voxels += ab[i] != 0;
+    roi_voxels += roi[i] != 0;
+  }
+  double vv = spacing[0] * spacing[1] * spacing[2];
+  m->bone_volume = (double)bone_voxels * vv;
+  m->total_volume = (double)roi_voxels * vv;
+  m->bone_volume_fraction = m->bone_volume / m->total_volume;
+  oracle_cubical_measures(ab, w, h, d, &m->cubical);
+  if (oracle_analyze_topology(ab, w, h, d, m->cubical.raw_euler, &m->topology)) {
+    free(ab);
+    return 1;
+  }
+  oracle_bonej_connectivity(ab, w, h, d, m->cubical.raw_euler, &m->bonej);
+  m->bonej_connectivity_density = m->bonej.connectivity / m->total_volume;
+  m->connectivity = (double)m->topology.beta1;
+  m->connectivity_density = m->connectivity / m->total_volume;
+  m->interface_faces = oracle_interface_faces(ab, roi, w, h, d);
+  m->bone_surface = (double)m->interface_faces * spacing[0] * spacing[1];
+  m->bone_surface_to_bone_volume = m->bone_surface / m->bone_volume;
+  m->bone_surface_to_total_volume = m->bone_surface / m->total_volume;
+  m->mean_breadth = m->cubical.mean_breadth_lattice_units * spacing[0];
+  uint64_t bb = 0, bt = 0;
+  oracle_roi_boundary_counts(ab, roi, w, h, d, &bb, &bt);
+  m->roi_boundary_bone_fraction = (double)bt / (double)bb;
+  free(ab);
+  return 0;
+}

@@ -181,3 +181,106 @@ def signed_distance_field_um(bone, w, h, d, hs):
     _chk(L.oracle_signed_distance_field_um(bone.ctypes.data, w, h, d, float(hs),
                                            out.ctypes.data))
     return out
+
+
+class Cubical(ctypes.Structure):
+    _fields_ = [("raw_euler", ctypes.c_int64), ("mean_breadth_lattice_units", D),
+                ("vertices", ctypes.c_int64), ("edges", ctypes.c_int64),
+                ("faces", ctypes.c_int64), ("cubes", ctypes.c_int64)]
+
+
+class Bonej(ctypes.Structure):
+    _fields_ = [("edge_correction", D), ("delta_chi", D), ("connectivity", D)] + [
+        (n, ctypes.c_int64) for n in ("f", "e", "c", "d", "a", "b")]
+
+
+class Topology(ctypes.Structure):
+    _fields_ = [("raw_euler", ctypes.c_int64), ("beta0", ctypes.c_uint64),
+                ("beta1", ctypes.c_uint64), ("beta2", ctypes.c_uint64)]
+
+
+class Rest(ctypes.Structure):
+    _fields_ = [("bone_volume", D), ("total_volume", D), ("bone_volume_fraction", D),
+                ("topology", Topology), ("bonej", Bonej),
+                ("bonej_connectivity_density", D), ("connectivity", D),
+                ("connectivity_density", D), ("roi_boundary_bone_fraction", D),
+                ("bone_surface", D), ("bone_surface_to_bone_volume", D),
+                ("bone_surface_to_total_volume", D), ("mean_breadth", D),
+                ("interface_faces", ctypes.c_int64), ("cubical", Cubical)]
+
+
+def _bind_topology():
+    L = lib()
+    if getattr(L, "_topology_bound", False):
+        return L
+    L.oracle_cubical_measures.argtypes = [P, Z, Z, Z, ctypes.POINTER(Cubical)]
+    L.oracle_cubical_measures.restype = None
+    L.oracle_bonej_connectivity.argtypes = [P, Z, Z, Z, ctypes.c_int64, ctypes.POINTER(Bonej)]
+    L.oracle_bonej_connectivity.restype = None
+    L.oracle_analyze_topology.argtypes = [P, Z, Z, Z, ctypes.c_int64, ctypes.POINTER(Topology)]
+    L.oracle_interface_faces.argtypes = [P, P, Z, Z, Z]
+    L.oracle_interface_faces.restype = ctypes.c_int64
+    L.oracle_roi_boundary_counts.argtypes = [P, P, Z, Z, Z, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64)]
+    L.oracle_roi_boundary_counts.restype = None
+    L.oracle_analyze_rest.argtypes = [P, P, Z, Z, Z, ctypes.POINTER(D), ctypes.POINTER(Rest)]
+    L._topology_bound = True
+    return L
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, np.uint8).reshape(-1)
+
+
+def cubical_measures(vol, w, h, d):
+    """cubical_measures, src/topology.rs:53-171."""
+    L = _bind_topology()
+    vol = _u8(vol)
+    out = Cubical()
+    L.oracle_cubical_measures(vol.ctypes.data, w, h, d, ctypes.byref(out))
+    return out
+
+
+def bonej_connectivity(vol, w, h, d, raw_euler):
+    """bonej_connectivity, src/topology.rs:175-201."""
+    L = _bind_topology()
+    vol = _u8(vol)
+    out = Bonej()
+    L.oracle_bonej_connectivity(vol.ctypes.data, w, h, d, int(raw_euler), ctypes.byref(out))
+    return out
+
+
+def analyze_topology(vol, w, h, d, raw_euler):
+    """analyze_topology, src/topology.rs:29-51."""
+    L = _bind_topology()
+    vol = _u8(vol)
+    out = Topology()
+    _chk(L.oracle_analyze_topology(vol.ctypes.data, w, h, d, int(raw_euler), ctypes.byref(out)))
+    return out
+
+
+def interface_faces(bone, roi, w, h, d):
+    """bone_marrow_interface_faces, src/topology.rs:442-490."""
+    L = _bind_topology()
+    bone, roi = _u8(bone), _u8(roi)
+    return L.oracle_interface_faces(bone.ctypes.data, roi.ctypes.data, w, h, d)
+
+
+def roi_boundary_counts(abone, roi, w, h, d):
+    """The two counts of roi_boundary_bone_fraction, src/metrics.rs:166-202."""
+    L = _bind_topology()
+    abone, roi = _u8(abone), _u8(roi)
+    b, t = ctypes.c_uint64(), ctypes.c_uint64()
+    L.oracle_roi_boundary_counts(abone.ctypes.data, roi.ctypes.data, w, h, d, ctypes.byref(b),
+                                 ctypes.byref(t))
+    return b.value, t.value
+
+
+def analyze_rest(bone, roi, w, h, d, spacing):
+    """The Metrics fields of analyze beyond thickness/spacing, src/metrics.rs:123-164."""
+    L = _bind_topology()
+    bone, roi = _u8(bone), _u8(roi)
+    sp = (D * 3)(*[float(s) for s in spacing])
+    out = Rest()
+    _chk(L.oracle_analyze_rest(bone.ctypes.data, roi.ctypes.data, w, h, d, sp, ctypes.byref(out)))
+    return out
