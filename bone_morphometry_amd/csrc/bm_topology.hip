@@ -2,7 +2,11 @@
 // spacing, on the analysis bone A = (bone != 0) inside the ROI
 // (bone-morphometry 0.7.0-rc.3, src/topology.rs, src/metrics.rs:134-202).
 //
-// Lattice pass: one wavefront per row (y, z) of the vertex lattice
+// Everything works on bit rows: a packing pass turns the analysis bone and the
+// ROI into one 64-bit word per 64 voxels of a row (bits past the row's end are
+// 0), so the measures below handle 64 voxels with a few word operations.
+//
+// Lattice pass: one thread per (row (y, z), word) of the vertex lattice
 // [0..W] x [0..H] x [0..D].  Lattice point (x, y, z) sees the octant of
 // voxels (x-1..x, y-1..y, z-1..z); it is a vertex of the complex when any of
 // the eight is bone, and the lattice edges leaving it along +x, +y, +z exist
@@ -10,8 +14,9 @@
 // The voxel (x, y, z) itself adds its cube, the faces it shares with its -x,
 // -y, -z neighbours, its interface faces with them (both inside the ROI,
 // one bone; src/topology.rs:442-490) and, when bone, whether a 6-neighbour
-// lies outside the ROI or the image (src/metrics.rs:166-202).  The -x values
-// come from the neighbouring lane.
+// lies outside the ROI or the image (src/metrics.rs:166-202).  The x-1 and
+// x+1 columns are the words shifted by one bit, with the neighbouring word's
+// end bit carried in; each count is a population count.
 //
 // BoneJ's edge correction (src/topology.rs:175-427) sums small predicates
 // over the six faces, twelve edges and eight corners of the image box; the
@@ -19,13 +24,14 @@
 //
 // Betti numbers (src/topology.rs:29-51,504-701): beta0 counts 26-connected
 // components of A, beta2 the 6-connected components of its complement that
-// do not reach the image boundary.  Both come from a union-find over runs:
-// every voxel of a run along x is labelled with the run's first voxel, then
-// runs are united where they first touch a run of a previous row (one union
-// per contact instead of one per voxel pair), background runs on the image
-// boundary are united with an extra node N standing for the exterior, and
-// the components are the roots.  Roots are linked smaller index first, so
-// the count is independent of the order in which unions land.
+// do not reach the image boundary.  Both come from a union-find whose nodes
+// are runs along x, named by their first voxel: a run's start is found from
+// the bit row (the bit after the last 0 before it).  A word's runs are united
+// with the runs of earlier rows they touch, one union per contact (the bits
+// where a contact begins), background runs on the image boundary with an
+// extra node N standing for the exterior, and the components are the roots.
+// Roots are linked under the smaller index, so the count does not depend on
+// the order in which unions land.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,13 +64,6 @@ __device__ __forceinline__ uint32_t abone(const VolumeView& v, int64_t x, int64_
   return gbl(v.bone)[i] != 0 && (!v.roi || gbl(v.roi)[i] != 0);
 }
 
-// Inside the ROI (0 outside the image).
-__device__ __forceinline__ uint32_t inroi(const VolumeView& v, int64_t x, int64_t y, int64_t z) {
-  if (x < 0 || y < 0 || z < 0 || x >= v.W || y >= v.H || z >= v.D) return 0;
-  if (!v.roi) return 1;
-  return gbl(v.roi)[((uint64_t)z * v.H + (uint64_t)y) * v.W + (uint64_t)x] != 0;
-}
-
 __device__ __forceinline__ void add_counter(unsigned long long* c, uint32_t k, uint64_t v) {
   if (v) atomicAdd(c + k, (unsigned long long)v);
 }
@@ -74,70 +73,110 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
   return v;
 }
 
+// Adds kN per-thread counts of a 256-thread block into counters[first..]:
+// one atomic per counter and block (all blocks hit the same few addresses, so
+// per-wave atomics would serialise).
+template <int kN>
+__device__ __forceinline__ void block_add_counters(const uint64_t (&v)[kN],
+                                                   unsigned long long* counters, uint32_t first) {
+  __shared__ uint64_t s[4][kN];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kN; ++k) {
+    const uint64_t w = wave_sum64(v[k]);
+    if (lane == 0) s[wave][k] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)kN)
+    add_counter(counters, first + threadIdx.x,
+                s[0][threadIdx.x] + s[1][threadIdx.x] + s[2][threadIdx.x] + s[3][threadIdx.x]);
+}
+
 __global__ void __launch_bounds__(64) k_topo_zero(unsigned long long* c) {
   if (threadIdx.x < kCounters) c[threadIdx.x] = 0;
 }
 
-// The lattice pass (see the file comment); one wavefront per lattice row.
+// Bit rows of the analysis bone (A) and of the ROI (R); words per row WW.
+struct BitRows {
+  const unsigned long long* A;
+  const unsigned long long* R;
+  uint32_t W, H, D, WW;
+};
+
+// Word k of row (y, z), 0 outside the image.
+__device__ __forceinline__ unsigned long long row_word(const unsigned long long* bits,
+                                                       const BitRows& b, int64_t y, int64_t z,
+                                                       int64_t k) {
+  if (y < 0 || z < 0 || y >= b.H || z >= b.D || k < 0 || k >= b.WW) return 0;
+  return gbl(bits)[((uint64_t)z * b.H + (uint64_t)y) * b.WW + (uint64_t)k];
+}
+
+// Packing: one wavefront per row; a ballot per 64 voxels.
 __global__ void __launch_bounds__(256)
-k_topo_lattice(VolumeView v, unsigned long long* __restrict__ counters) {
+k_topo_pack(VolumeView v, uint32_t WW, unsigned long long* __restrict__ A,
+            unsigned long long* __restrict__ R) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t rows = (uint64_t)(v.H + 1) * (v.D + 1);
+  const uint64_t rows = (uint64_t)v.H * v.D;
   const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  uint64_t vert = 0, edges = 0, cubes = 0, shared = 0, iface = 0, bone = 0, boundary = 0;
   for (uint64_t row = wave0; row < rows; row += waves) {
-    const int64_t y = (int64_t)(row % (v.H + 1)), z = (int64_t)(row / (v.H + 1));
-    // octant rows (y-1|y, z-1|z): A and R at x and (from the lane below) x-1
-    uint32_t cA[4] = {0, 0, 0, 0}, cR[4] = {0, 0, 0, 0};  // lane 63 of the previous chunk
-    for (int64_t xb = 0; xb <= (int64_t)v.W; xb += 64) {
-      const int64_t x = xb + lane;
-      uint32_t A[4], R[4], Ap[4], Rp[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {  // k bit 0: +y, bit 1: +z
-        const int64_t yy = y - 1 + (k & 1), zz = z - 1 + (k >> 1);
-        A[k] = abone(v, x, yy, zz);
-        R[k] = inroi(v, x, yy, zz);
-        const uint32_t a_up = __shfl_up(A[k], 1), r_up = __shfl_up(R[k], 1);
-        Ap[k] = lane ? a_up : cA[k];
-        Rp[k] = lane ? r_up : cR[k];
-        cA[k] = __shfl(A[k], 63);
-        cR[k] = __shfl(R[k], 63);
+    const uint64_t base = row * v.W;
+    for (uint32_t k = 0; k < WW; ++k) {
+      const uint32_t x = k * 64 + lane;
+      uint32_t a = 0, r = 0;
+      if (x < v.W) {
+        r = !v.roi || gbl(v.roi)[base + x] != 0;
+        a = r && gbl(v.bone)[base + x] != 0;
       }
-      // octant bits: a{dx}{dy}{dz} with d = 1 for x, y, z and 0 for x-1, y-1, z-1
-      const uint32_t a000 = Ap[0], a100 = A[0], a010 = Ap[1], a110 = A[1];
-      const uint32_t a001 = Ap[2], a101 = A[2], a011 = Ap[3], a111 = A[3];
-      vert += a000 | a100 | a010 | a110 | a001 | a101 | a011 | a111;
-      edges += (a100 | a110 | a101 | a111) + (a010 | a110 | a011 | a111) +
-               (a001 | a101 | a011 | a111);
-      cubes += a111;
-      shared += (a111 & a011) + (a111 & a101) + (a111 & a110);
-      // voxel c = (x, y, z): R[3] / A[3]; its -x, -y, -z neighbours
-      if (R[3]) iface += (Rp[3] & (a111 ^ a011)) + (R[2] & (a111 ^ a101)) + (R[1] & (a111 ^ a110));
-      if (a111) {
-        bone += 1;
-        const uint32_t inside = Rp[3] & R[2] & R[1] & inroi(v, x + 1, y, z) &
-                                inroi(v, x, y + 1, z) & inroi(v, x, y, z + 1);
-        boundary += inside ? 0 : 1;
+      const unsigned long long wa = __ballot(a), wr = __ballot(r);
+      if (lane == 0) {
+        gbl(A)[row * WW + k] = wa;
+        gbl(R)[row * WW + k] = wr;
       }
     }
   }
-  vert = wave_sum64(vert);
-  edges = wave_sum64(edges);
-  cubes = wave_sum64(cubes);
-  shared = wave_sum64(shared);
-  iface = wave_sum64(iface);
-  bone = wave_sum64(bone);
-  boundary = wave_sum64(boundary);
-  if (lane == 0) {
-    add_counter(counters, kCntVertices, vert);
-    add_counter(counters, kCntEdges, edges);
-    add_counter(counters, kCntCubes, cubes);
-    add_counter(counters, kCntShared, shared);
-    add_counter(counters, kCntInterface, iface);
-    add_counter(counters, kCntBone, bone);
-    add_counter(counters, kCntBoundary, boundary);
+}
+
+// The lattice pass (see the file comment): one thread per lattice row and word.
+__global__ void __launch_bounds__(256)
+k_topo_lattice(BitRows b, unsigned long long* __restrict__ counters) {
+  const uint32_t WL = b.W / 64 + 1;  // words covering lattice points 0..W
+  const uint64_t total = (uint64_t)(b.H + 1) * (b.D + 1) * WL;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t vert = 0, edges = 0, cubes = 0, shared = 0, iface = 0, bone = 0, boundary = 0;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int64_t k = (int64_t)(g % WL);
+    const uint64_t row = g / WL;
+    const int64_t y = (int64_t)(row % (b.H + 1)), z = (int64_t)(row / (b.H + 1));
+    unsigned long long A[4], Ap[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // q bit 0: +y, bit 1: +z
+      const int64_t yy = y - 1 + (q & 1), zz = z - 1 + (q >> 1);
+      A[q] = row_word(b.A, b, yy, zz, k);
+      Ap[q] = (A[q] << 1) | (row_word(b.A, b, yy, zz, k - 1) >> 63);
+    }
+    vert += __popcll(A[0] | A[1] | A[2] | A[3] | Ap[0] | Ap[1] | Ap[2] | Ap[3]);
+    edges += __popcll(A[0] | A[1] | A[2] | A[3]) + __popcll(Ap[1] | A[1] | Ap[3] | A[3]) +
+             __popcll(Ap[2] | A[2] | Ap[3] | A[3]);
+    const unsigned long long c = A[3];  // the voxels (x, y, z)
+    cubes += __popcll(c);
+    shared += __popcll(c & Ap[3]) + __popcll(c & A[2]) + __popcll(c & A[1]);
+    const unsigned long long r0 = row_word(b.R, b, y, z, k);
+    if (r0 | c) {
+      const unsigned long long rm = (r0 << 1) | (row_word(b.R, b, y, z, k - 1) >> 63);
+      const unsigned long long rp = (r0 >> 1) | (row_word(b.R, b, y, z, k + 1) << 63);
+      const unsigned long long ry = row_word(b.R, b, y - 1, z, k), rz = row_word(b.R, b, y, z - 1, k);
+      iface += __popcll(r0 & rm & (c ^ Ap[3])) + __popcll(r0 & ry & (c ^ A[2])) +
+               __popcll(r0 & rz & (c ^ A[1]));
+      bone += __popcll(c);
+      const unsigned long long inside =
+          rm & rp & ry & rz & row_word(b.R, b, y + 1, z, k) & row_word(b.R, b, y, z + 1, k);
+      boundary += __popcll(c & ~inside);
+    }
   }
+  // kCntVertices .. kCntBoundary are consecutive
+  const uint64_t v[7] = {vert, edges, cubes, shared, iface, bone, boundary};
+  block_add_counters<7>(v, counters, kCntVertices);
 }
 
 // ---------------------------------------------------------------------------
@@ -235,106 +274,141 @@ __device__ void uf_unite(uint32_t* L, uint32_t a, uint32_t b) {
   }
 }
 
-// Voxel of the phase being labelled (A == fg).
-__device__ __forceinline__ bool in_phase(const VolumeView& v, bool fg, int64_t x, int64_t y,
-                                         int64_t z) {
-  if (x < 0 || y < 0 || z < 0 || x >= v.W || y >= v.H || z >= v.D) return false;
-  return (abone(v, x, y, z) != 0) == fg;
+// Word k of row (y, z) of the phase being labelled: A, or its complement
+// inside the image.  0 outside the image.
+__device__ __forceinline__ unsigned long long phase_word(const BitRows& b, bool fg, int64_t y,
+                                                         int64_t z, int64_t k) {
+  if (y < 0 || z < 0 || y >= b.H || z >= b.D || k < 0 || k >= b.WW) return 0;
+  const unsigned long long a = gbl(b.A)[((uint64_t)z * b.H + (uint64_t)y) * b.WW + (uint64_t)k];
+  if (fg) return a;
+  const uint32_t rest = b.W - (uint32_t)k * 64;  // valid bits in this word
+  const unsigned long long valid = rest >= 64 ? ~0ull : ((1ull << rest) - 1);
+  return ~a & valid;
 }
 
-// Run labels: each voxel of the phase gets the index of the first voxel of its
-// run along x.  One wavefront per row; the exterior node is initialised too.
+// First x of the run containing x in row (y, z) (x must be in the phase).
+__device__ __forceinline__ uint32_t run_start(const BitRows& b, bool fg, int64_t y, int64_t z,
+                                              uint32_t x) {
+  int64_t k = x >> 6;
+  const uint32_t bit = x & 63;
+  unsigned long long gaps = ~phase_word(b, fg, y, z, k) & (~0ull >> (63 - bit));
+  while (!gaps) {
+    if (--k < 0) return 0;
+    gaps = ~phase_word(b, fg, y, z, k);
+  }
+  return (uint32_t)(k * 64 + 64 - __clzll((long long)gaps));
+}
+
+// Run starts in word k of a row: phase bits whose left neighbour is not.
+__device__ __forceinline__ unsigned long long starts_of(const BitRows& b, bool fg, int64_t y,
+                                                        int64_t z, int64_t k,
+                                                        unsigned long long p) {
+  return p & ~((p << 1) | (phase_word(b, fg, y, z, k - 1) >> 63));
+}
+
+// Labels of the run starts (each its own node) and of the exterior node.
 __global__ void __launch_bounds__(256)
-k_ccl_runs(VolumeView v, bool fg, uint32_t* __restrict__ labels) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t rows = (uint64_t)v.H * v.D, n = rows * v.W;
+k_ccl_init(BitRows b, bool fg, uint32_t* __restrict__ labels) {
+  const uint64_t rows = (uint64_t)b.H * b.D, total = rows * b.WW, n = rows * b.W;
   if (blockIdx.x == 0 && threadIdx.x == 0) labels[n] = (uint32_t)n;
-  const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t row = wave0; row < rows; row += waves) {
-    const int64_t y = (int64_t)(row % v.H), z = (int64_t)(row / v.H);
-    const uint64_t base = row * v.W;
-    bool open = false;    // a run continues into this chunk
-    uint32_t carried = 0;  // its first x
-    for (uint32_t xb = 0; xb < v.W; xb += 64) {
-      const uint32_t x = xb + lane;
-      const bool t = in_phase(v, fg, x, y, z);
-      const uint64_t m = __ballot(t);
-      const uint64_t starts = m & ~((m << 1) | (open ? 1ull : 0ull));
-      const uint64_t le = starts & (~0ull >> (63 - lane));
-      const uint32_t first = le ? xb + 63 - (uint32_t)__clzll((long long)le) : carried;
-      if (t) labels[base + x] = (uint32_t)(base + first);
-      if ((m >> 63) & 1) {
-        if (starts) carried = xb + 63 - (uint32_t)__clzll((long long)starts);
-        open = true;
-      } else {
-        open = false;
-      }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int64_t k = (int64_t)(g % b.WW);
+    const uint64_t row = g / b.WW;
+    const int64_t y = (int64_t)(row % b.H), z = (int64_t)(row / b.H);
+    unsigned long long s = starts_of(b, fg, y, z, k, phase_word(b, fg, y, z, k));
+    while (s) {
+      const uint32_t x = (uint32_t)(k * 64 + __ffsll((long long)s) - 1);
+      s &= s - 1;
+      labels[row * b.W + x] = (uint32_t)(row * b.W + x);
     }
   }
 }
 
-// Unions where runs first touch a run of an earlier row (and, for the
-// background, the exterior).  26-connectivity looks at rows (y-1, z) and
-// (y-1..y+1, z-1) at x-1..x+1; 6-connectivity at rows (y-1, z), (y, z-1) at x.
+// Unions where runs of word k of row (y, z) first touch runs of earlier rows
+// (and, for the background, the exterior).  26-connectivity looks at rows
+// (y-1, z) and (y-1..y+1, z-1) at x-1..x+1; 6-connectivity at rows (y-1, z),
+// (y, z-1) at x.  A run start unites with the neighbour row's runs at x-1, at
+// x (when x-1 did not already cover it) and at x+1; a later voxel of the run
+// only with a run beginning at x+1 (26) or x (6).
 template <bool kConn26>
 __global__ void __launch_bounds__(256)
-k_ccl_unite(VolumeView v, bool fg, uint32_t* __restrict__ labels) {
-  const uint64_t n = (uint64_t)v.W * v.H * v.D;
+k_ccl_unite(BitRows b, bool fg, uint32_t* __restrict__ labels) {
+  const uint64_t rows = (uint64_t)b.H * b.D, total = rows * b.WW, n = rows * b.W;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t x = (int64_t)(i % v.W);
-    const uint64_t yz = i / v.W;
-    const int64_t y = (int64_t)(yz % v.H), z = (int64_t)(yz / v.H);
-    if (!in_phase(v, fg, x, y, z)) continue;
-    const bool start = !in_phase(v, fg, x - 1, y, z);
-    auto idx = [&](int64_t a, int64_t b, int64_t c) {
-      return (uint32_t)(((uint64_t)c * v.H + (uint64_t)b) * v.W + (uint64_t)a);
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int64_t k = (int64_t)(g % b.WW);
+    const uint64_t row = g / b.WW;
+    const int64_t y = (int64_t)(row % b.H), z = (int64_t)(row / b.H);
+    const unsigned long long p = phase_word(b, fg, y, z, k);
+    if (!p) continue;
+    const unsigned long long s = starts_of(b, fg, y, z, k, p);
+    auto node = [&](int64_t yy, int64_t zz, uint32_t x) {
+      return (uint32_t)(((uint64_t)zz * b.H + (uint64_t)yy) * b.W + run_start(b, fg, yy, zz, x));
     };
-    if (kConn26) {
-      const int64_t ry[4] = {y - 1, y - 1, y, y + 1}, rz[4] = {z, z - 1, z - 1, z - 1};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool lo = in_phase(v, fg, x - 1, ry[r], rz[r]);
-        const bool mid = in_phase(v, fg, x, ry[r], rz[r]);
-        const bool hi = in_phase(v, fg, x + 1, ry[r], rz[r]);
-        if (start) {
-          if (lo) uf_unite(labels, (uint32_t)i, idx(x - 1, ry[r], rz[r]));
-          if (mid && !lo) uf_unite(labels, (uint32_t)i, idx(x, ry[r], rz[r]));
-          if (hi && !mid) uf_unite(labels, (uint32_t)i, idx(x + 1, ry[r], rz[r]));
-        } else if (hi && !mid) {
-          uf_unite(labels, (uint32_t)i, idx(x + 1, ry[r], rz[r]));
-        }
+    auto unite_bits = [&](unsigned long long u, int64_t yy, int64_t zz, int dx) {
+      while (u) {
+        const uint32_t x = (uint32_t)(k * 64 + __ffsll((long long)u) - 1);
+        u &= u - 1;
+        uf_unite(labels, node(y, z, x), node(yy, zz, (uint32_t)((int64_t)x + dx)));
       }
-    } else {
-      const int64_t ry[2] = {y - 1, y}, rz[2] = {z, z - 1};
+    };
+    constexpr int kRows = kConn26 ? 4 : 2;
+    const int64_t ry[4] = {y - 1, y - 1, y, y + 1}, rz[4] = {z, z - 1, z - 1, z - 1};
+    const int64_t sy[2] = {y - 1, y}, sz[2] = {z, z - 1};
 #pragma unroll
-      for (int r = 0; r < 2; ++r)
-        if (in_phase(v, fg, x, ry[r], rz[r]) && (start || !in_phase(v, fg, x - 1, ry[r], rz[r])))
-          uf_unite(labels, (uint32_t)i, idx(x, ry[r], rz[r]));
-      const bool edge = x == 0 || x == (int64_t)v.W - 1 || y == 0 || y == (int64_t)v.H - 1 ||
-                        z == 0 || z == (int64_t)v.D - 1;
-      if (!fg && edge && (start || x == (int64_t)v.W - 1)) uf_unite(labels, (uint32_t)i, (uint32_t)n);
+    for (int r = 0; r < kRows; ++r) {
+      const int64_t yy = kConn26 ? ry[r] : sy[r], zz = kConn26 ? rz[r] : sz[r];
+      const unsigned long long q = phase_word(b, fg, yy, zz, k);
+      const unsigned long long qlo = (q << 1) | (phase_word(b, fg, yy, zz, k - 1) >> 63);
+      if (kConn26) {
+        const unsigned long long qhi = (q >> 1) | (phase_word(b, fg, yy, zz, k + 1) << 63);
+        unite_bits(s & qlo, yy, zz, -1);
+        unite_bits(s & q & ~qlo, yy, zz, 0);
+        unite_bits(p & qhi & ~q, yy, zz, 1);
+      } else {
+        unite_bits(p & q & (s | ~qlo), yy, zz, 0);
+      }
+    }
+    if (!kConn26 && !fg) {  // background runs on the image boundary reach the exterior
+      unsigned long long e;
+      if (y == 0 || y == (int64_t)b.H - 1 || z == 0 || z == (int64_t)b.D - 1) {
+        e = s;
+      } else {
+        const uint32_t last = b.W - 1;
+        e = (k == 0 ? (p & 1ull) : 0ull) |
+            ((uint32_t)k == last / 64 ? (p & (1ull << (last & 63))) : 0ull);
+      }
+      while (e) {
+        const uint32_t x = (uint32_t)(k * 64 + __ffsll((long long)e) - 1);
+        e &= e - 1;
+        uf_unite(labels, node(y, z, x), (uint32_t)n);
+      }
     }
   }
 }
 
-// Components = roots among the phase's voxels; for the background also
-// whether the exterior node was reached.
+// Components = run starts that are roots; for the background also whether
+// the exterior node was reached.
 __global__ void __launch_bounds__(256)
-k_ccl_count(VolumeView v, bool fg, uint32_t* __restrict__ labels, uint32_t counter,
+k_ccl_count(BitRows b, bool fg, uint32_t* __restrict__ labels, uint32_t counter,
             unsigned long long* __restrict__ counters) {
-  const uint64_t n = (uint64_t)v.W * v.H * v.D;
+  const uint64_t rows = (uint64_t)b.H * b.D, total = rows * b.WW, n = rows * b.W;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t roots = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t x = (int64_t)(i % v.W);
-    const uint64_t yz = i / v.W;
-    const int64_t y = (int64_t)(yz % v.H), z = (int64_t)(yz / v.H);
-    if (in_phase(v, fg, x, y, z) && labels[i] == (uint32_t)i) ++roots;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const int64_t k = (int64_t)(g % b.WW);
+    const uint64_t row = g / b.WW;
+    const int64_t y = (int64_t)(row % b.H), z = (int64_t)(row / b.H);
+    unsigned long long s = starts_of(b, fg, y, z, k, phase_word(b, fg, y, z, k));
+    while (s) {
+      const uint64_t i = row * b.W + (uint64_t)(k * 64 + __ffsll((long long)s) - 1);
+      s &= s - 1;
+      roots += labels[i] == (uint32_t)i;
+    }
   }
-  roots = wave_sum64(roots);
-  if ((threadIdx.x & 63) == 0) add_counter(counters, counter, roots);
+  const uint64_t v[1] = {roots};
+  block_add_counters<1>(v, counters, counter);
   if (!fg && blockIdx.x == 0 && threadIdx.x == 0 && labels[n] != (uint32_t)n)
     add_counter(counters, kCntExterior, 1);
 }
@@ -408,32 +482,43 @@ void topology_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_roi, con
   const uint64_t n = dims.n();
   if (n >= 0xFFFFFFFFull) throw std::range_error("topology needs fewer than 2^32 - 1 voxels");
   VolumeView v{d_bone, d_roi, dims.w, dims.h, dims.d};
+  const uint32_t WW = (dims.w + 63) / 64;
+  const uint64_t rows = (uint64_t)dims.h * dims.d;
+  const uint64_t words = rows * WW;
+  // slot 2: labels (n + 1), then the A and R bit rows
+  const size_t label_bytes = ((n + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+  char* base = static_cast<char*>(e.scratch_bytes(label_bytes + 2 * words * 8, 2));
+  uint32_t* labels = reinterpret_cast<uint32_t*>(base);
+  unsigned long long* bitsA = reinterpret_cast<unsigned long long*>(base + label_bytes);
+  unsigned long long* bitsR = bitsA + words;
   unsigned long long* d_cnt =
       static_cast<unsigned long long*>(e.scratch_bytes(kCounters * sizeof(unsigned long long), 3));
   BM_LAUNCH("topo_zero", k_topo_zero, 1, 64, 0, s, d_cnt);
+  const uint32_t pblocks = (uint32_t)std::min<uint64_t>((rows + 3) / 4, 16384);
+  BM_LAUNCH("topo_pack", k_topo_pack, pblocks, 256, 0, s, v, WW, bitsA, bitsR);
+  const BitRows b{bitsA, bitsR, dims.w, dims.h, dims.d, WW};
   if (parts & kTopoCubical) {
-    const uint64_t rows = (uint64_t)(dims.h + 1) * (dims.d + 1);
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>((rows + 3) / 4, 8192);
-    BM_LAUNCH("topo_lattice", k_topo_lattice, blocks, 256, 0, s, v, d_cnt);
+    const uint64_t total = (uint64_t)(dims.h + 1) * (dims.d + 1) * (dims.w / 64 + 1);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 2048);
+    BM_LAUNCH("topo_lattice", k_topo_lattice, blocks, 256, 0, s, b, d_cnt);
   }
   if (parts & kTopoBonej) {
     uint64_t total = 0;
     const StackTable t = stack_table(dims, &total);
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 2048));
+    const uint32_t blocks =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 2048));
     BM_LAUNCH("topo_stack", k_topo_stack, blocks, 256, 0, s, v, t, total, d_cnt);
   }
   if (parts & kTopoBetti) {
-    uint32_t* labels = static_cast<uint32_t*>(e.scratch_bytes((n + 1) * sizeof(uint32_t), 2));
-    const uint64_t rows = (uint64_t)dims.h * dims.d;
-    const uint32_t rblocks = (uint32_t)std::min<uint64_t>((rows + 3) / 4, 8192);
-    const uint32_t vblocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
-    BM_LAUNCH("ccl_runs", k_ccl_runs, rblocks, 256, 0, s, v, true, labels);
-    BM_LAUNCH("ccl_unite", k_ccl_unite<true>, vblocks, 256, 0, s, v, true, labels);
-    BM_LAUNCH("ccl_count", k_ccl_count, vblocks, 256, 0, s, v, true, labels, (uint32_t)kCntBeta0,
+    const uint32_t wblocks = (uint32_t)std::min<uint64_t>((words + 255) / 256, 16384);
+    BM_LAUNCH("ccl_init", k_ccl_init, wblocks, 256, 0, s, b, true, labels);
+    const uint32_t cblocks = std::min<uint32_t>(wblocks, 2048);
+    BM_LAUNCH("ccl_unite26", k_ccl_unite<true>, wblocks, 256, 0, s, b, true, labels);
+    BM_LAUNCH("ccl_count", k_ccl_count, cblocks, 256, 0, s, b, true, labels, (uint32_t)kCntBeta0,
               d_cnt);
-    BM_LAUNCH("ccl_runs", k_ccl_runs, rblocks, 256, 0, s, v, false, labels);
-    BM_LAUNCH("ccl_unite", k_ccl_unite<false>, vblocks, 256, 0, s, v, false, labels);
-    BM_LAUNCH("ccl_count", k_ccl_count, vblocks, 256, 0, s, v, false, labels, (uint32_t)kCntBeta2,
+    BM_LAUNCH("ccl_init", k_ccl_init, wblocks, 256, 0, s, b, false, labels);
+    BM_LAUNCH("ccl_unite6", k_ccl_unite<false>, wblocks, 256, 0, s, b, false, labels);
+    BM_LAUNCH("ccl_count", k_ccl_count, cblocks, 256, 0, s, b, false, labels, (uint32_t)kCntBeta2,
               d_cnt);
   }
   BM_HIP(hipGetLastError());
