@@ -17,6 +17,8 @@ Beside it:
   analyze  the whole analyze (every Metrics field, topology and surface included)
            on the plain workload: time per call, the topology passes' share,
            exact agreement of the integer measures with the oracle.
+  tiff     the input step: the same bone volume as a 512-page TIFF (LZW and
+           uncompressed) read into HBM on all host threads and on one.
   sdf      the signed distance field of the same bone volume smoothed with the
            30 um Gaussian (signed_distance_field_um + gaussian_smooth_nearest_in_place),
            per-pass HBM fractions, the oracle's time and bit-exact parity.
@@ -61,8 +63,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain_512", choices=["plain_512", "plain_small", "large"])
     ap.add_argument("--mode", default="standard", choices=["standard", "classic"])
-    ap.add_argument("--forms", default="plain,batched,serving,sdf,analyze",
-                    help="comma list of plain, batched, serving, sdf, analyze")
+    ap.add_argument("--forms", default="plain,batched,serving,sdf,analyze,tiff",
+                    help="comma list of plain, batched, serving, sdf, analyze, tiff")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-host-call", action="store_true",
                     help="skip timing the host-pointer entry point")
@@ -359,6 +361,41 @@ def analyze_form(args, bm, native, hip, wl, cpu_threads, plain_cpu):
             "cpu_baseline": cpu, "topology_exact_vs_oracle": parity}
 
 
+def tiff_form(args, bm, hip, wl, cpu_threads):
+    """The input step: the plain workload's bone volume as a 512-page 8-bit
+    TIFF (LZW and uncompressed, in the page cache) read into HBM by
+    bm_read_binary_tiff_device, on all host threads and on one (the
+    reference reads pages one after another)."""
+    import tempfile
+    import numpy as np
+    from PIL import Image
+    out = {}
+    d_out = hip.DeviceBuffer(wl.n)
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        pages = [Image.fromarray(p) for p in wl.bone.reshape(wl.depth, wl.height, wl.width)]
+        for comp in ("tiff_lzw", "raw"):
+            path = os.path.join(tmp, comp + ".tif")
+            pages[0].save(path, save_all=True, append_images=pages[1:], compression=comp)
+            size = os.path.getsize(path)
+            row = {"file_mb": size / 1e6}
+            for label, threads in (("all_threads", cpu_threads), ("one_thread", 1)):
+                bm.read_binary_tiff_device(path, d_out.value, wl.n, strict_binary=True,
+                                           threads=threads)
+                reps = 3 if threads > 1 else 1
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    bm.read_binary_tiff_device(path, d_out.value, wl.n, strict_binary=True,
+                                               threads=threads)
+                s = (time.perf_counter() - t0) / reps
+                row[label] = {"ms": s * 1e3, "gvox_per_s": wl.n / s / 1e9, "threads": threads}
+            got = np.empty(wl.n, np.uint8)
+            d_out.download(got)
+            row["exact"] = bool(np.array_equal(got, (wl.bone > 0).astype(np.uint8)))
+            out[comp] = row
+            log(f"tiff {comp}: {row['all_threads']['ms']:.1f} ms")
+    return out
+
+
 def batched_form(args, bm, hip, workloads, cpu_threads):
     t0 = time.time()
     vols = workloads.batched(threads=cpu_threads)
@@ -537,6 +574,7 @@ def main():
     sdf = sdf_form(args, bm, _native, hip, wl, cpu_threads) if "sdf" in forms else None
     full = (analyze_form(args, bm, _native, hip, wl, cpu_threads, p["cpu"])
             if "analyze" in forms else None)
+    tiff = tiff_form(args, bm, hip, wl, cpu_threads) if "tiff" in forms else None
     line = {
         "metric": "Gvoxel/s EDT+local thickness at 512^3",
         "value": p["gvox"],
@@ -568,6 +606,7 @@ def main():
         "serving": serving,
         "smoothed_sdf": sdf,
         "full_analyze": full,
+        "tiff_input": tiff,
     }
     print(json.dumps(line))
     return 0

@@ -12,6 +12,8 @@ interface for this path (bone-morphometry 0.7.0-rc.3):
   cubical_measures, analyze_topology, bonej_connectivity,
   bone_marrow_interface_faces         src/topology.rs:29-490
   FeatureMode                         src/metrics.rs:12-24
+  read_binary_tiff, read_roi_tiff,
+  read_isotropic_spacing_um           src/tiff_io.rs:18-317 (the input step)
   require_isotropic_spacing           src/sdf_curvature.rs:147-160
   signed_distance_field_um            src/sdf_curvature.rs:162-180
   gaussian_kernel                     src/sdf_curvature.rs:182-198
@@ -37,7 +39,9 @@ from .api import (Batch, BinaryVolume, BmError, DistributionStats, FeatureMode, 
                   bonej_connectivity, cubical_measures,
                   autotune, euclidean_distance_transform_squared, gaussian_kernel,
                   gaussian_smooth_nearest_in_place, gaussian_smooth_nearest_in_place_device,
-                  local_thickness, local_thickness_stages, require_isotropic_spacing,
+                  local_thickness, local_thickness_stages, read_binary_tiff,
+                  read_binary_tiff_device, read_isotropic_spacing_um, read_roi_tiff,
+                  read_roi_tiff_device, require_isotropic_spacing,
                   signed_distance_field_um, signed_distance_field_um_device,
                   smoothed_signed_distance_field_um, smoothed_signed_distance_field_um_device,
                   tuning_load)

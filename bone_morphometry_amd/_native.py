@@ -35,6 +35,8 @@ EXPORTS = (
     "bm_smoothed_signed_distance_field_um_device",
     "bm_cubical_measures", "bm_analyze_topology", "bm_bonej_connectivity",
     "bm_bone_marrow_interface_faces", "bm_analyze", "bm_analyze_device",
+    "bm_read_binary_tiff", "bm_read_roi_tiff", "bm_read_binary_tiff_device",
+    "bm_read_roi_tiff_device", "bm_read_isotropic_spacing_um",
 )
 
 
@@ -177,6 +179,13 @@ def lib():
         L.bm_analyze.argtypes = [VOL, VOL, ctypes.POINTER(D), I, ctypes.POINTER(BmMetrics)]
         L.bm_analyze_device.argtypes = [P, P, U64, U64, U64, ctypes.POINTER(D), I,
                                         ctypes.POINTER(BmMetrics), VP]
+        CS, U8 = ctypes.c_char_p, ctypes.c_uint8
+        DIMS = ctypes.POINTER(U64)
+        L.bm_read_binary_tiff.argtypes = [CS, U8, I, I, P, U64, DIMS]
+        L.bm_read_roi_tiff.argtypes = [CS, I, P, U64, DIMS]
+        L.bm_read_binary_tiff_device.argtypes = [CS, U8, I, I, P, U64, DIMS, VP]
+        L.bm_read_roi_tiff_device.argtypes = [CS, I, P, U64, DIMS, VP]
+        L.bm_read_isotropic_spacing_um.argtypes = [CS, ctypes.POINTER(D)]
         L.bm_profile_enable.argtypes = [I]
         L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
         for name in EXPORTS:

@@ -1,5 +1,6 @@
 """Reference-shaped calls over libbm.so (see the package docstring)."""
 import ctypes
+import os
 import dataclasses
 import enum
 
@@ -349,6 +350,58 @@ def analyze_device(d_bone_ptr, d_roi_ptr, width, height, depth, voxel_spacing,
     _check(_native.lib().bm_analyze_device(d_bone_ptr, d_roi_ptr, width, height, depth, sp,
                                            int(feature_mode), ctypes.byref(out), stream))
     return Metrics._from_c(out)
+
+
+# ---------------------------------------------------------------------------
+# The input step (src/tiff_io.rs).
+
+def _path(path):
+    return os.fsencode(path)
+
+
+def _read_tiff(call, path, *args):
+    dims = (ctypes.c_uint64 * 3)()
+    _check(call(_path(path), *args, None, 0, dims))
+    w, h, d = dims
+    out = np.empty(w * h * d, np.uint8)
+    _check(call(_path(path), *args, out.ctypes.data, out.size, dims))
+    return BinaryVolume(out, w, h, d)
+
+
+def read_binary_tiff(path, threshold=0, strict_binary=False, threads=0):
+    """src/tiff_io.rs:18-26: a multi-page 8-bit TIFF as 0/1 voxels (value > threshold)."""
+    return _read_tiff(_native.lib().bm_read_binary_tiff, path, int(threshold), int(strict_binary),
+                      int(threads))
+
+
+def read_roi_tiff(path, threads=0):
+    """src/tiff_io.rs:28-31: a 0/1 or 0/255 ROI mask."""
+    return _read_tiff(_native.lib().bm_read_roi_tiff, path, int(threads))
+
+
+def read_binary_tiff_device(path, d_out_ptr, capacity, threshold=0, strict_binary=False,
+                            threads=0, stream=None):
+    """Device form: the volume decoded into d_out_ptr; returns (width, height, depth)."""
+    dims = (ctypes.c_uint64 * 3)()
+    _check(_native.lib().bm_read_binary_tiff_device(_path(path), int(threshold),
+                                                    int(strict_binary), int(threads), d_out_ptr,
+                                                    capacity, dims, stream))
+    return tuple(dims)
+
+
+def read_roi_tiff_device(path, d_out_ptr, capacity, threads=0, stream=None):
+    """Device form of read_roi_tiff; returns (width, height, depth)."""
+    dims = (ctypes.c_uint64 * 3)()
+    _check(_native.lib().bm_read_roi_tiff_device(_path(path), int(threads), d_out_ptr, capacity,
+                                                 dims, stream))
+    return tuple(dims)
+
+
+def read_isotropic_spacing_um(path):
+    """src/tiff_io.rs:149-317: [x, y, z] voxel spacing in micrometres."""
+    out = (ctypes.c_double * 3)()
+    _check(_native.lib().bm_read_isotropic_spacing_um(_path(path), out))
+    return [out[0], out[1], out[2]]
 
 
 # ---------------------------------------------------------------------------

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["bm_kernels.hip", "bm_tiles.hip", "bm_engine.hip", "bm_quantiles.hip", "bm_smooth.hip",
                "bm_topology.hip",
-               "bm_tune.cpp", "bm_serve.cpp", "bm_api.cpp"]
+               "bm_tune.cpp", "bm_serve.cpp", "bm_tiff.cpp", "bm_api.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -72,7 +72,7 @@ def build_libbm(jobs=8):
         raise RuntimeError("hipcc failed on: " + ", ".join(failed))
     out = os.path.join(HERE, "libbm.so")
     tmp = out + ".tmp"
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, CSRC)
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"], CSRC)
     os.replace(tmp, out)
     return out
 

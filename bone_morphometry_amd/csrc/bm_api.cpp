@@ -27,6 +27,7 @@
 #include "bm_engine.h"
 #include "bm_serve.h"
 #include "bm_smooth.h"
+#include "bm_tiff.h"
 #include "bm_topology.h"
 
 namespace bm {
@@ -674,6 +675,65 @@ int bm_local_thickness_stages(const bm_volume* volume, int phase, uint32_t* squa
               d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12], d[13],
               d[14], d[15], so.status.overflow_tiles);
     }
+  });
+}
+
+// ---- the input step: TIFF volumes ---------------------------------------------
+static void tiff_dims(const TiffVolumeInfo& info, uint64_t dims[3]) {
+  dims[0] = info.width;
+  dims[1] = info.height;
+  dims[2] = info.depth;
+}
+
+int bm_read_binary_tiff(const char* path, uint8_t threshold, int strict_binary, int threads,
+                        uint8_t* out, uint64_t capacity, uint64_t dims[3]) {
+  return guarded([&] {
+    if (!path || !dims) throw Error("null argument");
+    tiff_dims(read_binary_tiff(path, threshold,
+                               strict_binary ? TiffSamples::ZeroOr255 : TiffSamples::Any, out,
+                               capacity, threads),
+              dims);
+  });
+}
+
+int bm_read_roi_tiff(const char* path, int threads, uint8_t* out, uint64_t capacity,
+                     uint64_t dims[3]) {
+  return guarded([&] {
+    if (!path || !dims) throw Error("null argument");
+    tiff_dims(read_binary_tiff(path, 0, TiffSamples::RoiMask, out, capacity, threads), dims);
+  });
+}
+
+int bm_read_binary_tiff_device(const char* path, uint8_t threshold, int strict_binary,
+                               int threads, uint8_t* d_out, uint64_t capacity, uint64_t dims[3],
+                               void* stream) {
+  return guarded([&] {
+    if (!path || !dims || !d_out) throw Error("null argument");
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    tiff_dims(read_binary_tiff_device(e, path, threshold,
+                                      strict_binary ? TiffSamples::ZeroOr255 : TiffSamples::Any,
+                                      d_out, capacity, threads, stream_or_default(e, stream)),
+              dims);
+  });
+}
+
+int bm_read_roi_tiff_device(const char* path, int threads, uint8_t* d_out, uint64_t capacity,
+                            uint64_t dims[3], void* stream) {
+  return guarded([&] {
+    if (!path || !dims || !d_out) throw Error("null argument");
+    Engine& e = engine_for_device(current_device());
+    std::lock_guard<std::mutex> g(e.mutex());
+    tiff_dims(read_binary_tiff_device(e, path, 0, TiffSamples::RoiMask, d_out, capacity, threads,
+                                      stream_or_default(e, stream)),
+              dims);
+  });
+}
+
+int bm_read_isotropic_spacing_um(const char* path, double spacing[3]) {
+  return guarded([&] {
+    if (!path || !spacing) throw Error("null argument");
+    read_isotropic_spacing_um(path, spacing);
   });
 }
 

@@ -245,6 +245,38 @@ int bm_smoothed_signed_distance_field_um_device(const uint8_t *d_bone, uint64_t 
                                                 uint64_t height, uint64_t depth, double h,
                                                 double sigma_vox, double *d_out, void *stream);
 
+/* ---- The input step: multi-page 8-bit TIFF volumes and their calibration
+ * (src/tiff_io.rs).  path is a NUL-terminated file name; messages name it as
+ * given.  Pages are decoded on `threads` host threads (0: every core). */
+
+/* Replaces read_binary_tiff(path, threshold, strict_binary) -> Result<BinaryVolume>,
+ * src/tiff_io.rs:18-26,33-147: dims receives width, height, depth.  With out
+ * NULL only the page directory is read (to size the buffer); otherwise every
+ * page is decoded, its samples checked (0 or 255 when strict_binary) and
+ * written as value > threshold, 0/1, into out (capacity bytes). */
+int bm_read_binary_tiff(const char *path, uint8_t threshold, int strict_binary, int threads,
+                        uint8_t *out, uint64_t capacity, uint64_t dims[3]);
+
+/* Replaces read_roi_tiff(path) -> Result<BinaryVolume>, src/tiff_io.rs:28-31:
+ * samples 0/1 or 0/255, written as value > 0. */
+int bm_read_roi_tiff(const char *path, int threads, uint8_t *out, uint64_t capacity,
+                     uint64_t dims[3]);
+
+/* Device forms: decoded pages stream through pinned staging buffers into
+ * d_out (capacity bytes of device memory) on stream while later pages decode;
+ * they return when the volume is resident. */
+int bm_read_binary_tiff_device(const char *path, uint8_t threshold, int strict_binary,
+                               int threads, uint8_t *d_out, uint64_t capacity, uint64_t dims[3],
+                               void *stream);
+int bm_read_roi_tiff_device(const char *path, int threads, uint8_t *d_out, uint64_t capacity,
+                            uint64_t dims[3], void *stream);
+
+/* Replaces read_isotropic_spacing_um(path) -> Result<[f64; 3]>,
+ * src/tiff_io.rs:149-317: x/y from XResolution, YResolution and
+ * ResolutionUnit (or the ImageJ unit=), z from the ImageJ spacing=, with the
+ * reference's isotropy checks and messages. */
+int bm_read_isotropic_spacing_um(const char *path, double spacing[3]);
+
 /* ---- Batched form (no counterpart in the reference: many analyze() calls,
  * src/metrics.rs:64-132, in one launch sequence).  The volumes are
  * device-resident, of any shapes; each is analyzed exactly as by

@@ -1,0 +1,240 @@
+"""The input step (src/tiff_io.rs) through the C ABI, on the CPU: multi-page
+8-bit TIFF volumes and their calibration.  The tiff crate the reference uses
+is not vendored, so decoded pages are checked against what the files hold:
+files written by PIL (libtiff) and read back by PIL, and files written here
+(tests/_tiffwrite.py) from known arrays.  The messages are the reference's."""
+import os
+
+import numpy as np
+import pytest
+from PIL import Image, ImageSequence
+
+import _tiffwrite as tw
+import bone_morphometry_amd as bm
+
+
+def volume(seed, d=5, h=33, w=70, values=(0, 255)):
+    rng = np.random.default_rng(seed)
+    return rng.choice(np.array(values, np.uint8), size=(d, h, w))
+
+
+def pil_write(path, vol, **kw):
+    pages = [Image.fromarray(p) for p in vol]
+    pages[0].save(path, save_all=True, append_images=pages[1:], **kw)
+
+
+def pil_read(path):
+    with Image.open(path) as im:
+        return np.stack([np.array(p) for p in ImageSequence.Iterator(im)])
+
+
+@pytest.mark.parametrize("compression", ["raw", "tiff_lzw", "tiff_adobe_deflate", "packbits",
+                                         "tiff_deflate"])
+def test_pil_files_match_pil_reader(tmp_path, compression):
+    vol = volume(1, values=(0, 1, 7, 128, 200, 255))
+    path = str(tmp_path / f"v_{compression}.tif")
+    pil_write(path, vol, compression=compression)
+    ref = pil_read(path)
+    assert np.array_equal(ref, vol)
+    for threshold in (0, 100, 254):
+        v = bm.read_binary_tiff(path, threshold=threshold)
+        assert (v.width, v.height, v.depth) == (70, 33, 5)
+        assert np.array_equal(v.data, (ref > threshold).astype(np.uint8).reshape(-1))
+
+
+def test_lzw_large_pages_match_pil_reader(tmp_path):
+    # long enough for 12-bit codes and table resets
+    rng = np.random.default_rng(3)
+    vol = (rng.random((4, 300, 257)) < 0.3).astype(np.uint8) * 255
+    vol[1] = 255
+    vol[2, :, :100] = 0
+    path = str(tmp_path / "lzw.tif")
+    pil_write(path, vol, compression="tiff_lzw")
+    v = bm.read_binary_tiff(path, strict_binary=True, threads=3)
+    assert np.array_equal(v.data, (pil_read(path) > 0).astype(np.uint8).reshape(-1))
+
+
+def test_bigtiff_from_pil(tmp_path):
+    vol = volume(2)
+    path = str(tmp_path / "big.tif")
+    try:
+        pil_write(path, vol, big_tiff=True)
+    except TypeError:
+        pytest.skip("this PIL cannot write BigTIFF")
+    with open(path, "rb") as fh:
+        assert fh.read(4)[2:] == b"+\x00"  # version 43
+    v = bm.read_binary_tiff(path)
+    assert np.array_equal(v.data, (vol > 0).astype(np.uint8).reshape(-1))
+
+
+@pytest.mark.parametrize("kw", [
+    dict(order=">"),
+    dict(big=True),
+    dict(big=True, order=">", compression=8),
+    dict(compression=8, predictor=2),
+    dict(compression=32946, predictor=2, order=">"),
+    dict(compression=32773, rows_per_strip=7),
+    dict(rows_per_strip=1),
+    dict(tile=(32, 16)),
+    dict(tile=(16, 16), compression=8, predictor=2),
+    dict(tile=(64, 48), compression=32773, big=True),
+])
+def test_written_files_decode_exactly(tmp_path, kw):
+    vol = volume(4, values=(0, 3, 255))
+    path = str(tmp_path / "w.tif")
+    tw.write(path, list(vol), **kw)
+    v = bm.read_binary_tiff(path, threshold=2)
+    assert (v.width, v.height, v.depth) == (70, 33, 5)
+    assert np.array_equal(v.data, (vol > 2).astype(np.uint8).reshape(-1))
+
+
+def test_roi_encodings(tmp_path):
+    for values in ((0, 1), (0, 255), (0, 1, 255)):
+        vol = volume(5, values=values)
+        path = str(tmp_path / f"roi{len(values)}.tif")
+        pil_write(path, vol)
+        v = bm.read_roi_tiff(path)
+        assert np.array_equal(v.data, (vol > 0).astype(np.uint8).reshape(-1))
+
+
+def test_threads_do_not_change_the_result(tmp_path):
+    vol = volume(6, d=17, values=(0, 90, 255))
+    path = str(tmp_path / "t.tif")
+    pil_write(path, vol, compression="tiff_adobe_deflate")
+    a = bm.read_binary_tiff(path, threshold=50, threads=1).data
+    for t in (2, 5, 0):
+        assert np.array_equal(bm.read_binary_tiff(path, threshold=50, threads=t).data, a)
+
+
+# --- the reference's messages (src/tiff_io.rs:33-147) ------------------------------------
+def err(fn, *a, **k):
+    with pytest.raises(bm.BmError) as e:
+        fn(*a, **k)
+    return str(e.value)
+
+
+def test_errors(tmp_path):
+    missing = str(tmp_path / "missing.tif")
+    assert err(bm.read_binary_tiff, missing) == f"opening {missing}"
+    junk = str(tmp_path / "junk.tif")
+    open(junk, "wb").write(b"not a tiff at all")
+    assert err(bm.read_binary_tiff, junk) == f"initializing TIFF decoder for {junk}"
+
+    rgb = str(tmp_path / "rgb.tif")
+    Image.fromarray(np.zeros((4, 5, 3), np.uint8)).save(rgb)
+    assert err(bm.read_binary_tiff, rgb) == f"{rgb} must contain 8-bit grayscale pages; found RGB(8)"
+    g16 = str(tmp_path / "g16.tif")
+    tw.write(g16, [np.zeros((4, 5), np.uint8)], bits=16)
+    assert err(bm.read_binary_tiff, g16) == f"{g16} must contain 8-bit grayscale pages; found Gray(16)"
+
+    sizes = str(tmp_path / "sizes.tif")
+    tw.write(sizes, [np.zeros((4, 5), np.uint8), np.zeros((4, 5), np.uint8),
+                     np.zeros((6, 5), np.uint8)])
+    assert err(bm.read_binary_tiff, sizes) == (
+        f"TIFF pages in {sizes} do not have consistent dimensions: expected 5x4, got 5x6")
+
+    bad = volume(7)
+    bad[2, 3, 11] = 7
+    badp = str(tmp_path / "bad.tif")
+    pil_write(badp, bad)
+    assert err(bm.read_binary_tiff, badp, strict_binary=True) == (
+        f"{badp} is not a valid binary image: page 3, sample {3 * 70 + 11} has value 7 "
+        f"(expected 0 or 255)")
+    bm.read_binary_tiff(badp)  # any value is accepted when not strict
+    roi = volume(8, values=(0, 1))
+    roi[4, 0, 1] = 2
+    roip = str(tmp_path / "roi.tif")
+    pil_write(roip, roi)
+    assert err(bm.read_roi_tiff, roip) == (
+        f"{roip} is not a valid binary image: page 5, sample 1 has value 2 (expected 0/1 or 0/255)")
+
+    one = str(tmp_path / "one.tif")
+    pil_write(one, volume(9, d=1))
+    assert err(bm.read_binary_tiff, one) == (
+        "all dimensions must be at least 2 for 3-D morphometry; got 70x33x1")
+
+
+def test_first_failing_page_wins(tmp_path):
+    # page 2 has a bad sample, page 4 a wrong size: the sequential reader stops at page 2
+    pages = [np.zeros((6, 8), np.uint8) for _ in range(5)]
+    pages[1][2, 3] = 9
+    pages[3] = np.zeros((7, 8), np.uint8)
+    path = str(tmp_path / "two.tif")
+    tw.write(path, pages)
+    for t in (1, 4):
+        assert err(bm.read_binary_tiff, path, strict_binary=True, threads=t) == (
+            f"{path} is not a valid binary image: page 2, sample 19 has value 9 (expected 0 or 255)")
+
+
+# --- calibration (src/tiff_io.rs:149-317) ---------------------------------------------------
+def calibrated(path, description=None, unit=None, xres=None, yres=None):
+    extra = {}
+    if description is not None:
+        extra[270] = ("ASCII", description)
+    if unit is not None:
+        extra[296] = ("SHORT", [unit])
+    if xres is not None:
+        extra[282] = ("RATIONAL", [xres])
+    if yres is not None:
+        extra[283] = ("RATIONAL", [yres])
+    tw.write(path, [np.zeros((4, 5), np.uint8)] * 2, first_page_extra=extra)
+    return path
+
+
+def test_spacing_known_answers(tmp_path):
+    # ImageJ: unit=mm in the description, resolution in pixels per mm (ResolutionUnit 1)
+    p = calibrated(str(tmp_path / "a.tif"), "ImageJ=1.54\nimages=10\nspacing=0.025\nunit=mm\n",
+                   unit=1, xres=(40, 1), yres=(40, 1))
+    assert bm.read_isotropic_spacing_um(p) == [25.0, 25.0, 25.0]
+    # src/tiff_io.rs:320-331: 1e6 pixels per inch = 0.0254 um
+    p = calibrated(str(tmp_path / "b.tif"), "spacing=0.0000254\nunit=mm", unit=2,
+                   xres=(1000000, 1), yres=(1000000, 1))
+    s = bm.read_isotropic_spacing_um(p)
+    assert abs(s[0] - 0.0254) < 1e-12 and abs(s[1] - 0.0254) < 1e-12
+    # centimetre unit, micron named unit, CRLF lines and padding NULs
+    p = calibrated(str(tmp_path / "c.tif"), "ImageJ=1.54\r\nunit=micron\r\nspacing=20\0\0",
+                   unit=3, xres=(500, 1), yres=(500, 1))
+    assert bm.read_isotropic_spacing_um(p) == [20.0, 20.0, 20.0]
+
+
+def test_spacing_errors(tmp_path):
+    p = calibrated(str(tmp_path / "a.tif"), "spacing=20\nunit=um", unit=1, xres=(1, 20),
+                   yres=(1, 21))
+    assert err(bm.read_isotropic_spacing_um, p) == (
+        f"{p} is not isotropic in XY: x spacing=20 µm, y spacing=21 µm")
+    p = calibrated(str(tmp_path / "b.tif"), "spacing=21\nunit=um", unit=1, xres=(1, 20),
+                   yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == (
+        f"{p} is not isotropic in 3D: x spacing=20 µm, y spacing=20 µm, z spacing=21 µm")
+    p = calibrated(str(tmp_path / "c.tif"), "unit=um", unit=1, xres=(1, 20), yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == (
+        f"z spacing was not found in the 'spacing=' ImageDescription metadata of {p}; use "
+        f"--spacing-dir if the binarized TIFF lost calibration")
+    p = calibrated(str(tmp_path / "d.tif"), "spacing=20")
+    assert err(bm.read_isotropic_spacing_um, p) == (
+        f"x/y spacing was not found in the TIFF resolution metadata of {p}")
+    p = calibrated(str(tmp_path / "e.tif"), "spacing=20", unit=5, xres=(1, 20), yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == f"{p} has unsupported TIFF ResolutionUnit code 5"
+    p = calibrated(str(tmp_path / "f.tif"), "spacing=abc\nunit=um", unit=1, xres=(1, 20),
+                   yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == f"parsing spacing metadata in {p}"
+    p = calibrated(str(tmp_path / "g.tif"), "spacing=-2\nunit=um", unit=1, xres=(1, 20),
+                   yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == (
+        f"spacing in {p} must be finite and positive; got -2 µm")
+    p = calibrated(str(tmp_path / "h.tif"), "spacing=20\nunit=um", unit=1, xres=(0, 1),
+                   yres=(1, 20))
+    assert err(bm.read_isotropic_spacing_um, p) == f"{p} has invalid X/Y resolution values (0, 0.05)"
+
+
+def test_capacity_is_checked(tmp_path):
+    import ctypes
+    from bone_morphometry_amd import _native
+    path = str(tmp_path / "v.tif")
+    pil_write(path, volume(10))
+    dims = (ctypes.c_uint64 * 3)()
+    buf = np.empty(100, np.uint8)
+    rc = _native.lib().bm_read_binary_tiff(os.fsencode(path), 0, 0, 0, buf.ctypes.data, buf.size,
+                                           dims)
+    assert rc == _native.BM_ERR_INVALID
+    assert _native.lib().bm_last_error() == b"output buffer holds 100 bytes, the volume has 11550 voxels"
