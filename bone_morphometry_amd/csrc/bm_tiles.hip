@@ -121,13 +121,23 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   if (t == kNoTile) return;
   const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
-  for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
+  // stage the 10^3 block (tile + 1 halo): both of a thread's loads in flight
+  // before either is stored
+  auto halo_value = [&](uint32_t i) -> uint32_t {
     const int hx = (int)(i % 10), hy = (int)((i / 10) % 10), hz = (int)(i / 100);
     const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
     uint32_t v = 0;  // outside the array: never >= a template, never selected
     if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
       v = d2[((size_t)gz * H + gy) * W + gx];
-    s_d2[i] = v;
+    return v;
+  };
+  static_assert(2 * 512 >= 1000, "two staging loads per thread");
+  {
+    const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + 512;
+    const uint32_t v0 = halo_value(i0);
+    const uint32_t v1 = i1 < 1000 ? halo_value(i1) : 0u;
+    s_d2[i0] = v0;
+    if (i1 < 1000) s_d2[i1] = v1;
   }
   for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_cnt[i] = 0;
   __syncthreads();
@@ -144,18 +154,24 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   }
   bool ridge = sel;
   if (sel) {
+    // A ridge unless a neighbour's squared distance reaches the template of
+    // its offset class (1: face, 2: edge, 3: corner neighbour).  All 26
+    // neighbours are read at once (no early exit: one LDS wait, not 26).
     const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
-    for (int dz = -1; dz <= 1 && ridge; ++dz)
-      for (int dy = -1; dy <= 1 && ridge; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int k = (dx != 0) + (dy != 0) + (dz != 0);
-          if (k == 0) continue;
-          const uint32_t tv = k == 1 ? t1 : (k == 2 ? t2 : t3);
-          if (s_d2[c + dx + 10 * dy + 100 * dz] >= tv) {
-            ridge = false;
-            break;
-          }
-        }
+    uint32_t nb[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+      nb[q] = q == 13 ? 0u : s_d2[c + dx + 10 * dy + 100 * dz];
+    }
+    bool reached = false;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const int k = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
+      if (k == 0) continue;
+      reached |= nb[q] >= (k == 1 ? t1 : (k == 2 ? t2 : t3));
+    }
+    ridge = !reached;
   }
   const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
   if ((li & 63) == 0) {
