@@ -129,11 +129,13 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
   }
   __syncthreads();
   const uint32_t ng = s_ng;
-  // four consecutive values per thread and trip (one 16-byte load; the map is
-  // 256-byte aligned), so each thread keeps four loads in flight
+  // four consecutive values per 16-byte load (the map is 256-byte aligned),
+  // and kLoads such loads per thread and trip, each coalesced across the
+  // wave and all issued before the first is used
+  constexpr int kLoads = 4;
   const uint64_t n4 = (n + 3) / 4;
   const uint64_t stride = (uint64_t)I.qblocks * blockDim.x;
-  const uint64_t iters = (n4 + stride - 1) / stride;  // uniform trip count (wave ballots)
+  const uint64_t iters = (n4 + stride * kLoads - 1) / (stride * kLoads);  // uniform trip count
   auto classify = [&](uint32_t b, uint32_t& digit) {
     uint32_t g = ng;
     digit = 0;
@@ -145,24 +147,33 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
     return g;
   };
   for (uint64_t it = 0; it < iters; ++it) {
-    const uint64_t i4 = it * stride + (uint64_t)vb * blockDim.x + threadIdx.x;
-    uint32_t b[4] = {0, 0, 0, 0};
-    if (4 * i4 + 3 < n) {
-      const v4u32 w = *(BM_GLOBAL const v4u32*)(map + 4 * i4);
-      b[0] = w.x;
-      b[1] = w.y;
-      b[2] = w.z;
-      b[3] = w.w;
+    uint32_t b[kLoads][4];
+    const uint64_t trip0 = it * kLoads * stride + (uint64_t)vb * blockDim.x + threadIdx.x;
+    if (4 * (((it + 1) * kLoads) * stride) <= n) {  // the whole trip in bounds (uniform)
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const v4u32 w = *(BM_GLOBAL const v4u32*)(map + 4 * (trip0 + u * stride));
+        b[u][0] = w.x;
+        b[u][1] = w.y;
+        b[u][2] = w.z;
+        b[u][3] = w.w;
+      }
     } else {
-      for (int k = 0; k < 4; ++k)
-        if (4 * i4 + k < n) b[k] = __float_as_uint(map[4 * i4 + k]);
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const uint64_t i4 = trip0 + u * stride;
+        for (int k = 0; k < 4; ++k)
+          b[u][k] = 4 * i4 + k < n ? __float_as_uint(map[4 * i4 + k]) : 0u;
+      }
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      uint32_t digit;
-      const uint32_t g = classify(b[k], digit);
-      wave_hist_add(&h[0][0], g < ng, g * 256 + digit);
-    }
+    for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        uint32_t digit;
+        const uint32_t g = classify(b[u][k], digit);
+        wave_hist_add(&h[0][0], g < ng, g * 256 + digit);
+      }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) {
