@@ -64,7 +64,7 @@ __device__ __forceinline__ float diameter_of(uint32_t m) {
 // ---------------------------------------------------------------------------
 // NEIGHBORS_26 in Fiji's visiting order (src/local_thickness.rs:283-312); the
 // f32 sum of the clean-up runs in this order.
-static __constant__ int8_t c_nb26[26][3] = {
+constexpr int8_t kNb26[26][3] = {
     {0, 0, -1},  {0, 0, 1},   {0, -1, 0},  {0, 1, 0},   {-1, 0, 0},  {1, 0, 0},  {0, 1, -1},
     {0, 1, 1},   {1, -1, 0},  {1, 1, 0},   {-1, 0, 1},  {1, 0, 1},   {0, -1, -1}, {0, -1, 1},
     {-1, -1, 0}, {-1, 1, 0},  {-1, 0, -1}, {1, 0, -1},  {1, 1, 1},   {1, -1, 1},  {-1, 1, 1},
@@ -148,6 +148,23 @@ __device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint3
     if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[v], (uint32_t)__popcll(same));
     if (bin == v) active = false;
     act &= ~same;
+  }
+}
+
+// The same counts without a loop: one LDS atomic per run of consecutive active
+// lanes holding the same bin, the run's first lane adding the run's length.
+// Suited to lanes that hold neighbouring voxels of a map (runs are long); the
+// wavefront must be converged.
+__device__ __forceinline__ void wave_hist_add_runs(uint32_t* hist, bool active, uint32_t bin) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t key = active ? bin : 0xFFFFFFFFu;
+  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+  const bool head = active && (lane == 0 || prev != key);
+  // a run ends before the next head or the next inactive lane
+  const unsigned long long stop = __ballot(head) | ~__ballot(active);
+  if (head) {
+    const unsigned long long rest = lane == 63 ? 0ull : stop >> (lane + 1);  // lanes above
+    atomicAdd(&hist[bin], rest ? (uint32_t)__ffsll(rest) : 64u - lane);
   }
 }
 

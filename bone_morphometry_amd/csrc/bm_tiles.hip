@@ -881,6 +881,7 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
   __shared__ uint32_t s_zero[12 * 12];    // per (z, y) row: bit x set where M == 0
   __shared__ uint32_t s_mark[10 * 10];    // per row of the 1-halo box: surface bits
+  __shared__ float s_c[10 * 10 * 10];     // per voxel of the 1-halo box: its part of a mean
   __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
   __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
   uint32_t vb;
@@ -889,7 +890,7 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   BM_GLOBAL const unsigned long long* __restrict__ masks = gbl(I.masks);
   BM_GLOBAL const float* __restrict__ diam = gbl(I.diam);
   BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
-  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY;
   const float scale_f32 = I.scale;
   const int lsb_exp = I.lsb;
   float* __restrict__ map_out = (I.flags & kItemMap) ? I.map : nullptr;
@@ -909,15 +910,25 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // is loaded one tile ahead into registers to hide the HBM latency.
   uint32_t pre[4];
   unsigned long long pre_sel = 0;
+  // Halo coordinates of the 4 values, packed one byte each (0xFFFFFFFF: none).
+  // The empty asm in fetch keeps the compiler from hoisting the 12 unpacked
+  // coordinates out of the tile loop, which would spill them.
+  uint32_t hpk[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = threadIdx.x + 512u * k;
+    hpk[k] = i < 1728 ? (i % 12) | ((i / 12) % 12) << 8 | (i / 144) << 16 : 0xFFFFFFFFu;
+  }
   auto fetch = [&](uint32_t t) {
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
     pre_sel = masks[(size_t)t * 16 + (threadIdx.x >> 6)];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const uint32_t i = threadIdx.x + 512u * k;
+      uint32_t pk = hpk[k];
+      asm volatile("" : "+v"(pk));
       uint32_t v = kOut;
-      if (i < 1728) {
-        const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
+      if (pk != 0xFFFFFFFFu) {
+        const int hx = (int)(pk & 0xFF), hy = (int)((pk >> 8) & 0xFF), hz = (int)(pk >> 16);
         const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
         if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
           v = M[((size_t)gz * H + gy) * W + gx];
@@ -986,6 +997,18 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       s_mark[threadIdx.x] = (dil & nonzero) >> 1;  // bit hx (0..9) of the 1-halo box
     }
     __syncthreads();
+    // What a voxel of the 1-halo box adds to the mean of a surface neighbour
+    // (pass 2, src/local_thickness.rs:243-277): its diameter when it is
+    // positive, inside the array and not on the surface, else 0 (diameters of
+    // positive r^2 are positive).  Computed once per tile, so the 26 reads of
+    // a mean below are independent of each other.
+    for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
+      const uint32_t bx = i % 10, by = (i / 10) % 10, bz = i / 100;
+      const uint32_t mn = s_m[(bz + 1) * 144 + (by + 1) * 12 + (bx + 1)];
+      const bool part = mn != 0 && mn != kOut && !((s_mark[bz * 10 + by] >> bx) & 1u);
+      s_c[i] = part ? dia(mn) : 0.0f;
+    }
+    __syncthreads();
     const bool sel = (selw >> (li & 63)) & 1ull;
     float tv = 0.0f;
     if (sel) {
@@ -999,14 +1022,25 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       } else {
         // pass 2 (src/local_thickness.rs:243-277): f32 mean of the positive,
         // non-surface neighbours in NEIGHBORS_26 order
+        // (two groups of 13 reads in flight, kept apart so that the 26 do not
+        // all hold registers at once)
+        const int c10 = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
         float sum = 0.0f;
         uint32_t count = 0;
-        for (int k = 0; k < 26; ++k) {
-          const int dx = c_nb26[k][0], dy = c_nb26[k][1], dz = c_nb26[k][2];
-          const uint32_t mn = s_m[c12 + dx + 12 * dy + 144 * dz];
-          if (mn == 0 || mn == kOut || marked(dx, dy, dz)) continue;
-          sum += dia(mn);
-          count += 1;
+#pragma unroll
+        for (int g = 0; g < 26; g += 13) {
+          float c[13];
+#pragma unroll
+          for (int k = 0; k < 13; ++k)
+            c[k] = s_c[c10 + kNb26[g + k][0] + 10 * kNb26[g + k][1] + 100 * kNb26[g + k][2]];
+#pragma unroll
+          for (int k = 0; k < 13; ++k) {
+            if (c[k] > 0.0f) {
+              sum += c[k];
+              count += 1;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
         tv = count > 0 ? sum / (float)count : dia(m);
       }
@@ -1016,7 +1050,7 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       const uint32_t bin = (__float_as_uint(tv) >> 16) - kTopBase;
       const bool in = tv > 0.0f && bin < (uint32_t)kTopBins;
       if (tv > 0.0f && !in) flags |= 2u;  // outside the histogram's range
-      wave_hist_add(s_top, in, in ? bin : 0u);
+      wave_hist_add_runs(s_top, in, in ? bin : 0u);
     }
     if (map_out && valid) map_out[gidx] = tv;
     __syncthreads();  // s_m, s_mark are rewritten by the next tile
