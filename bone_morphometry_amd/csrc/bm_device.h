@@ -201,6 +201,50 @@ __device__ __forceinline__ uint32_t tile_at(uint32_t j, uint32_t TX, uint32_t TY
   return (tz * TY + ty) * TX + tx;
 }
 
+// Exact quotient n / d of 32-bit unsigned numbers for a divisor d >= 1 fixed
+// in advance: one high multiply and two shifts (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1).
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+  __host__ __device__ explicit FastDiv(uint32_t divisor) : d(divisor) {
+    uint32_t l = 0;  // ceil(log2 d)
+    while (l < 32 && (1ull << l) < (unsigned long long)divisor) ++l;
+    m = (uint32_t)(((((1ull << l) - divisor) << 32) / divisor) + 1);
+    s1 = l > 0 ? 1u : 0u;
+    s2 = l > 0 ? l - 1 : 0u;
+  }
+  __host__ __device__ uint32_t div(uint32_t n) const {
+    const uint32_t t = (uint32_t)(((unsigned long long)m * n) >> 32);
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
+// Tile coordinates of position j in the tile order above without runtime
+// divisions (for loops over many positions); false for kNoTile.
+struct TileWalk {
+  uint32_t TX, TY, TZ;
+  bool grouped;
+  FastDiv fx, fy;  // SX, SY (grouped) or TX, TY (plain)
+  __device__ TileWalk(uint32_t tx_, uint32_t ty_, uint32_t tz_, bool g)
+      : TX(tx_), TY(ty_), TZ(tz_), grouped(g), fx(g ? (tx_ + 3) >> 2 : tx_),
+        fy(g ? (ty_ + 3) >> 2 : ty_) {}
+  __device__ bool at(uint32_t j, uint32_t& tx, uint32_t& ty, uint32_t& tz) const {
+    if (!grouped) {
+      const uint32_t q = fx.div(j), r = fy.div(q);
+      tx = j - q * TX;
+      ty = q - r * TY;
+      tz = r;
+      return tz < TZ;
+    }
+    const uint32_t s = j >> 6, l = j & 63;
+    const uint32_t q = fx.div(s), r = fy.div(q);
+    tx = (s - q * fx.d) * 4 + (l & 3);
+    ty = (q - r * fy.d) * 4 + ((l >> 2) & 3);
+    tz = r * 4 + (l >> 4);
+    return tx < TX && ty < TY && tz < TZ;
+  }
+};
+
 // Item of block b in a stage: the last item whose start is <= b (items with
 // no block in the stage share their start with the next one).  starts has
 // nitems + 1 entries; the result and the block of the item are uniform over

@@ -879,7 +879,7 @@ __global__ void __launch_bounds__(512, 6)
 k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                 uint32_t nitems) {
   __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
-  __shared__ uint32_t s_zero[12 * 12];    // per (z, y) row: bit x set where M == 0
+  __shared__ uint32_t s_zb[54];           // bit i set where halo value i is 0 (1728 bits)
   __shared__ uint32_t s_mark[10 * 10];    // per row of the 1-halo box: surface bits
   __shared__ float s_c[10 * 10 * 10];     // per voxel of the 1-halo box: its part of a mean
   __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
@@ -910,25 +910,40 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // is loaded one tile ahead into registers to hide the HBM latency.
   uint32_t pre[4];
   unsigned long long pre_sel = 0;
-  // Halo coordinates of the 4 values, packed one byte each (0xFFFFFFFF: none).
-  // The empty asm in fetch keeps the compiler from hoisting the 12 unpacked
-  // coordinates out of the tile loop, which would spill them.
-  uint32_t hpk[4];
+  // When 12 planes of the volume span less than 4 GiB, a thread's voxels are
+  // at fixed 32-bit byte offsets from a tile's first voxel (map) and from its
+  // halo's first voxel (the 4 halo values; halo values past 1728 are never
+  // read).  Interior tiles then load and store with no index arithmetic.
+  const bool near = (unsigned long long)H * W * 12 * 4 < (1ull << 32);
+  uint32_t hoff[4], loff = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const uint32_t i = threadIdx.x + 512u * k;
-    hpk[k] = i < 1728 ? (i % 12) | ((i / 12) % 12) << 8 | (i / 144) << 16 : 0xFFFFFFFFu;
+    const uint32_t i = min(threadIdx.x + 512u * k, 1727u);
+    hoff[k] = near ? 4 * (((i / 144) * H + (i / 12) % 12) * W + i % 12) : 0u;
   }
-  auto fetch = [&](uint32_t t) {
-    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
-    pre_sel = masks[(size_t)t * 16 + (threadIdx.x >> 6)];
+  if (near) loff = 4 * (((uint32_t)lz * H + (uint32_t)ly) * W + (uint32_t)lx);
+  auto fetch = [&](uint32_t tx, uint32_t ty, uint32_t tz) {
+    const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
+    pre_sel = masks[((size_t)(tz * TY + ty) * TX + tx) * 16 + (threadIdx.x >> 6)];
+    const bool interior = near && ox >= 2 && oy >= 2 && oz >= 2 && ox + 10 <= (int)W &&
+                          oy + 10 <= (int)H && oz + 10 <= (int)D;
+    if (interior) {
+      const char* hb = (const char*)(M + (((size_t)(oz - 2) * H + (oy - 2)) * W + (ox - 2)));
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        pre[k] = (k < 3 || threadIdx.x < 1728 - 3 * 512) ? *(const uint32_t*)(hb + hoff[k]) : kOut;
+      return;
+    }
+    // (the empty asm keeps the coordinates below from being hoisted out of the
+    // tile loop, which would spill them)
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      uint32_t pk = hpk[k];
-      asm volatile("" : "+v"(pk));
+      const uint32_t i = tid + 512u * k;
       uint32_t v = kOut;
-      if (pk != 0xFFFFFFFFu) {
-        const int hx = (int)(pk & 0xFF), hy = (int)((pk >> 8) & 0xFF), hz = (int)(pk >> 16);
+      if (i < 1728) {
+        const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
         const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
         if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
           v = M[((size_t)gz * H + gy) * W + gx];
@@ -944,69 +959,85 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const uint32_t per = (npos + kXcds - 1) / kXcds;
   const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, npos);
   const uint32_t jstep = I.cblocks / kXcds;
+  const TileWalk walk(TX, I.TY, I.TZ, grouped);
   // next position at or after j that holds a tile (uniform over the block)
-  auto next_tile = [&](uint32_t j, uint32_t& t) {
-    while (j < xend && (t = tile_at(j, TX, I.TY, I.TZ, grouped)) == kNoTile) j += jstep;
+  auto next_tile = [&](uint32_t j, uint32_t& tx, uint32_t& ty, uint32_t& tz) {
+    while (j < xend && !walk.at(j, tx, ty, tz)) j += jstep;
     return j;
   };
-  uint32_t t = kNoTile, tn = kNoTile;
-  uint32_t j = valid_phase ? next_tile(xbase + vb / kXcds, t) : xend;
-  if (j < xend) fetch(t);
+  uint32_t tx = 0, ty = 0, tz = 0, nx = 0, ny = 0, nz = 0;
+  uint32_t j = valid_phase ? next_tile(xbase + vb / kXcds, tx, ty, tz) : xend;
+  if (j < xend) fetch(tx, ty, tz);
   while (j < xend) {
-    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
     const unsigned long long selw = pre_sel;
-    const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
-    const size_t gidx = ((size_t)(oz + lz) * H + (oy + ly)) * W + (ox + lx);
+    const bool full = ox + 8 <= (int)W && oy + 8 <= (int)H && oz + 8 <= (int)D;
+    const bool valid = full || (ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D);
+    const size_t tbase = ((size_t)oz * H + oy) * W + ox;  // the tile's first voxel
+    auto store = [&](float v) {
+      if (!map_out || !valid) return;
+      if (near) *(float*)((char*)(map_out + tbase) + loff) = v;
+      else map_out[tbase + ((size_t)lz * H + ly) * W + lx] = v;
+    };
+    // (this barrier also ends every read of the previous tile's arrays)
     const bool any_sel = __syncthreads_or(selw != 0);
     if (any_sel) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (threadIdx.x + 512u * k < 1728) s_m[threadIdx.x + 512u * k] = pre[k];
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t i = threadIdx.x + 512u * k;
+        if (i < 1728) s_m[i] = pre[k];
+        // zero bits of the halo from a ballot: 64 values per wavefront and k
+        const unsigned long long zb = __ballot(i < 1728 && pre[k] == 0);
+        const uint32_t w = 2 * (8 * k + (threadIdx.x >> 6));  // word holding bit i - lane
+        const uint32_t lane = threadIdx.x & 63;
+        if (lane < 2 && w + lane < 54) s_zb[w + lane] = lane ? (uint32_t)(zb >> 32) : (uint32_t)zb;
+      }
     }
-    const uint32_t jn = next_tile(j + jstep, tn);
-    if (jn < xend) fetch(tn);  // next tile, in flight meanwhile
+    const uint32_t jn = next_tile(j + jstep, nx, ny, nz);
+    if (jn < xend) fetch(nx, ny, nz);  // next tile, in flight meanwhile
     if (!any_sel) {  // no selected voxel in this tile
-      if (map_out && valid) map_out[gidx] = 0.0f;
+      store(0.0f);
       j = jn;
-      t = tn;
+      tx = nx;
+      ty = ny;
+      tz = nz;
       continue;
-    }
-    __syncthreads();
-    if (threadIdx.x < 144) {  // zero mask of a 12-voxel row
-      uint32_t z = 0;
-      for (int x = 0; x < 12; ++x) z |= (s_m[threadIdx.x * 12 + x] == 0 ? 1u : 0u) << x;
-      s_zero[threadIdx.x] = z;
     }
     __syncthreads();
     if (threadIdx.x < 100) {
       // surface (pass 1, src/local_thickness.rs:214-238): M != 0 with a zero
       // among the 26 neighbours; the neighbourhood test is a 3x3x3 dilation of
-      // the zero bits (out-of-array voxels hold kOut, never zero)
-      const int ry = (int)(threadIdx.x % 10) + 1, rz = (int)(threadIdx.x / 10) + 1;
+      // the zero bits (out-of-array voxels hold kOut, never zero).  One thread
+      // per row (rz, ry) of the 1-halo box.  (The empty asm keeps the
+      // row's addresses from being hoisted out of the tile loop and spilled.)
+      uint32_t row = threadIdx.x;
+      asm volatile("" : "+v"(row));
+      const int ry = (int)(row % 10) + 1, rz = (int)(row / 10) + 1;
       uint32_t dil = 0;
       for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy) {
-          const uint32_t z = s_zero[(rz + dz) * 12 + (ry + dy)];
+          const uint32_t b = 12u * (uint32_t)((rz + dz) * 12 + (ry + dy));  // row's first bit
+          const uint32_t w = b >> 5;
+          const unsigned long long pair =
+              (unsigned long long)s_zb[w] | (unsigned long long)s_zb[min(w + 1, 53u)] << 32;
+          const uint32_t z = (uint32_t)(pair >> (b & 31)) & 0xFFFu;
           dil |= z | (z << 1) | (z >> 1);
         }
-      uint32_t nonzero = 0;
+      // Also what each voxel of the row adds to the mean of a surface
+      // neighbour (pass 2, src/local_thickness.rs:243-277): its diameter when
+      // it is positive, inside the array and not on the surface, else 0
+      // (diameters of positive r^2 are positive).  Tabled once per tile, so
+      // the 26 reads of a mean below are independent of each other.
+      uint32_t mark = 0;
       for (int x = 1; x <= 10; ++x) {
         const uint32_t v = s_m[(rz * 12 + ry) * 12 + x];
-        nonzero |= (v != 0 && v != kOut ? 1u : 0u) << x;
+        const bool nonzero = v != 0 && v != kOut;
+        const bool surface = nonzero && ((dil >> x) & 1u);
+        mark |= (surface ? 1u : 0u) << (x - 1);  // bit hx (0..9) of the 1-halo box
+        const bool part = nonzero && !surface;
+        s_c[row * 10 + (x - 1)] = part ? dia(part ? v : 0u) : 0.0f;
       }
-      s_mark[threadIdx.x] = (dil & nonzero) >> 1;  // bit hx (0..9) of the 1-halo box
-    }
-    __syncthreads();
-    // What a voxel of the 1-halo box adds to the mean of a surface neighbour
-    // (pass 2, src/local_thickness.rs:243-277): its diameter when it is
-    // positive, inside the array and not on the surface, else 0 (diameters of
-    // positive r^2 are positive).  Computed once per tile, so the 26 reads of
-    // a mean below are independent of each other.
-    for (uint32_t i = threadIdx.x; i < 1000; i += blockDim.x) {
-      const uint32_t bx = i % 10, by = (i / 10) % 10, bz = i / 100;
-      const uint32_t mn = s_m[(bz + 1) * 144 + (by + 1) * 12 + (bx + 1)];
-      const bool part = mn != 0 && mn != kOut && !((s_mark[bz * 10 + by] >> bx) & 1u);
-      s_c[i] = part ? dia(mn) : 0.0f;
+      s_mark[row] = mark;
     }
     __syncthreads();
     const bool sel = (selw >> (li & 63)) & 1ull;
@@ -1052,10 +1083,11 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       if (tv > 0.0f && !in) flags |= 2u;  // outside the histogram's range
       wave_hist_add_runs(s_top, in, in ? bin : 0u);
     }
-    if (map_out && valid) map_out[gidx] = tv;
-    __syncthreads();  // s_m, s_mark are rewritten by the next tile
+    store(tv);
     j = jn;
-    t = tn;
+    tx = nx;
+    ty = ny;
+    tz = nz;
   }
   if (top16) {
     __syncthreads();
