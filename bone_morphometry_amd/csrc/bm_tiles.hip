@@ -235,31 +235,6 @@ __device__ __forceinline__ uint32_t reach_len(BM_GLOBAL const TileMeta* __restri
   return m->pref[j];
 }
 
-// Runs of equal bucket among the kept lanes of a wavefront whose buckets do
-// not decrease with the lane index (a chunk of a sorted list).  Returns, for a
-// kept lane, its rank within its run and (for the run's first kept lane,
-// `head`) the run length; one LDS atomic per run then replaces one per lane.
-__device__ __forceinline__ void run_of_bucket(bool keep, uint32_t bucket, uint32_t lane,
-                                              uint32_t& rank, uint32_t& len, bool& head) {
-  const unsigned long long km = __ballot(keep);
-  // a kept lane starts a run when no earlier kept lane has the same bucket;
-  // buckets are monotone, so it suffices to find the previous kept lane
-  const unsigned long long below = km & ((1ull << lane) - 1ull);
-  const int pk = below ? 63 - __clzll(below) : -1;
-  const uint32_t pkb = (uint32_t)__shfl((int)bucket, pk < 0 ? 0 : pk);
-  head = keep && (pk < 0 || pkb != bucket);
-  const unsigned long long hm = __ballot(head);
-  // run of this lane: from the last head at or below it to the next head
-  const unsigned long long hb = hm & ((2ull << lane) - 1ull);  // heads at or below (lane<63 ok)
-  const int h0 = lane == 63 ? (hm ? 63 - __clzll(hm) : 0) : (hb ? 63 - __clzll(hb) : 0);
-  const unsigned long long above = lane == 63 ? 0ull : (hm & (~0ull << (lane + 1)));
-  const int h1 = above ? __ffsll(above) - 1 : 64;
-  const unsigned long long seg = (h1 == 64 ? ~0ull : ((1ull << h1) - 1ull)) & ~((1ull << h0) - 1ull);
-  rank = (uint32_t)__popcll(km & seg & ((1ull << lane) - 1ull));
-  len = (uint32_t)__popcll(km & seg);
-}
-
-
 // Candidate of a list entry of tile (ux, uy, uz) (tile units) for the tile box
 // [ox, hx] x [oy, hy] x [oz, hz]: position relative to (ox, oy, oz) and r^2;
 // returns false when the ball misses the box.
@@ -496,15 +471,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       }
     }
   };
-  // P2: histogram by bucket, one LDS atomic per run of equal buckets; the
-  // candidates are also staged in LDS (one slot reservation per chunk) so that
-  // the placement passes need not read the lists again.
+  // P2: histogram by bucket, one LDS atomic per kept candidate (the LDS is
+  // otherwise idle here, the vector units are not); the candidates are also
+  // staged in LDS (one slot reservation per chunk) so that the placement
+  // passes need not read the lists again.
   chunk_pass([&](bool keep, uint32_t pos, uint32_t r2) {
     const uint32_t b = keep ? gtop - (r2 >> gshift) : 0u;
-    uint32_t rank, len;
-    bool head;
-    run_of_bucket(keep, b, lane, rank, len, head);
-    if (head) atomicAdd(&s_cnt[b], len);
+    if (keep) atomicAdd(&s_cnt[b], 1u);
     const unsigned long long km = __ballot(keep);
     if (km) {
       uint32_t k0 = 0;
@@ -561,21 +534,14 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     }
     const uint32_t base = s_start[b0], cnt_round = s_start[b1] - base;
     const unsigned long long c3 = diag ? wall_clock64() : 0ull;
-    // P3: place this round's candidates at their sorted positions.
+    // P3: place this round's candidates at their sorted positions (any order
+    // inside a bucket: the scan below finishes a bucket before it leaves, and
+    // a maximum does not depend on the order of its terms).
     auto place = [&](bool keep0, uint32_t pos, uint32_t r2) {
       const uint32_t b = keep0 ? gtop - (r2 >> gshift) : 0u;
       const bool keep = keep0 && b >= b0 && b < b1;
-      uint32_t rank, len;
-      bool head;
-      run_of_bucket(keep, b, lane, rank, len, head);
-      uint32_t k0 = 0;
-      if (head) k0 = atomicAdd(&s_cnt[b], len);
-      const unsigned long long hm = __ballot(head);
-      const unsigned long long hb = hm & ((2ull << lane) - 1ull);
-      const int hl = hb ? 63 - __clzll(hb) : 0;
-      k0 = (uint32_t)__shfl((int)k0, hl);
       if (keep) {
-        const uint32_t k = k0 + rank - base;
+        const uint32_t k = atomicAdd(&s_cnt[b], 1u) - base;
         s_pos[k] = pos;
         s_r2[k] = r2;
       }
