@@ -189,16 +189,24 @@ __host__ __device__ inline uint32_t tile_positions(uint32_t TX, uint32_t TY, uin
                                                    bool grouped) {
   return grouped ? ((TX + 3) / 4) * ((TY + 3) / 4) * ((TZ + 3) / 4) * 64 : TX * TY * TZ;
 }
-__device__ __forceinline__ uint32_t tile_at(uint32_t j, uint32_t TX, uint32_t TY, uint32_t TZ,
-                                            bool grouped) {
-  if (!grouped) return j < TX * TY * TZ ? j : kNoTile;
+// Coordinates of the tile at position j of that order, with two divisions;
+// false when the position holds no tile (kNoTile).
+__device__ __forceinline__ bool tile_xyz(uint32_t j, uint32_t TX, uint32_t TY, uint32_t TZ,
+                                         bool grouped, uint32_t& tx, uint32_t& ty, uint32_t& tz) {
+  if (!grouped) {
+    const uint32_t q = j / TX, r = q / TY;
+    tx = j - q * TX;
+    ty = q - r * TY;
+    tz = r;
+    return tz < TZ;
+  }
   const uint32_t SX = (TX + 3) >> 2, SY = (TY + 3) >> 2;
   const uint32_t s = j >> 6, l = j & 63;
-  const uint32_t tx = (s % SX) * 4 + (l & 3);
-  const uint32_t ty = ((s / SX) % SY) * 4 + ((l >> 2) & 3);
-  const uint32_t tz = (s / (SX * SY)) * 4 + (l >> 4);
-  if (tx >= TX || ty >= TY || tz >= TZ) return kNoTile;
-  return (tz * TY + ty) * TX + tx;
+  const uint32_t q = s / SX, r = q / SY;
+  tx = (s - q * SX) * 4 + (l & 3);
+  ty = (q - r * SY) * 4 + ((l >> 2) & 3);
+  tz = r * 4 + (l >> 4);
+  return tx < TX && ty < TY && tz < TZ;
 }
 
 // Exact quotient n / d of 32-bit unsigned numbers for a divisor d >= 1 fixed
@@ -219,8 +227,7 @@ struct FastDiv {
   }
 };
 
-// Tile coordinates of position j in the tile order above without runtime
-// divisions (for loops over many positions); false for kNoTile.
+// tile_xyz without runtime divisions (for loops over many positions).
 struct TileWalk {
   uint32_t TX, TY, TZ;
   bool grouped;

@@ -49,60 +49,14 @@ __device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
   return b > kBucketBits ? b - kBucketBits : 0;
 }
 
-__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* s_tmp) {
-  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) s_tmp[wv] = v;
-  __syncthreads();
-  uint32_t r = 0;
-  for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) r = max(r, s_tmp[i]);
-  __syncthreads();
-  return r;
-}
-
-// Exclusive scan of s_cnt[0..kBuckets) in place by a 512-thread block; returns
-// the total.  Each thread owns kBuckets/512 consecutive counters.
-__device__ uint32_t block_exclusive_scan_buckets(uint32_t* s_cnt, uint32_t* s_tmp) {
-  constexpr int per = kBuckets / 512;
-  const uint32_t t = threadIdx.x;
-  uint32_t local[per], sum = 0;
-#pragma unroll
-  for (int i = 0; i < per; ++i) {
-    local[i] = s_cnt[t * per + i];
-    sum += local[i];
-  }
-  // inclusive scan of sum across the block
-  const uint32_t lane = t & 63, wv = t >> 6;
-  uint32_t x = sum;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
-  }
-  __syncthreads();
-  if (lane == 63) s_tmp[wv] = x;
-  __syncthreads();
-  uint32_t wave_off = 0;
-  for (uint32_t i = 0; i < wv; ++i) wave_off += s_tmp[i];
-  uint32_t total = 0;
-  for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) total += s_tmp[i];
-  uint32_t run = wave_off + x - sum;  // exclusive prefix of this thread
-#pragma unroll
-  for (int i = 0; i < per; ++i) {
-    s_cnt[t * per + i] = run;
-    run += local[i];
-  }
-  __syncthreads();
-  return total;
-}
-
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(512)
 k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                uint32_t nitems) {
   __shared__ uint32_t s_d2[10 * 10 * 10];
   __shared__ uint32_t s_cnt[kBuckets];
-  __shared__ uint32_t s_tmp[16];
+  __shared__ uint32_t s_pref[16];
+  __shared__ uint32_t s_max, s_count;
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const uint32_t* __restrict__ d2 = gbl(I.d2);
@@ -117,9 +71,9 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
   const uint32_t j = xcd_tile(vb, npos);
   if (j >= npos) return;
-  const uint32_t t = tile_at(j, TX, TY, TZ, grouped);
-  if (t == kNoTile) return;
-  const uint32_t tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+  uint32_t tx, ty, tz;
+  if (!tile_xyz(j, TX, TY, TZ, grouped, tx, ty, tz)) return;
+  const uint32_t t = (tz * TY + ty) * TX + tx;
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   // stage the 10^3 block (tile + 1 halo): both of a thread's loads in flight
   // before either is stored
@@ -139,9 +93,11 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
     s_d2[i0] = v0;
     if (i1 < 1000) s_d2[i1] = v1;
   }
-  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_cnt[i] = 0;
+  static_assert(kBuckets == 512, "one bucket counter per thread");
+  s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_max = 0;
   __syncthreads();
-  const uint32_t li = threadIdx.x;
+  const uint32_t li = threadIdx.x, lane = li & 63;
   const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
   const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
   const int c = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
@@ -155,8 +111,9 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   bool ridge = sel;
   if (sel) {
     // A ridge unless a neighbour's squared distance reaches the template of
-    // its offset class (1: face, 2: edge, 3: corner neighbour).  All 26
-    // neighbours are read at once (no early exit: one LDS wait, not 26).
+    // its offset class (1: face, 2: edge, 3: corner neighbour), i.e. unless
+    // the largest value of a class reaches that class's template.  All 26
+    // neighbours are read at once (one LDS wait, not 26).
     const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
     uint32_t nb[27];
 #pragma unroll
@@ -164,40 +121,65 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
       const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
       nb[q] = q == 13 ? 0u : s_d2[c + dx + 10 * dy + 100 * dz];
     }
-    bool reached = false;
+    uint32_t mx[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       const int k = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
-      if (k == 0) continue;
-      reached |= nb[q] >= (k == 1 ? t1 : (k == 2 ? t2 : t3));
+      mx[k] = max(mx[k], nb[q]);
     }
-    ridge = !reached;
+    ridge = !(mx[1] >= t1 || mx[2] >= t2 || mx[3] >= t3);
   }
   const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
-  if ((li & 63) == 0) {
+  if (lane == 0) {
     masks[(size_t)t * 16 + (li >> 6)] = msel;
     masks[(size_t)t * 16 + 8 + (li >> 6)] = mtgt;
   }
-  const uint32_t maxr2 = block_max_u32(ridge ? r2 : 0u, s_tmp);
+  // largest ridge r^2 and bucket counts: LDS atomics (a tile holds few ridges)
   const uint32_t shift = bucket_shift(st->max_d2);  // one key for every tile
   const uint32_t top = st->max_d2 >> shift;
   uint32_t bucket = 0;
   if (ridge) {
     bucket = top - (r2 >> shift);  // 0 = largest radii
+    atomicMax(&s_max, r2);
     atomicAdd(&s_cnt[bucket], 1u);
   }
   __syncthreads();
-  const uint32_t count = block_exclusive_scan_buckets(s_cnt, s_tmp);
-  // prefix counts: entries with key >= kPrefR2(j) >> shift are the buckets
-  // 0 .. top - (kPrefR2(j) >> shift), i.e. the start of the next bucket
-  if (threadIdx.x < 16) {
-    const uint32_t kj = kPrefR2((int)threadIdx.x) >> shift;
-    uint32_t p = 0;
-    if (kj <= top) {
-      const uint32_t nb = top - kj + 1;  // first bucket below the threshold
-      p = nb < (uint32_t)kBuckets ? s_cnt[nb] : count;
+  if (li < 64) {
+    // exclusive scan of the counts by one wavefront, 8 buckets per lane
+    uint32_t local[8], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      local[i] = s_cnt[lane * 8 + i];
+      sum += local[i];
     }
-    s_tmp[threadIdx.x] = p;
+    uint32_t x = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= (uint32_t)off) x += y;
+    }
+    uint32_t run = x - sum;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      s_cnt[lane * 8 + i] = run;
+      run += local[i];
+    }
+    const uint32_t count = (uint32_t)__shfl((int)x, 63);
+    // (the starts written above are read by other lanes of this wavefront)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // prefix counts: entries with key >= kPrefR2(j) >> shift are the buckets
+    // 0 .. top - (kPrefR2(j) >> shift), i.e. the start of the next bucket
+    if (lane < 16) {
+      const uint32_t kj = kPrefR2((int)lane) >> shift;
+      uint32_t p = 0;
+      if (kj <= top) {
+        const uint32_t nb = top - kj + 1;  // first bucket below the threshold
+        p = nb < (uint32_t)kBuckets ? s_cnt[nb] : count;
+      }
+      s_pref[lane] = p;
+    }
+    if (lane == 0) s_count = count;
   }
   __syncthreads();
   if (ridge) {
@@ -206,11 +188,11 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   }
   if (threadIdx.x == 0) {
     TileMeta m;
-    m.count = count;
-    m.maxr2 = maxr2;
-    for (int j = 0; j < 16; ++j) m.pref[j] = (uint16_t)s_tmp[j];
+    m.count = s_count;
+    m.maxr2 = s_max;
+    for (int q = 0; q < 16; ++q) m.pref[q] = (uint16_t)s_pref[q];
     *(TileMeta*)&meta[t] = m;
-    if (maxr2 >= (1u << 23)) atomicOr((uint32_t*)&st->error, 1u);
+    if (m.maxr2 >= (1u << 23)) atomicOr((uint32_t*)&st->error, 1u);
   }
 }
 
@@ -319,11 +301,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
     const uint32_t jpos = xcd_tile(vb, npos);
     if (jpos >= npos) return;
-    t = tile_at(jpos, TX, TY, TZ, grouped);
-    if (t == kNoTile) return;
-    tx = t % TX;
-    ty = (t / TX) % TY;
-    tz = t / (TX * TY);
+    if (!tile_xyz(jpos, TX, TY, TZ, grouped, tx, ty, tz)) return;
+    t = (tz * TY + ty) * TX + tx;
   } else {
     const uint32_t TG = (TX + TXN - 1) / TXN;  // groups of TXN tiles along x
     const uint32_t npos = TG * TY * TZ;
@@ -431,12 +410,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const uint32_t gshift = bucket_shift(max_r2);
   const uint32_t gtop = max_r2 >> gshift;
   // Chunks q = wv, wv + NW, ... of the flattened (neighbour, 64 entries)
-  // range; one chunk is a sorted run of one list.  Read four chunks per trip.
+  // range; one chunk is a sorted run of one list.  Read kTrip chunks per trip.
+  constexpr int kTrip = 4;
   auto chunk_pass = [&](auto&& use) {
-    for (uint32_t q0 = wv; q0 < nchunks; q0 += 4 * NW) {
-      uint32_t ent[4], org[4];
+    for (uint32_t q0 = wv; q0 < nchunks; q0 += kTrip * NW) {
+      uint32_t ent[kTrip], org[kTrip];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kTrip; ++k) {
         const uint32_t q = q0 + (uint32_t)NW * k;
         ent[k] = 0;
         org[k] = 0;
@@ -461,7 +441,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         }
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kTrip; ++k) {
         if (q0 + (uint32_t)NW * k >= nchunks) break;  // uniform over the wave
         uint32_t pos = 0, r2 = 0;
         const bool keep = ent[k] != 0 &&
