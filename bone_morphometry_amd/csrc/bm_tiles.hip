@@ -35,6 +35,8 @@
 // means in NEIGHBORS_26 order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bm_device.h"
 #include "bm_engine.h"
 #include "bm_tiles.h"
@@ -412,33 +414,40 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // Chunks q = wv, wv + NW, ... of the flattened (neighbour, 64 entries)
   // range; one chunk is a sorted run of one list.  Read kTrip chunks per trip.
   constexpr int kTrip = 4;
-  auto chunk_pass = [&](auto&& use) {
+  // When every chunk is in the chunk map (the usual case; uniform over the
+  // block) a trip has no loop in it, so the LDS lookups of its chunks and then
+  // their list loads are all issued together; otherwise a chunk past the map
+  // finds its slot by a binary search over the offsets.
+  const bool mapped = nchunks <= (uint32_t)kCmap;
+  auto chunk_trips = [&](auto mapped_tag, auto&& use) {
+    constexpr bool kMapped = decltype(mapped_tag)::value;
     for (uint32_t q0 = wv; q0 < nchunks; q0 += kTrip * NW) {
       uint32_t ent[kTrip], org[kTrip];
 #pragma unroll
       for (int k = 0; k < kTrip; ++k) {
         const uint32_t q = q0 + (uint32_t)NW * k;
-        ent[k] = 0;
-        org[k] = 0;
-        if (q < nchunks) {
-          uint32_t cur;
-          if (q < (uint32_t)kCmap) {
-            cur = s_cmap[q];
-          } else {  // last slot with s_off <= q (offsets increase strictly)
-            uint32_t lo = 0, hi = nn - 1;
-            while (lo < hi) {
-              const uint32_t mid = (lo + hi + 1) >> 1;
-              if (s_off[mid] <= q) lo = mid;
-              else hi = mid - 1;
-            }
-            cur = lo;
+        uint32_t cur;
+        if (kMapped || q < (uint32_t)kCmap) {
+          cur = s_cmap[min(q, nchunks - 1)];
+        } else {  // last slot with s_off <= q (offsets increase strictly)
+          uint32_t lo = 0, hi = nn - 1;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= q) lo = mid;
+            else hi = mid - 1;
           }
-          const uint32_t e = (q - s_off[cur]) * 64 + lane;
-          const uint32_t o = s_org[cur];
-          org[k] = o;
-          const uint32_t u = ((o >> 20) * TY + ((o >> 10) & 1023)) * TX + (o & 1023);
-          if (e < s_len[cur]) ent[k] = lists[(size_t)u * 512 + e];
+          cur = lo;
         }
+        const uint32_t e = (q - s_off[cur]) * 64 + lane;
+        const uint32_t o = s_org[cur], len = s_len[cur];
+        org[k] = o;
+        const uint32_t u = ((o >> 20) * TY + ((o >> 10) & 1023)) * TX + (o & 1023);
+        // Every read here is unconditional (the list index clamped to the
+        // tile's 512-entry slot), so no branch separates the trip's chunks.  A
+        // chunk q >= nchunks has e >= len: its slot is the last chunk's, which
+        // ends at q - 1 < q.
+        const uint32_t v = lists[(size_t)u * 512 + min(e, 511u)];
+        ent[k] = e < len ? v : 0u;
       }
 #pragma unroll
       for (int k = 0; k < kTrip; ++k) {
@@ -450,6 +459,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         use(keep, pos, r2);
       }
     }
+  };
+  auto chunk_pass = [&](auto&& use) {
+    if (mapped) chunk_trips(std::true_type{}, use);
+    else chunk_trips(std::false_type{}, use);
   };
   // P2: histogram by bucket, one LDS atomic per kept candidate (the LDS is
   // otherwise idle here, the vector units are not); the candidates are also
