@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace bm {
@@ -17,6 +18,8 @@ struct TileMeta {
   uint32_t maxr2;
   uint16_t pref[16];
 };
+// (the fill reads count and maxr2 as one 8-byte word)
+static_assert(sizeof(TileMeta) == 40 && offsetof(TileMeta, maxr2) == 4, "TileMeta layout");
 
 constexpr int kBucketBits = 9;
 constexpr int kBuckets = 1 << kBucketBits;  // radius buckets of the counting sorts

@@ -276,6 +276,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ uint32_t s_start[kBuckets + 1];    // bucket starts over all rounds
   __shared__ uint16_t s_len[kNbr];              // useful list prefix of a tile in reach
   __shared__ uint32_t s_org[kNbr];              // its tile coordinates, packed
+  __shared__ uint32_t s_tile[kNbr];             // its tile index
   __shared__ uint16_t s_cmap[kCmap];            // chunk -> tile in reach
   __shared__ uint32_t s_off[kNbr];              // chunk offsets
   __shared__ uint32_t s_spos[kStage], s_sr2[kStage];  // candidates in list order
@@ -369,7 +370,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const int ux = (int)tx + ix - K, uy = (int)ty + iy - K, uz = (int)tz + iz - K;
     if (ux < 0 || uy < 0 || uz < 0 || ux >= (int)TX || uy >= (int)TY || uz >= (int)TZ) continue;
     const uint32_t u = ((uint32_t)uz * TY + (uint32_t)uy) * TX + (uint32_t)ux;
-    const uint32_t mcount = meta[u].count, mmax = meta[u].maxr2;
+    // count and maxr2 in one 8-byte load (TileMeta is 40 bytes, its first two
+    // fields adjacent)
+    const v2u32 cm = *(BM_GLOBAL const v2u32*)&meta[u];
+    const uint32_t mcount = cm.x, mmax = cm.y;
     if (mcount == 0) continue;
     const int ulx = ux * 8, uly = uy * 8, ulz = uz * 8;
     const int uhx = min(ulx + 7, (int)W - 1), uhy = min(uly + 7, (int)H - 1),
@@ -385,6 +389,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     if (slot < (uint32_t)kNbr) {
       s_len[slot] = (uint16_t)len;
       s_org[slot] = (uint32_t)ux | ((uint32_t)uy << 10) | ((uint32_t)uz << 20);
+      s_tile[slot] = u;
     } else {
       s_over = 1;
     }
@@ -439,14 +444,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           cur = lo;
         }
         const uint32_t e = (q - s_off[cur]) * 64 + lane;
-        const uint32_t o = s_org[cur], len = s_len[cur];
+        const uint32_t o = s_org[cur], len = s_len[cur], u = s_tile[cur];
         org[k] = o;
-        const uint32_t u = ((o >> 20) * TY + ((o >> 10) & 1023)) * TX + (o & 1023);
         // Every read here is unconditional (the list index clamped to the
-        // tile's 512-entry slot), so no branch separates the trip's chunks.  A
-        // chunk q >= nchunks has e >= len: its slot is the last chunk's, which
-        // ends at q - 1 < q.
-        const uint32_t v = lists[(size_t)u * 512 + min(e, 511u)];
+        // list's last useful entry; len >= 1 for a tile in reach), so no branch
+        // separates the trip's chunks.  A chunk q >= nchunks has e >= len: its
+        // slot is the last chunk's, which ends at q - 1 < q.
+        const uint32_t v = lists[(size_t)u * 512 + min(e, len - 1)];
         ent[k] = e < len ? v : 0u;
       }
 #pragma unroll
