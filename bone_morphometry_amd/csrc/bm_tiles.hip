@@ -207,6 +207,11 @@ __device__ __forceinline__ uint32_t pack_pos(int dx, int dy, int dz) {
 __device__ __forceinline__ uint32_t box_gap(int lo, int hi, int v) {
   return v < lo ? (uint32_t)(lo - v) : (v > hi ? (uint32_t)(v - hi) : 0u);
 }
+// Squared length of a gap vector.  Gaps are below 2^15 (axes are at most 16384
+// voxels), so 24-bit multiplies are exact and run at the full vector rate.
+__device__ __forceinline__ uint32_t gap2(uint32_t ex, uint32_t ey, uint32_t ez) {
+  return __umul24(ex, ex) + __umul24(ey, ey) + __umul24(ez, ez);
+}
 
 // Number of entries of a tile list that can reach a box at squared distance
 // bd2: those whose bucket key is at least that of kPrefR2(j), j the largest
@@ -214,8 +219,12 @@ __device__ __forceinline__ uint32_t box_gap(int lo, int hi, int v) {
 __device__ __forceinline__ uint32_t reach_len(BM_GLOBAL const TileMeta* __restrict__ m, uint32_t count,
                                               uint32_t bd2) {
   if (bd2 < 1) return count;
-  // (2j+1)^2 <= bd2  <=>  j <= (isqrt(bd2) - 1) / 2
-  const uint32_t j = min((isqrt_u32(bd2) - 1) >> 1, 15u);
+  // (2j+1)^2 <= bd2  <=>  j <= (isqrt(bd2) - 1) / 2; bd2 < 3 * 2^28 (gaps of
+  // at most 16384), so the root is below 2^15 and its checks fit 32 bits
+  uint32_t r = (uint32_t)sqrtf((float)bd2);
+  while (__umul24(r + 1, r + 1) <= bd2) ++r;
+  while (__umul24(r, r) > bd2) --r;
+  const uint32_t j = min((r - 1) >> 1, 15u);
   return m->pref[j];
 }
 
@@ -231,7 +240,7 @@ __device__ __forceinline__ bool entry_candidate(uint32_t ent, int ux, int uy, in
             cz = uz * 8 + (int)(l >> 6);
   const uint32_t ex = box_gap(ox, hx, cx), ey = box_gap(oy, hy, cy), ez = box_gap(oz, hz, cz);
   pos = pack_pos(cx - ox, cy - oy, cz - oz);
-  return ex * ex + ey * ey + ez * ez <= r2;
+  return gap2(ex, ey, ez) <= r2;
 }
 
 // Exclusive scan of a[0..n) in place by the block, n <= blockDim.x (one entry
@@ -381,7 +390,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const uint32_t gx = ulx > hx ? (uint32_t)(ulx - hx) : (ox > uhx ? (uint32_t)(ox - uhx) : 0u);
     const uint32_t gy = uly > hy ? (uint32_t)(uly - hy) : (oy > uhy ? (uint32_t)(oy - uhy) : 0u);
     const uint32_t gz = ulz > hz ? (uint32_t)(ulz - hz) : (oz > uhz ? (uint32_t)(oz - uhz) : 0u);
-    const uint32_t bd2 = gx * gx + gy * gy + gz * gz;
+    const uint32_t bd2 = gap2(gx, gy, gz);
     if (mmax < bd2) continue;
     const uint32_t len = reach_len(meta + u, mcount, bd2);
     if (len == 0) continue;
@@ -591,7 +600,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
                       cz = (int)(cp >> 20) - 512;
             const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
                       ez = max(max(bz0 - cz, cz - bz1), 0);
-            rel = (uint32_t)(ex * ex + ey * ey + ez * ez) <= cr;
+            rel = gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= cr;
             if (held != 0xFFFFFFFFu && (cr >> gshift) != held) rel = false;
           }
         }
