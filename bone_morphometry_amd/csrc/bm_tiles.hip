@@ -990,9 +990,9 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         for (int dy = -1; dy <= 1; ++dy) {
           const uint32_t b = 12u * (uint32_t)((rz + dz) * 12 + (ry + dy));  // row's first bit
           const uint32_t w = b >> 5;
-          const unsigned long long pair =
-              (unsigned long long)s_zb[w] | (unsigned long long)s_zb[min(w + 1, 53u)] << 32;
-          const uint32_t z = (uint32_t)(pair >> (b & 31)) & 0xFFFu;
+          // bits b .. b+11 of the two words (a 32-bit funnel shift)
+          const uint32_t z =
+              __builtin_amdgcn_alignbit(s_zb[min(w + 1, 53u)], s_zb[w], b & 31) & 0xFFFu;
           dil |= z | (z << 1) | (z >> 1);
         }
       // Also what each voxel of the row adds to the mean of a surface
@@ -1036,12 +1036,12 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
 #pragma unroll
           for (int k = 0; k < 13; ++k)
             c[k] = s_c[c10 + kNb26[g + k][0] + 10 * kNb26[g + k][1] + 100 * kNb26[g + k][2]];
+          // (non-contributors hold +0.0f and the sum starts at +0.0f, so adding
+          // them leaves the f32 sum bit for bit as skipping them would)
 #pragma unroll
           for (int k = 0; k < 13; ++k) {
-            if (c[k] > 0.0f) {
-              sum += c[k];
-              count += 1;
-            }
+            sum += c[k];
+            count += c[k] > 0.0f ? 1u : 0u;
           }
           __builtin_amdgcn_sched_barrier(0);
         }
