@@ -193,36 +193,84 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
 // The 1-D step of the Y and Z passes for one lane: d(q) = min_p (q-p)^2 + f(p)
 // for ROWS consecutive q per group, groups q0 = first, first + step, ... below
 // end.  At offset o the candidates of row k are q0+k-o and q0+k+o, a window
-// that slides by one row per offset, so two reads of f serve 2*ROWS
-// candidates.  The search stops once o^2 reaches every row's best, which is
-// exact: every later candidate adds at least o^2.  f(p) is kInf32 outside
-// [0, n); n bounds the offsets when no finite candidate exists.
+// that slides by one row per offset.  Offsets go in blocks of ROWS: a block
+// reads the ROWS new values on each side once, and every candidate of the
+// block is then a compile-time element of the previous or the new values, so
+// the window slides without register moves.  The search stops once o^2
+// reaches every row's best, which is exact: every later candidate adds at
+// least o^2.  The stop is taken when it holds for every lane of the
+// wavefront, so the offset and all window positions are the same across the
+// wavefront; a lane that could have stopped earlier only adds candidates that
+// cannot lower its minima.  f(p) is kInf32 outside [0, n) (kInf32 + o^2 stays
+// below 2^31 for every offset reached); n bounds the offsets when no finite
+// candidate exists.  fin(p) reads the staged tile without checks and is used
+// when a whole block's positions lie in [in_lo, in_hi).
 constexpr int kEdtRows = 8;
-template <int ROWS, class F, class Out>
+template <int ROWS, class F, class Fin, class Out>
 __device__ __forceinline__ void window_search(uint32_t first, uint32_t end, uint32_t step,
-                                              uint32_t n, F&& f, Out&& out) {
+                                              uint32_t n, int in_lo, int in_hi, F&& f, Fin&& fin,
+                                              Out&& out) {
   for (uint32_t q0 = first; q0 < end; q0 += step) {
-    uint32_t best[ROWS], lo[ROWS], up[ROWS];
+    const int iq = (int)q0;
+    uint32_t best[ROWS], lp[ROWS], up[ROWS];
+    if (__ballot(!(iq >= in_lo && iq + ROWS <= in_hi)) == 0) {
+#pragma unroll
+      for (int k = 0; k < ROWS; ++k) best[k] = fin(iq + k);
+    } else {
+#pragma unroll
+      for (int k = 0; k < ROWS; ++k) best[k] = f(iq + k);
+    }
     uint32_t bmax = 0;
 #pragma unroll
     for (int k = 0; k < ROWS; ++k) {
-      best[k] = lo[k] = up[k] = f((int)(q0 + k));
+      lp[k] = up[k] = best[k];  // f at q0 + k: the low side at o0 = 0, and the high side
       if (q0 + k < end) bmax = max(bmax, best[k]);
     }
-    for (uint32_t o = 1; o * o < bmax; ++o) {
-      if ((int)(q0 + ROWS - 1) < (int)o && q0 + o >= n) break;  // nothing left on either side
+    for (uint32_t o0 = 0;; o0 += ROWS) {  // offsets o0 + 1 .. o0 + ROWS
+      if (!__ballot((o0 + 1) * (o0 + 1) < bmax)) break;
+      // nothing left on either side of any row of any lane
+      if (!__ballot(!(iq + ROWS - 1 < (int)(o0 + 1) && q0 + o0 + 1 >= n))) break;
+      // ln[m] = f(q0 - o0 - ROWS + m), un[m] = f(q0 + o0 + ROWS + m); lp and up
+      // hold f(q0 - o0 + m) and f(q0 + o0 + m)
+      const int plo = iq - (int)o0 - ROWS, phi = iq + (int)o0 + ROWS;
+      uint32_t ln[ROWS], un[ROWS];
+      if (__ballot(!(plo >= in_lo && phi + ROWS <= in_hi)) == 0) {
 #pragma unroll
-      for (int k = ROWS - 1; k > 0; --k) lo[k] = lo[k - 1];
-      lo[0] = f((int)q0 - (int)o);
+        for (int m = 0; m < ROWS; ++m) {
+          ln[m] = fin(plo + m);
+          un[m] = fin(phi + m);
+        }
+      } else {
 #pragma unroll
-      for (int k = 0; k < ROWS - 1; ++k) up[k] = up[k + 1];
-      up[ROWS - 1] = f((int)(q0 + ROWS - 1 + o));
-      const uint32_t oo = o * o;
-      bmax = 0;
+        for (int m = 0; m < ROWS; ++m) {
+          ln[m] = f(plo + m);
+          un[m] = f(phi + m);
+        }
+      }
+      bool stop = false;
 #pragma unroll
-      for (int k = 0; k < ROWS; ++k) {
-        best[k] = min(best[k], min(lo[k], up[k]) + oo);
-        if (q0 + k < end) bmax = max(bmax, best[k]);
+      for (int j = 1; j <= ROWS; ++j) {
+        const uint32_t o = o0 + j, oo = o * o;
+        bmax = 0;
+#pragma unroll
+        for (int k = 0; k < ROWS; ++k) {
+          const int ci = ROWS + k - j;  // q0 + k - o = plo + ci
+          const int di = k + j;         // q0 + k + o = q0 + o0 + di
+          const uint32_t lo = ci < ROWS ? ln[ci] : lp[ci - ROWS];
+          const uint32_t hi = di < ROWS ? up[di] : un[di - ROWS];
+          best[k] = min(best[k], min(lo, hi) + oo);
+          if (q0 + k < end) bmax = max(bmax, best[k]);
+        }
+        if (j < ROWS && !__ballot((o + 1) * (o + 1) < bmax)) {
+          stop = true;
+          break;
+        }
+      }
+      if (stop) break;
+#pragma unroll
+      for (int m = 0; m < ROWS; ++m) {
+        lp[m] = ln[m];
+        up[m] = un[m];
       }
     }
 #pragma unroll
@@ -275,7 +323,8 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
     if (p < 0 || p >= (int)H) return kInf32;
     return sq16(col[(size_t)p * W]);
   };
-  window_search<kEdtRows>(y0 + kEdtRows * wv, y1, kEdtRows * nwv, H, fy,
+  auto fy_in = [&](int p) -> uint32_t { return sq16(ytile[(uint32_t)(p - (int)ya) * 64 + lane]); };
+  window_search<kEdtRows>(y0 + kEdtRows * wv, y1, kEdtRows * nwv, H, (int)ya, (int)yb, fy, fy_in,
                           [&](uint32_t q, uint32_t v) { out[sbase + (size_t)q * W + x] = v; });
 }
 
@@ -325,8 +374,9 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
       return col[(size_t)p * slice];
     };
     // the two half-waves take adjacent groups of rows
-    window_search<kEdtRows>(z0 + 2 * kEdtRows * wv + kEdtRows * zs, z1, 2 * kEdtRows * nwv, D, fz,
-                            [&](uint32_t q, uint32_t v) {
+    auto fz_in = [&](int p) -> uint32_t { return ztile[(uint32_t)(p - (int)za) * 32 + xl]; };
+    window_search<kEdtRows>(z0 + 2 * kEdtRows * wv + kEdtRows * zs, z1, 2 * kEdtRows * nwv, D,
+                            (int)za, (int)zb, fz, fz_in, [&](uint32_t q, uint32_t v) {
                               out[(size_t)q * slice + base + xl] = v;
                               mx = max(mx, v);
                             });
