@@ -102,11 +102,22 @@ k_top16_pick(const ItemDesc* __restrict__ items) {
 
 // One 8-bit digit (bits shift..shift+7) of the values sharing each rank's
 // prefix; ranks with equal prefixes share one histogram row.
+//
+// A value can share a prefix only if its top 16 bits name the top-16 bin of
+// one (a bin of the clean-up's histogram).  A table over those bins gives the
+// bin's group and the group's prefix bits below the top 16, so a value is
+// classified with one LDS read at an index clamped into the table; the reads
+// of a trip's 16 values have no branch between them.  A bin holding two
+// groups' prefixes (they then differ below bit 16) is flagged and its values
+// search the groups.
+constexpr uint32_t kNoGroup = 0xFFu, kMultiGroup = 0x100u;
 __global__ void __launch_bounds__(256)
 k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
              uint32_t nitems, int shift) {
   __shared__ uint32_t h[kMaxRanks][256];
   __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng;
+  // per top-16 bin: group (kNoGroup: none) | kMultiGroup | prefix bits 0..15 << 16
+  __shared__ uint32_t s_btab[kTopBins];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const SelectState* __restrict__ st = gbl(I.sel);
@@ -116,19 +127,30 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
   const uint32_t nr = st->nranks;
   if (nr == 0) return;
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) (&h[0][0])[i] = 0;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_btab[i] = kNoGroup;
   const uint32_t hi_mask = shift >= 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-  if (threadIdx.x == 0) {  // distinct prefixes
+  __syncthreads();
+  if (threadIdx.x == 0) {  // distinct prefixes, and the table of their top-16 bins
     uint32_t ng = 0;
     for (uint32_t r = 0; r < nr; ++r) {
+      const uint32_t pre = st->prefix[r];
       uint32_t g = 0;
-      while (g < ng && gpre[g] != st->prefix[r]) ++g;
-      if (g == ng) gpre[ng++] = st->prefix[r];
+      while (g < ng && gpre[g] != pre) ++g;
       grow[r] = g;
+      if (g < ng) continue;
+      gpre[ng++] = pre;
+      // (a rank k_top16_pick did not place -- values beyond the histogram, a
+      // case the statistics report as an error -- keeps prefix 0: no bin)
+      const uint32_t bin = (pre >> 16) - kTopBase;
+      if (bin >= (uint32_t)kTopBins) continue;
+      const uint32_t e = s_btab[bin];
+      s_btab[bin] = (e & 0xFFu) == kNoGroup ? (g | (pre << 16)) : (e | kMultiGroup);
     }
     s_ng = ng;
   }
   __syncthreads();
   const uint32_t ng = s_ng;
+  const uint32_t lo_mask = hi_mask & 0xFFFFu;  // prefix bits below the top 16
   // four consecutive values per 16-byte load (the map is 256-byte aligned),
   // and kLoads such loads per thread and trip, each coalesced across the
   // wave and all issued before the first is used
@@ -136,16 +158,6 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
   const uint64_t n4 = (n + 3) / 4;
   const uint64_t stride = (uint64_t)I.qblocks * blockDim.x;
   const uint64_t iters = (n4 + stride * kLoads - 1) / (stride * kLoads);  // uniform trip count
-  auto classify = [&](uint32_t b, uint32_t& digit) {
-    uint32_t g = ng;
-    digit = 0;
-    if (b != 0 && b < 0x7F800000u) {  // positive finite values only
-      digit = (b >> shift) & 0xFFu;
-      for (uint32_t k = 0; k < ng; ++k)
-        if ((b & hi_mask) == gpre[k]) g = k;
-    }
-    return g;
-  };
   for (uint64_t it = 0; it < iters; ++it) {
     uint32_t b[kLoads][4];
     const uint64_t trip0 = it * kLoads * stride + (uint64_t)vb * blockDim.x + threadIdx.x;
@@ -166,13 +178,30 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
           b[u][k] = 4 * i4 + k < n ? __float_as_uint(map[4 * i4 + k]) : 0u;
       }
     }
+    uint32_t t[kLoads][4];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        t[u][k] = s_btab[min((b[u][k] >> 16) - kTopBase, (uint32_t)kTopBins - 1)];
 #pragma unroll
     for (int u = 0; u < kLoads; ++u)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        uint32_t digit;
-        const uint32_t g = classify(b[u][k], digit);
-        wave_hist_add(&h[0][0], g < ng, g * 256 + digit);
+        const uint32_t v = b[u][k];
+        // positive finite values in the table's range only
+        const bool in = v != 0 && v < 0x7F800000u && (v >> 16) - kTopBase < (uint32_t)kTopBins;
+        const uint32_t e = t[u][k];
+        uint32_t g = in && (v & lo_mask) == (e >> 16) ? (e & 0xFFu) : kNoGroup;
+        const bool multi = in && (e & kMultiGroup);
+        if (__ballot(multi)) {
+          if (multi) {
+            g = kNoGroup;
+            for (uint32_t q = 0; q < ng; ++q)
+              if ((v & hi_mask) == gpre[q]) g = q;
+          }
+        }
+        wave_hist_add(&h[0][0], g < ng, (g < ng ? g : 0u) * 256 + ((v >> shift) & 0xFFu));
       }
   }
   __syncthreads();
