@@ -201,7 +201,8 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
               if ((v & hi_mask) == gpre[q]) g = q;
           }
         }
-        wave_hist_add(&h[0][0], g < ng, (g < ng ? g : 0u) * 256 + ((v >> shift) & 0xFFu));
+        // few lanes hold a candidate, so a plain LDS atomic per candidate
+        if (g < ng) atomicAdd(&h[g][(v >> shift) & 0xFFu], 1u);
       }
   }
   __syncthreads();
