@@ -158,3 +158,28 @@ def test_autotune_writes_cache_and_keeps_results(tmp_path):
     batch = bm.Batch(items).run()
     assert all(_same(a, b) for a, b in zip(batch, before))
     bm.tuning_load(tmp_path / "absent.json")  # back to the defaults
+
+
+@pytest.mark.parametrize("tile_order,fill_tiles,fill_cap", [(7, 1, 1024), (7, 2, 512), (2, 2, 1024),
+                                                            (4, 1, 512)])
+def test_tile_orders_and_fill_shapes_keep_results(tmp_path, tile_order, fill_tiles, fill_cap):
+    """The grouped (4x4x4) tile order of the ridge, fill and clean-up kernels,
+    two-tile fill blocks and the smaller fill capacity (more rounds) on
+    volumes whose sides are not multiples of the tile groups: every result
+    bit-identical to the default kernels'."""
+    vols = _volumes(23, 3, 40, 140)
+    bufs = _upload(vols)
+    items = _items(vols, bufs)
+    bm.tuning_load(tmp_path / "absent.json")  # the defaults
+    before = [_plain(it) for it in items]
+    classes = {f"h{(v['h'] + 63) // 64 * 64}_d{(v['d'] + 63) // 64 * 64}" for v in vols}
+    entry = {"y_rows": 128, "y_halo": 16, "z_rows": 128, "z_halo": 16, "fill_cap": fill_cap,
+             "tile_order": tile_order, "fill_tiles": fill_tiles}
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({"gfx950": {c: entry for c in classes}}))
+    bm.tuning_load(path)
+    try:
+        after = [_plain(it) for it in items]
+    finally:
+        bm.tuning_load(tmp_path / "absent.json")
+    assert all(_same(a, b) for a, b in zip(after, before))
