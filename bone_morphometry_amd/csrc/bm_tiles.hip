@@ -105,32 +105,28 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   const int c = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
   const uint32_t r2 = valid ? s_d2[c] : 0;
   const bool sel = r2 != 0;
-  bool target = sel;
-  if (valid && !sel) {
-    target = s_d2[c - 1] | s_d2[c + 1] | s_d2[c - 10] | s_d2[c + 10] | s_d2[c - 100] |
-             s_d2[c + 100];
-  }
-  bool ridge = sel;
-  if (sel) {
-    // A ridge unless a neighbour's squared distance reaches the template of
-    // its offset class (1: face, 2: edge, 3: corner neighbour), i.e. unless
-    // the largest value of a class reaches that class's template.  All 26
-    // neighbours are read at once (one LDS wait, not 26).
-    const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
-    uint32_t nb[27];
+  // The 26 neighbours (always inside the staged 10^3 block) and the templates
+  // are read by every lane, selected or not (r2 = 0 reads the tables' first
+  // entries), so no branch separates the reads.
+  const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
+  uint32_t nb[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
-      nb[q] = q == 13 ? 0u : s_d2[c + dx + 10 * dy + 100 * dz];
-    }
-    uint32_t mx[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const int k = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
-      mx[k] = max(mx[k], nb[q]);
-    }
-    ridge = !(mx[1] >= t1 || mx[2] >= t2 || mx[3] >= t3);
+  for (int q = 0; q < 27; ++q) {
+    const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+    nb[q] = q == 13 ? 0u : s_d2[c + dx + 10 * dy + 100 * dz];
   }
+  // fill target: selected, or in the image with a selected face neighbour
+  const bool target = sel || (valid && (nb[4] | nb[10] | nb[12] | nb[14] | nb[16] | nb[22]) != 0);
+  // A ridge unless a neighbour's squared distance reaches the template of its
+  // offset class (1: face, 2: edge, 3: corner neighbour), i.e. unless the
+  // largest value of a class reaches that class's template.
+  uint32_t mx[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int k = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
+    mx[k] = max(mx[k], nb[q]);
+  }
+  const bool ridge = sel && !(mx[1] >= t1 || mx[2] >= t2 || mx[3] >= t3);
   const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
   if (lane == 0) {
     masks[(size_t)t * 16 + (li >> 6)] = msel;
