@@ -105,7 +105,8 @@ Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
 
 // Kernel tunables for a volume: the autotuner's choice for its shape class,
 // with test overrides from the environment (BM_FILL_CAP: fill candidates per
-// round; BM_TILE_ORDER: tile order bits).
+// round; BM_FILL_TILES: tiles per fill block; BM_TILE_ORDER: tile order bits;
+// BM_FILL_DOT: 0 for the packed-position candidate test at every radius).
 static Tuning tuning_for(const Dims& dims) {
   Tuning t = tuned_for(dims);
   if (const char* env = std::getenv("BM_FILL_CAP")) {
@@ -115,6 +116,10 @@ static Tuning tuning_for(const Dims& dims) {
   if (const char* env = std::getenv("BM_FILL_TILES")) {  // tiles per fill block (1 or 2)
     const long v = std::strtol(env, nullptr, 10);
     if (v == 1 || v == 2) t.fill_tiles = (uint32_t)v;
+  }
+  if (const char* env = std::getenv("BM_FILL_DOT")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v == 0 || v == 1) t.fill_dot = (uint32_t)v;
   }
   if (const char* env = std::getenv("BM_TILE_ORDER")) {  // tile order bits (see Tuning)
     const long v = std::strtol(env, nullptr, 10);

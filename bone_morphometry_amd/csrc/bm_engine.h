@@ -120,6 +120,10 @@ struct Tuning {
   // threads per 16x8x8 pair sharing one candidate gathering).  One value per
   // plan: the first item's.
   uint32_t fill_tiles = 1;
+  // 1: the fill tests a candidate with one 4 x u8 dot product when the
+  // item's radii are at most 64; 0: the packed-position test at every radius
+  // (both forms give identical maps; 0 is a test override, not tuned).
+  uint32_t fill_dot = 1;
 };
 
 constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
@@ -137,6 +141,7 @@ constexpr uint32_t kItemThickness = 1u;  // run the local-thickness stages
 constexpr uint32_t kItemQuantiles = 2u;  // run the quantile selection (needs the map)
 constexpr uint32_t kItemDiag = 4u;       // fill work counters
 constexpr uint32_t kItemMap = 8u;        // the clean-up writes the thickness map
+constexpr uint32_t kItemPackedFill = 16u;  // fill: packed-position test at every radius
 
 struct TileMeta;
 

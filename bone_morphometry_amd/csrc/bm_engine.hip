@@ -109,6 +109,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   d.flags = (lt ? kItemThickness : 0u) | (quant ? kItemQuantiles : 0u) |
             (lt && s.diag ? kItemDiag : 0u) | (mapped ? kItemMap : 0u);
   const Tuning& t = s.tune;
+  if (lt && !t.fill_dot) d.flags |= kItemPackedFill;
   d.xblocks = grid_for((uint64_t)dims.h * dims.d * 64, 256, kEdtXBlocks);
   {
     const uint32_t H = dims.h;

@@ -421,6 +421,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   unsigned long long td = tc;
   const uint32_t gshift = bucket_shift(max_r2);
   const uint32_t gtop = max_r2 >> gshift;
+  const uint32_t kshift = gshift + 16;  // bucket of a key word
   // Chunks q = wv, wv + NW, ... of the flattened (neighbour, 64 entries)
   // range; one chunk is a sorted run of one list.  Read kTrip chunks per trip.
   constexpr int kTrip = 4;
@@ -508,6 +509,33 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   if (threadIdx.x == 0) s_start[kBuckets] = n;
   __syncthreads();
   const unsigned long long clk1 = diag ? wall_clock64() : 0ull;
+  // The sorted candidates are kept as a position word and a key word
+  // key = r^2 << 16 | (K & 0xFFFF), and a voxel's running maximum m is the
+  // largest key that covers it (r^2 <= 16384 < 2^16 in the gather, so keys
+  // order as their r^2 do and m >> 16 is the maximum r^2).  With R <= 64
+  // ("dot" form) the position word holds the block-relative centre c as
+  // bytes c + 64 (c lies in [-R, 8 * TXN - 1 + R]) and K = |c|^2 - r^2, so
+  // that for a voxel v of the block (coordinates below 16)
+  //   |v - c|^2 <= r^2  <=>  2 v.c - |v|^2 >= |c|^2 - r^2 = K
+  // and 2 v.c - |v|^2 = dot4(2v, c + 64) - (|v|^2 + 128 (vx + vy + vz)):
+  // one 4 x u8 dot product per test (|K| < 2^15 for R <= 64).  Otherwise the
+  // position word is the 10-bit packed one and K is 0.
+  const bool dotp = R <= 64 && !(I.flags & kItemPackedFill);
+  const uint32_t v2pk = (uint32_t)(2 * vx) | ((uint32_t)(2 * vy) << 8) | ((uint32_t)(2 * vz) << 16);
+  const uint32_t nega =
+      0u - (uint32_t)(vx * vx + vy * vy + vz * vz + 128 * (vx + vy + vz));
+  auto sorted_words = [&](uint32_t pos, uint32_t r2, uint32_t& pw, uint32_t& kw) {
+    if (dotp) {
+      const int cx = (int)(pos & 1023) - 512, cy = (int)((pos >> 10) & 1023) - 512,
+                cz = (int)(pos >> 20) - 512;
+      pw = (uint32_t)(cx + 64) | ((uint32_t)(cy + 64) << 8) | ((uint32_t)(cz + 64) << 16);
+      const int k = cx * cx + cy * cy + cz * cz - (int)r2;
+      kw = (r2 << 16) | ((uint32_t)k & 0xFFFFu);
+    } else {
+      pw = pos;
+      kw = r2 << 16;
+    }
+  };
   // Rounds over bucket ranges that fit in LDS, largest radii first.
   const bool target = (s_tgt[tgt_word] >> (li & 63)) & 1ull;
   uint32_t m = 0;
@@ -544,8 +572,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       const bool keep = keep0 && b >= b0 && b < b1;
       if (keep) {
         const uint32_t k = atomicAdd(&s_cnt[b], 1u) - base;
-        s_pos[k] = pos;
-        s_r2[k] = r2;
+        uint32_t pw, kw;
+        sorted_words(pos, r2, pw, kw);
+        s_pos[k] = pw;
+        s_r2[k] = kw;
       }
     };
     if (staged) staged_pass(place);
@@ -574,9 +604,9 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         // x + 4y + 16z of the sub-brick) from the wave's mask with scalar bit
         // operations; a candidate that misses it cannot change any result.
         const uint32_t first_key =
-            (uint32_t)__builtin_amdgcn_readfirstlane((int)(s_r2[c0] >> gshift));
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(s_r2[c0] >> kshift));
         const uint32_t cmax = ((first_key + 1u) << gshift) - 1u;
-        const unsigned long long open = __ballot(target && m < cmax);
+        const unsigned long long open = __ballot(target && (m >> 16) < cmax);
         bool rel = false;
         if (open) {
           uint32_t zm = 0, rows = 0;
@@ -592,33 +622,65 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           const int by0 = sy + __ffs(ym) - 1, by1 = sy + 31 - __clz(ym);
           const int bz0 = sz + __ffs(zm) - 1, bz1 = sz + 31 - __clz(zm);
           if (ci < cnt_round) {
-            const int cx = (int)(cp & 1023) - 512, cy = (int)((cp >> 10) & 1023) - 512,
-                      cz = (int)(cp >> 20) - 512;
+            int cx, cy, cz;
+            if (dotp) {
+              cx = (int)(cp & 255) - 64;
+              cy = (int)((cp >> 8) & 255) - 64;
+              cz = (int)((cp >> 16) & 255) - 64;
+            } else {
+              cx = (int)(cp & 1023) - 512;
+              cy = (int)((cp >> 10) & 1023) - 512;
+              cz = (int)(cp >> 20) - 512;
+            }
             const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
                       ez = max(max(bz0 - cz, cz - bz1), 0);
-            rel = gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= cr;
-            if (held != 0xFFFFFFFFu && (cr >> gshift) != held) rel = false;
+            rel = gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= (cr >> 16);
+            if (held != 0xFFFFFFFFu && (cr >> kshift) != held) rel = false;
           }
         }
         unsigned long long rm = __ballot(rel);
-        while (rm) {
-          const int j = __ffsll((unsigned long long)rm) - 1;
-          rm &= rm - 1;
-          const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)cp, j);
-          const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j);
-          const int dx = vx - ((int)(p & 1023) - 512), dy = vy - ((int)((p >> 10) & 1023) - 512),
-                    dz = vz - ((int)(p >> 20) - 512);
-          if ((uint32_t)(dx * dx + dy * dy + dz * dz) <= r2) m = max(m, r2);
-          ++tests;
-          if (rm) {  // a second candidate per trip
-            const int j2 = __ffsll((unsigned long long)rm) - 1;
+        if (dotp) {
+          // two candidates per trip; the test is one dot product and a compare
+          while (rm) {
+            const int j = __ffsll((unsigned long long)rm) - 1;
             rm &= rm - 1;
-            const uint32_t p2 = (uint32_t)__builtin_amdgcn_readlane((int)cp, j2);
-            const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j2);
-            const int ex2 = vx - ((int)(p2 & 1023) - 512), ey2 = vy - ((int)((p2 >> 10) & 1023) - 512),
-                      ez2 = vz - ((int)(p2 >> 20) - 512);
-            if ((uint32_t)(ex2 * ex2 + ey2 * ey2 + ez2 * ez2) <= q2) m = max(m, q2);
+            const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)cp, j);
+            const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)cr, j);
+            if ((int)__builtin_amdgcn_udot4(v2pk, pk, nega, false) >= (int)(int16_t)(key & 0xFFFFu))
+              m = max(m, key);
             ++tests;
+            if (rm) {
+              const int j2 = __ffsll((unsigned long long)rm) - 1;
+              rm &= rm - 1;
+              const uint32_t pk2 = (uint32_t)__builtin_amdgcn_readlane((int)cp, j2);
+              const uint32_t key2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j2);
+              if ((int)__builtin_amdgcn_udot4(v2pk, pk2, nega, false) >=
+                  (int)(int16_t)(key2 & 0xFFFFu))
+                m = max(m, key2);
+              ++tests;
+            }
+          }
+        } else {
+          while (rm) {
+            const int j = __ffsll((unsigned long long)rm) - 1;
+            rm &= rm - 1;
+            const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)cp, j);
+            const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)cr, j);
+            const int dx = vx - ((int)(p & 1023) - 512), dy = vy - ((int)((p >> 10) & 1023) - 512),
+                      dz = vz - ((int)(p >> 20) - 512);
+            if ((uint32_t)(dx * dx + dy * dy + dz * dz) <= (key >> 16)) m = max(m, key);
+            ++tests;
+            if (rm) {  // a second candidate per trip
+              const int j2 = __ffsll((unsigned long long)rm) - 1;
+              rm &= rm - 1;
+              const uint32_t p2 = (uint32_t)__builtin_amdgcn_readlane((int)cp, j2);
+              const uint32_t key2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j2);
+              const int ex2 = vx - ((int)(p2 & 1023) - 512),
+                        ey2 = vy - ((int)((p2 >> 10) & 1023) - 512),
+                        ez2 = vz - ((int)(p2 >> 20) - 512);
+              if ((uint32_t)(ex2 * ex2 + ey2 * ey2 + ez2 * ez2) <= (key2 >> 16)) m = max(m, key2);
+              ++tests;
+            }
           }
         }
         wait = __ballot(target && m == 0);
@@ -632,10 +694,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           break;
         }
         const uint32_t last = min(c0 + 63, cnt_round - 1);
-        const uint32_t lastb = s_r2[last] >> gshift;
+        const uint32_t lastb = s_r2[last] >> kshift;
         if (held == 0xFFFFFFFFu && wait == 0) held = lastb;
         if (held != 0xFFFFFFFFu) {
-          const bool more = last + 1 < cnt_round && (s_r2[last + 1] >> gshift) == held;
+          const bool more = last + 1 < cnt_round && (s_r2[last + 1] >> kshift) == held;
           if (!more) {
             done = true;
             break;
@@ -652,7 +714,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     b0 = b1;
     __syncthreads();
   }
-  if (valid) M[gidx] = m;
+  if (valid) M[gidx] = m >> 16;
   // Work counters and phase clocks (diagnostics only): plain stores into this
   // block's slot, summed afterwards by k_reduce_diag.
   if (diag && lane == 0) {

@@ -153,6 +153,52 @@ def test_fill_mixed_radius_buckets_match_oracle(sites, seed):
         np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
 
+@pytest.mark.parametrize("tiles", ["1", "2"])
+@pytest.mark.parametrize("radius", [64, 65])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fill_radius_limit_of_dot_form(axis, radius, tiles, monkeypatch):
+    """Two planes of sites 2R apart along one axis: the largest radius is
+    exactly R.  R = 64 is the largest radius of the fill's dot-product test
+    (candidate offsets from the block origin span -64 .. 8 * tiles - 1 + 64);
+    R = 65 takes the packed-position test.  One- and two-tile fill blocks."""
+    monkeypatch.setenv("BM_FILL_TILES", tiles)
+    dims = [11, 10, 9]  # (w, h, d)
+    dims[axis] = 2 * radius + 1
+    w, h, d = dims
+    a = np.ones((d, h, w), np.uint8)
+    ends = [slice(None)] * 3
+    for pos in (0, 2 * radius):
+        ends[2 - axis] = pos
+        a[tuple(ends)] = 0
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    assert int(ref["squared_edt"].max()) == radius * radius
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
+@pytest.mark.parametrize("tiles", ["1", "2"])
+@pytest.mark.parametrize("case", ["structure", "sparse_sites"])
+def test_fill_packed_position_form_matches_oracle(case, tiles, monkeypatch):
+    """The packed-position candidate test (taken above radius 64) forced at
+    small radii, on the volumes the dot-product test normally serves: both
+    forms give the oracle's map."""
+    monkeypatch.setenv("BM_FILL_DOT", "0")
+    monkeypatch.setenv("BM_FILL_TILES", tiles)
+    if case == "structure":
+        w, h, d = 40, 36, 44
+        a = _structure(5, (d, h, w), 0.75)
+    else:
+        rng = np.random.default_rng(23)
+        w, h, d = 72, 64, 56
+        a = np.ones(w * h * d, np.uint8)
+        a[rng.choice(a.size, 30, replace=False)] = 0
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
 def test_analyze_large_radius_volume_matches_oracle():
     """A volume of the batched workload whose marrow distances exceed 511."""
     p = workloads.batched_params()[29]
