@@ -195,14 +195,8 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Candidate position relative to the tile origin, biased by 512 per axis.
-__device__ __forceinline__ uint32_t pack_pos(int dx, int dy, int dz) {
-  return (uint32_t)(dx + 512) | ((uint32_t)(dy + 512) << 10) | ((uint32_t)(dz + 512) << 20);
-}
-
-__device__ __forceinline__ uint32_t box_gap(int lo, int hi, int v) {
-  return v < lo ? (uint32_t)(lo - v) : (v > hi ? (uint32_t)(v - hi) : 0u);
-}
+// A candidate's packed position: its offset from the block origin plus 512 in
+// bits 0-9 (x), 10-19 (y) and 20-29 (z).
 // Squared length of a gap vector.  Gaps are below 2^15 (axes are at most 16384
 // voxels), so 24-bit multiplies are exact and run at the full vector rate.
 __device__ __forceinline__ uint32_t gap2(uint32_t ex, uint32_t ey, uint32_t ez) {
@@ -222,21 +216,6 @@ __device__ __forceinline__ uint32_t reach_len(BM_GLOBAL const TileMeta* __restri
   while (__umul24(r, r) > bd2) --r;
   const uint32_t j = min((r - 1) >> 1, 15u);
   return m->pref[j];
-}
-
-// Candidate of a list entry of tile (ux, uy, uz) (tile units) for the tile box
-// [ox, hx] x [oy, hy] x [oz, hz]: position relative to (ox, oy, oz) and r^2;
-// returns false when the ball misses the box.
-__device__ __forceinline__ bool entry_candidate(uint32_t ent, int ux, int uy, int uz, int ox,
-                                                int oy, int oz, int hx, int hy, int hz,
-                                                uint32_t& pos, uint32_t& r2) {
-  r2 = ent >> 9;
-  const uint32_t l = ent & 511;
-  const int cx = ux * 8 + (int)(l & 7), cy = uy * 8 + (int)((l >> 3) & 7),
-            cz = uz * 8 + (int)(l >> 6);
-  const uint32_t ex = box_gap(ox, hx, cx), ey = box_gap(oy, hy, cy), ez = box_gap(oz, hz, cz);
-  pos = pack_pos(cx - ox, cy - oy, cz - oz);
-  return gap2(ex, ey, ez) <= r2;
 }
 
 // Exclusive scan of a[0..n) in place by the block, n <= blockDim.x (one entry
@@ -279,9 +258,9 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ uint32_t s_pos[kCap], s_r2[kCap];  // one round, sorted by bucket
   __shared__ uint32_t s_cnt[kBuckets];          // histogram, then cursors
   __shared__ uint32_t s_start[kBuckets + 1];    // bucket starts over all rounds
-  __shared__ uint16_t s_len[kNbr];              // useful list prefix of a tile in reach
-  __shared__ uint32_t s_org[kNbr];              // its tile coordinates, packed
-  __shared__ uint32_t s_tile[kNbr];             // its tile index
+  // a tile in reach: its index, and its offset (ix, iy, iz) in the reach box
+  // (6 bits each) with its useful list prefix length - 1 above bit 18
+  __shared__ uint32_t s_nb_tile[kNbr], s_nb_word[kNbr];
   __shared__ uint16_t s_cmap[kCmap];            // chunk -> tile in reach
   __shared__ uint32_t s_off[kNbr];              // chunk offsets
   __shared__ uint32_t s_spos[kStage], s_sr2[kStage];  // candidates in list order
@@ -392,9 +371,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     if (len == 0) continue;
     const uint32_t slot = atomicAdd(&s_nn, 1u);
     if (slot < (uint32_t)kNbr) {
-      s_len[slot] = (uint16_t)len;
-      s_org[slot] = (uint32_t)ux | ((uint32_t)uy << 10) | ((uint32_t)uz << 20);
-      s_tile[slot] = u;
+      s_nb_tile[slot] = u;
+      s_nb_word[slot] = (uint32_t)ix | ((uint32_t)iy << 6) | ((uint32_t)iz << 12) | ((len - 1) << 18);
     } else {
       s_over = 1;
     }
@@ -409,11 +387,11 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     return;
   }
   const uint32_t nn = s_nn;
-  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = (s_len[j] + 63) >> 6;
+  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = ((s_nb_word[j] >> 18) + 64) >> 6;
   __syncthreads();
   const uint32_t nchunks = block_exclusive_scan_n(s_off, nn, s_tmp);  // s_off: chunk offsets
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
-    const uint32_t c0 = s_off[j], c1 = min(c0 + ((s_len[j] + 63u) >> 6), (uint32_t)kCmap);
+    const uint32_t c0 = s_off[j], c1 = min(c0 + (((s_nb_word[j] >> 18) + 64u) >> 6), (uint32_t)kCmap);
     for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = (uint16_t)j;
   }
   __syncthreads();
@@ -430,10 +408,16 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // their list loads are all issued together; otherwise a chunk past the map
   // finds its slot by a binary search over the offsets.
   const bool mapped = nchunks <= (uint32_t)kCmap;
+  // A list entry of a tile in reach is a candidate if its ball meets the
+  // block box.  Positions relative to the block origin, offset by 512 (the
+  // packed-position fields): p = (ix - K) * 8 + 512 + the entry's local
+  // coordinate, with the tile's part (ix - K) * 8 + 512 uniform over a chunk;
+  // the gap to the box [0, w] is max(512 - p, p - (512 + w), 0).
+  const int wbx = hx - ox + 512, wby = hy - oy + 512, wbz = hz - oz + 512;
   auto chunk_trips = [&](auto mapped_tag, auto&& use) {
     constexpr bool kMapped = decltype(mapped_tag)::value;
     for (uint32_t q0 = wv; q0 < nchunks; q0 += kTrip * NW) {
-      uint32_t ent[kTrip], org[kTrip];
+      uint32_t ent[kTrip], nbw[kTrip];
 #pragma unroll
       for (int k = 0; k < kTrip; ++k) {
         const uint32_t q = q0 + (uint32_t)NW * k;
@@ -449,23 +433,34 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           }
           cur = lo;
         }
-        const uint32_t e = (q - s_off[cur]) * 64 + lane;
-        const uint32_t o = s_org[cur], len = s_len[cur], u = s_tile[cur];
-        org[k] = o;
+        // the chunk's values are uniform over the wave: scalar registers
+        cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)cur);
+        const uint32_t off = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_off[cur]);
+        const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_tile[cur]);
+        const uint32_t w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_word[cur]);
+        const uint32_t last = w1 >> 18;  // useful prefix length - 1
+        const uint32_t e = (q - off) * 64 + lane;
+        nbw[k] = w1;
         // Every read here is unconditional (the list index clamped to the
-        // list's last useful entry; len >= 1 for a tile in reach), so no branch
-        // separates the trip's chunks.  A chunk q >= nchunks has e >= len: its
-        // slot is the last chunk's, which ends at q - 1 < q.
-        const uint32_t v = lists[(size_t)u * 512 + min(e, len - 1)];
-        ent[k] = e < len ? v : 0u;
+        // list's last useful entry), so no branch separates the trip's
+        // chunks.  A chunk q >= nchunks has e > last: its slot is the last
+        // chunk's, which ends at q - 1 < q.
+        const uint32_t v = (lists + (size_t)u * 512)[min(e, last)];
+        ent[k] = e <= last ? v : 0u;
       }
 #pragma unroll
       for (int k = 0; k < kTrip; ++k) {
         if (q0 + (uint32_t)NW * k >= nchunks) break;  // uniform over the wave
-        uint32_t pos = 0, r2 = 0;
-        const bool keep = ent[k] != 0 &&
-                          entry_candidate(ent[k], (int)(org[k] & 1023), (int)((org[k] >> 10) & 1023),
-                                          (int)(org[k] >> 20), ox, oy, oz, hx, hy, hz, pos, r2);
+        const uint32_t w1 = nbw[k], en = ent[k];
+        const int bx = ((int)(w1 & 63) - K) * 8 + 512, by = ((int)((w1 >> 6) & 63) - K) * 8 + 512,
+                  bz = ((int)((w1 >> 12) & 63) - K) * 8 + 512;
+        const int px = bx + (int)(en & 7), py = by + (int)((en >> 3) & 7), pz = bz + (int)((en >> 6) & 7);
+        const int ex = max(max(512 - px, px - wbx), 0), ey = max(max(512 - py, py - wby), 0),
+                  ez = max(max(512 - pz, pz - wbz), 0);
+        const uint32_t r2 = en >> 9;
+        // (no short circuit: every lane computes the gap, no branch around it)
+        const bool keep = (en != 0) & (gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= r2);
+        const uint32_t pos = (uint32_t)px | ((uint32_t)py << 10) | ((uint32_t)pz << 20);
         use(keep, pos, r2);
       }
     }

@@ -199,6 +199,22 @@ def test_fill_packed_position_form_matches_oracle(case, tiles, monkeypatch):
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fill_axis_beyond_1024_tiles_matches_oracle(axis):
+    """An axis of 8216 voxels (1027 tiles): tile coordinates past 10 bits in
+    the fill's gathering of the tiles in reach."""
+    dims = [6, 5, 7]
+    dims[axis] = 8216
+    w, h, d = dims
+    rng = np.random.default_rng(40 + axis)
+    a = (rng.random(w * h * d) > 0.04).astype(np.uint8)
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
 def test_analyze_large_radius_volume_matches_oracle():
     """A volume of the batched workload whose marrow distances exceed 511."""
     p = workloads.batched_params()[29]
