@@ -189,26 +189,6 @@ __host__ __device__ inline uint32_t tile_positions(uint32_t TX, uint32_t TY, uin
                                                    bool grouped) {
   return grouped ? ((TX + 3) / 4) * ((TY + 3) / 4) * ((TZ + 3) / 4) * 64 : TX * TY * TZ;
 }
-// Coordinates of the tile at position j of that order, with two divisions;
-// false when the position holds no tile (kNoTile).
-__device__ __forceinline__ bool tile_xyz(uint32_t j, uint32_t TX, uint32_t TY, uint32_t TZ,
-                                         bool grouped, uint32_t& tx, uint32_t& ty, uint32_t& tz) {
-  if (!grouped) {
-    const uint32_t q = j / TX, r = q / TY;
-    tx = j - q * TX;
-    ty = q - r * TY;
-    tz = r;
-    return tz < TZ;
-  }
-  const uint32_t SX = (TX + 3) >> 2, SY = (TY + 3) >> 2;
-  const uint32_t s = j >> 6, l = j & 63;
-  const uint32_t q = s / SX, r = q / SY;
-  tx = (s - q * SX) * 4 + (l & 3);
-  ty = (q - r * SY) * 4 + ((l >> 2) & 3);
-  tz = r * 4 + (l >> 4);
-  return tx < TX && ty < TY && tz < TZ;
-}
-
 // Exact quotient n / d of 32-bit unsigned numbers for a divisor d >= 1 fixed
 // in advance: one high multiply and two shifts (Granlund and Montgomery,
 // "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1).
@@ -221,13 +201,16 @@ struct FastDiv {
     s1 = l > 0 ? 1u : 0u;
     s2 = l > 0 ? l - 1 : 0u;
   }
+  __host__ __device__ FastDiv(uint32_t d_, uint32_t m_, uint32_t s1_, uint32_t s2_)
+      : d(d_), m(m_), s1(s1_), s2(s2_) {}
   __host__ __device__ uint32_t div(uint32_t n) const {
     const uint32_t t = (uint32_t)(((unsigned long long)m * n) >> 32);
     return (t + ((n - t) >> s1)) >> s2;
   }
 };
 
-// tile_xyz without runtime divisions (for loops over many positions).
+// Coordinates of the tile at position j of that order, with invariant-divisor
+// quotients; at() is false when the position holds no tile (kNoTile).
 struct TileWalk {
   uint32_t TX, TY, TZ;
   bool grouped;
@@ -235,6 +218,9 @@ struct TileWalk {
   __device__ TileWalk(uint32_t tx_, uint32_t ty_, uint32_t tz_, bool g)
       : TX(tx_), TY(ty_), TZ(tz_), grouped(g), fx(g ? (tx_ + 3) >> 2 : tx_),
         fy(g ? (ty_ + 3) >> 2 : ty_) {}
+  // with the divisor constants computed in advance (fx: TX or SX, fy: TY or SY)
+  __device__ TileWalk(uint32_t tx_, uint32_t ty_, uint32_t tz_, bool g, FastDiv fx_, FastDiv fy_)
+      : TX(tx_), TY(ty_), TZ(tz_), grouped(g), fx(fx_), fy(fy_) {}
   __device__ bool at(uint32_t j, uint32_t& tx, uint32_t& ty, uint32_t& tz) const {
     if (!grouped) {
       const uint32_t q = fx.div(j), r = fy.div(q);

@@ -178,6 +178,9 @@ struct ItemDesc {
   uint32_t zgx, zgy, zgz, zchunk, zhalo;
   uint32_t lblocks, fill_cap, cblocks, qblocks;
   uint32_t tile_order;  // Tuning::tile_order
+  // invariant-divisor constants (d, m, s1, s2) of TX, TY, (TX + 3) / 4 and
+  // (TY + 3) / 4 for the tile kernels' position -> tile mapping
+  uint32_t divs[4][4];
   float scale;
   int lsb;
 };

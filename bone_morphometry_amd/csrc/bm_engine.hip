@@ -132,6 +132,16 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   d.lblocks = lt ? std::min<uint32_t>(std::max<uint32_t>((d.bound + 4) / 4, 1), 128) : 0;
   d.fill_cap = t.fill_cap ? std::min<uint32_t>(t.fill_cap, kFillCap) : kFillCap;
   d.tile_order = t.tile_order & 7u;
+  {
+    const uint32_t dv[4] = {d.TX, d.TY, (d.TX + 3) / 4, (d.TY + 3) / 4};
+    for (int i = 0; i < 4; ++i) {
+      const FastDiv f(dv[i]);
+      d.divs[i][0] = f.d;
+      d.divs[i][1] = f.m;
+      d.divs[i][2] = f.s1;
+      d.divs[i][3] = f.s2;
+    }
+  }
   d.cblocks = lt ? std::min<uint32_t>(std::max<uint32_t>(round8((d.ntiles + 7) / 8), 8),
                                       kStatsBlocks)
                  : 0;

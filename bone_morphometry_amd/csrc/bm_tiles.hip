@@ -45,6 +45,15 @@ namespace bm {
 
 __device__ __forceinline__ uint32_t bitlen(uint32_t v) { return v ? 32 - __clz(v) : 0; }
 
+// Position -> tile mapping of an item in the given tile order, with the
+// item's divisor constants (no runtime division).
+__device__ __forceinline__ TileWalk walk_of(const ItemDesc& I, bool grouped) {
+  const uint32_t* a = I.divs[grouped ? 2 : 0];
+  const uint32_t* b = I.divs[grouped ? 3 : 1];
+  return TileWalk(I.TX, I.TY, I.TZ, grouped, FastDiv(a[0], a[1], a[2], a[3]),
+                  FastDiv(b[0], b[1], b[2], b[3]));
+}
+
 // Shift so that (max_r2 >> shift) < kBuckets.
 __device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
   const uint32_t b = bitlen(max_r2);
@@ -74,7 +83,7 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   const uint32_t j = xcd_tile(vb, npos);
   if (j >= npos) return;
   uint32_t tx, ty, tz;
-  if (!tile_xyz(j, TX, TY, TZ, grouped, tx, ty, tz)) return;
+  if (!walk_of(I, grouped).at(j, tx, ty, tz)) return;
   const uint32_t t = (tz * TY + ty) * TX + tx;
   const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
   // stage the 10^3 block (tile + 1 halo): both of a thread's loads in flight
@@ -218,27 +227,32 @@ __device__ __forceinline__ uint32_t reach_len(BM_GLOBAL const TileMeta* __restri
   return m->pref[j];
 }
 
-// Exclusive scan of a[0..n) in place by the block, n <= blockDim.x (one entry
-// per thread); returns the total.
-__device__ uint32_t block_exclusive_scan_n(uint32_t* a, uint32_t n, uint32_t* s_tmp) {
-  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
-  const uint32_t v = t < n ? a[t] : 0u;
-  uint32_t x = v;
+// Exclusive scan by one wavefront of val(0 .. n-1), n <= 64 * kPer, kPer
+// consecutive values per lane: out(j, prefix) receives each exclusive prefix;
+// returns the total (in every lane).  No barrier: the other wavefronts wait
+// at the caller's next one.
+template <int kPer, typename Val, typename Out>
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t n, uint32_t lane, Val val, Out out) {
+  uint32_t local[kPer], sum = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const uint32_t j = lane * kPer + (uint32_t)i;
+    local[i] = j < n ? val(j) : 0u;
+    sum += local[i];
+  }
+  uint32_t x = sum;
   for (int off = 1; off < 64; off <<= 1) {
     const uint32_t y = __shfl_up(x, off);
     if (lane >= (uint32_t)off) x += y;
   }
-  __syncthreads();
-  if (lane == 63) s_tmp[wv] = x;
-  __syncthreads();
-  uint32_t wave_off = 0, total = 0;
-  for (uint32_t i = 0; i < nw; ++i) {
-    if (i < wv) wave_off += s_tmp[i];
-    total += s_tmp[i];
+  uint32_t run = x - sum;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const uint32_t j = lane * kPer + (uint32_t)i;
+    if (j < n) out(j, run);
+    run += local[i];
   }
-  if (t < n) a[t] = wave_off + x - v;
-  __syncthreads();
-  return total;
+  return (uint32_t)__shfl((int)x, 63);
 }
 
 // One block per TXN x-adjacent tiles (a 8*TXN x 8 x 8 target box, 512*TXN
@@ -264,8 +278,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ uint16_t s_cmap[kCmap];            // chunk -> tile in reach
   __shared__ uint32_t s_off[kNbr];              // chunk offsets
   __shared__ uint32_t s_spos[kStage], s_sr2[kStage];  // candidates in list order
-  __shared__ uint32_t s_tmp[NW];
-  __shared__ uint32_t s_nn, s_over, s_pending, s_nst;
+  __shared__ uint32_t s_nn, s_over, s_pending, s_nst, s_nchunks;
   __shared__ unsigned long long s_tgt[8 * TXN];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
@@ -288,7 +301,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
     const uint32_t jpos = xcd_tile(vb, npos);
     if (jpos >= npos) return;
-    if (!tile_xyz(jpos, TX, TY, TZ, grouped, tx, ty, tz)) return;
+    if (!walk_of(I, grouped).at(jpos, tx, ty, tz)) return;
     t = (tz * TY + ty) * TX + tx;
   } else {
     const uint32_t TG = (TX + TXN - 1) / TXN;  // groups of TXN tiles along x
@@ -387,9 +400,16 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     return;
   }
   const uint32_t nn = s_nn;
-  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) s_off[j] = ((s_nb_word[j] >> 18) + 64) >> 6;
+  // chunk offsets of the tiles in reach (64-entry chunks of their prefixes),
+  // scanned by one wavefront
+  if (wv == 0) {
+    const uint32_t tot = wave_exclusive_scan<kNbr / 64>(
+        nn, lane, [&](uint32_t j) { return ((s_nb_word[j] >> 18) + 64) >> 6; },
+        [&](uint32_t j, uint32_t p) { s_off[j] = p; });
+    if (lane == 0) s_nchunks = tot;
+  }
   __syncthreads();
-  const uint32_t nchunks = block_exclusive_scan_n(s_off, nn, s_tmp);  // s_off: chunk offsets
+  const uint32_t nchunks = s_nchunks;
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
     const uint32_t c0 = s_off[j], c1 = min(c0 + (((s_nb_word[j] >> 18) + 64u) >> 6), (uint32_t)kCmap);
     for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = (uint16_t)j;
@@ -499,10 +519,18 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       use(keep, keep ? s_spos[i] : 0u, keep ? s_sr2[i] : 0u);
     }
   };
-  const uint32_t n = block_exclusive_scan_n(s_cnt, kBuckets, s_tmp);
-  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_start[i] = s_cnt[i];
-  if (threadIdx.x == 0) s_start[kBuckets] = n;
+  // bucket starts (and the placement cursors), scanned by one wavefront
+  if (wv == 0) {
+    const uint32_t tot = wave_exclusive_scan<kBuckets / 64>(
+        (uint32_t)kBuckets, lane, [&](uint32_t j) { return s_cnt[j]; },
+        [&](uint32_t j, uint32_t p) {
+          s_cnt[j] = p;
+          s_start[j] = p;
+        });
+    if (lane == 0) s_start[kBuckets] = tot;
+  }
   __syncthreads();
+  const uint32_t n = s_start[kBuckets];
   const unsigned long long clk1 = diag ? wall_clock64() : 0ull;
   // The sorted candidates are kept as a position word and a key word
   // key = r^2 << 16 | (K & 0xFFFF), and a voxel's running maximum m is the
@@ -984,7 +1012,7 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const uint32_t per = (npos + kXcds - 1) / kXcds;
   const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, npos);
   const uint32_t jstep = I.cblocks / kXcds;
-  const TileWalk walk(TX, I.TY, I.TZ, grouped);
+  const TileWalk walk = walk_of(I, grouped);
   // next position at or after j that holds a tile (uniform over the block)
   auto next_tile = [&](uint32_t j, uint32_t& tx, uint32_t& ty, uint32_t& tz) {
     while (j < xend && !walk.at(j, tx, ty, tz)) j += jstep;
