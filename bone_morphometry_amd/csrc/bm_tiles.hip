@@ -933,7 +933,7 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
                 uint32_t nitems) {
   __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
   __shared__ uint32_t s_zb[54];           // bit i set where halo value i is 0 (1728 bits)
-  __shared__ uint32_t s_mark[10 * 10];    // per row of the 1-halo box: surface bits
+  __shared__ uint32_t s_dil[10 * 10];     // per row of the 1-halo box: zeros dilated by 3x3x3
   __shared__ float s_c[10 * 10 * 10];     // per voxel of the 1-halo box: its part of a mean
   __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
   __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
@@ -1057,14 +1057,16 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       continue;
     }
     __syncthreads();
-    if (threadIdx.x < 100) {
+    if (threadIdx.x < 500) {
       // surface (pass 1, src/local_thickness.rs:214-238): M != 0 with a zero
       // among the 26 neighbours; the neighbourhood test is a 3x3x3 dilation of
-      // the zero bits (out-of-array voxels hold kOut, never zero).  One thread
-      // per row (rz, ry) of the 1-halo box.  (The empty asm keeps the
-      // row's addresses from being hoisted out of the tile loop and spilled.)
-      uint32_t row = threadIdx.x;
-      asm volatile("" : "+v"(row));
+      // the zero bits (out-of-array voxels hold kOut, never zero).  Five
+      // threads per row (rz, ry) of the 1-halo box, two voxels each; each
+      // dilates its row.  (The empty asm keeps the row's addresses from being
+      // hoisted out of the tile loop and spilled.)
+      uint32_t t5 = threadIdx.x;
+      asm volatile("" : "+v"(t5));
+      const uint32_t row = t5 / 5, x0 = 2 * (t5 - 5 * row) + 1;
       const int ry = (int)(row % 10) + 1, rz = (int)(row / 10) + 1;
       uint32_t dil = 0;
       for (int dz = -1; dz <= 1; ++dz)
@@ -1076,21 +1078,20 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
               __builtin_amdgcn_alignbit(s_zb[min(w + 1, 53u)], s_zb[w], b & 31) & 0xFFFu;
           dil |= z | (z << 1) | (z >> 1);
         }
+      if (x0 == 1) s_dil[row] = dil;
       // Also what each voxel of the row adds to the mean of a surface
       // neighbour (pass 2, src/local_thickness.rs:243-277): its diameter when
       // it is positive, inside the array and not on the surface, else 0
       // (diameters of positive r^2 are positive).  Tabled once per tile, so
       // the 26 reads of a mean below are independent of each other.
-      uint32_t mark = 0;
-      for (int x = 1; x <= 10; ++x) {
+#pragma unroll
+      for (uint32_t k = 0; k < 2; ++k) {
+        const uint32_t x = x0 + k;
         const uint32_t v = s_m[(rz * 12 + ry) * 12 + x];
         const bool nonzero = v != 0 && v != kOut;
-        const bool surface = nonzero && ((dil >> x) & 1u);
-        mark |= (surface ? 1u : 0u) << (x - 1);  // bit hx (0..9) of the 1-halo box
-        const bool part = nonzero && !surface;
+        const bool part = nonzero && !((dil >> x) & 1u);
         s_c[row * 10 + (x - 1)] = part ? dia(part ? v : 0u) : 0.0f;
       }
-      s_mark[row] = mark;
     }
     __syncthreads();
     const bool sel = (selw >> (li & 63)) & 1ull;
@@ -1098,10 +1099,9 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     if (sel) {
       const int c12 = (lx + 2) + 12 * (ly + 2) + 144 * (lz + 2);
       const uint32_t m = s_m[c12];
-      auto marked = [&](int dx, int dy, int dz) {
-        return (s_mark[(lz + 1 + dz) * 10 + (ly + 1 + dy)] >> (lx + 1 + dx)) & 1u;
-      };
-      if (!marked(0, 0, 0)) {
+      // on the surface: nonzero with a zero among the 26 neighbours
+      const bool surface = m != 0 && ((s_dil[(lz + 1) * 10 + (ly + 1)] >> (lx + 2)) & 1u);
+      if (!surface) {
         tv = dia(m);
       } else {
         // pass 2 (src/local_thickness.rs:243-277): f32 mean of the positive,
