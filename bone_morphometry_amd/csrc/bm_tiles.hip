@@ -663,25 +663,32 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         }
         unsigned long long rm = __ballot(rel);
         if (dotp) {
-          // two candidates per trip; the test is one dot product and a compare
-          while (rm) {
-            const int j = __ffsll((unsigned long long)rm) - 1;
-            rm &= rm - 1;
-            const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)cp, j);
-            const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)cr, j);
-            if ((int)__builtin_amdgcn_udot4(v2pk, pk, nega, false) >= (int)(int16_t)(key & 0xFFFFu))
-              m = max(m, key);
-            ++tests;
-            if (rm) {
-              const int j2 = __ffsll((unsigned long long)rm) - 1;
-              rm &= rm - 1;
-              const uint32_t pk2 = (uint32_t)__builtin_amdgcn_readlane((int)cp, j2);
-              const uint32_t key2 = (uint32_t)__builtin_amdgcn_readlane((int)cr, j2);
-              if ((int)__builtin_amdgcn_udot4(v2pk, pk2, nega, false) >=
-                  (int)(int16_t)(key2 & 0xFFFFu))
-                m = max(m, key2);
-              ++tests;
+          if (rm) {
+            // Compact the chunk's relevant candidates into lanes 0 .. nrel-1
+            // with a forward permute (a bijection: the other lanes go after
+            // them and carry a key whose K = 0x7FFF no voxel reaches, since
+            // 2 v.c - |v|^2 < 2^15), then test them four per trip with
+            // uniform lane indices (at most 63: the last trip starts at or
+            // below 60).  The test is one dot product and a compare.
+            const uint32_t nrel = (uint32_t)__popcll(rm);
+            const uint32_t below = (uint32_t)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(rm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
+            const uint32_t dst = rel ? below : nrel + (lane - below);
+            const uint32_t pk_c =
+                (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), rel ? (int)cp : 0);
+            const uint32_t key_c =
+                (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), rel ? (int)cr : 0x7FFF);
+            for (uint32_t i = 0; i < nrel; i += 4) {
+#pragma unroll
+              for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)pk_c, (int)(i + u));
+                const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)key_c, (int)(i + u));
+                if ((int)__builtin_amdgcn_udot4(v2pk, pk, nega, false) >=
+                    (int)(int16_t)(key & 0xFFFFu))
+                  m = max(m, key);
+              }
             }
+            tests += nrel;
           }
         } else {
           while (rm) {
