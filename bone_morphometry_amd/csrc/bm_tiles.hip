@@ -632,18 +632,14 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         const unsigned long long open = __ballot(target && (m >> 16) < cmax);
         bool rel = false;
         if (open) {
-          uint32_t zm = 0, rows = 0;
-          for (int k = 0; k < 4; ++k) {
-            const uint32_t g = (uint32_t)(open >> (16 * k)) & 0xFFFFu;
-            zm |= (g != 0) << k;
-            rows |= g;
-          }
-          const uint32_t ym = ((rows & 0xF) != 0) | (((rows >> 4) & 0xF) != 0) << 1 |
-                              (((rows >> 8) & 0xF) != 0) << 2 | (((rows >> 12) & 0xF) != 0) << 3;
-          const uint32_t xm = (rows | rows >> 4 | rows >> 8 | rows >> 12) & 0xF;
-          const int bx0 = sx + __ffs(xm) - 1, bx1 = sx + 31 - __clz(xm);
-          const int by0 = sy + __ffs(ym) - 1, by1 = sy + 31 - __clz(ym);
-          const int bz0 = sz + __ffs(zm) - 1, bz1 = sz + 31 - __clz(zm);
+          // z from the lowest and highest open lane, y and x from the (y, x)
+          // mask of all four z-slices (open != 0: the bit scans are defined)
+          const int bz0 = sz + (__builtin_ctzll(open) >> 4), bz1 = sz + ((63 - __builtin_clzll(open)) >> 4);
+          const uint32_t t2 = (uint32_t)open | (uint32_t)(open >> 32);
+          const uint32_t rows = (t2 | (t2 >> 16)) & 0xFFFFu;  // bit x + 4y
+          const int by0 = sy + (__builtin_ctz(rows) >> 2), by1 = sy + ((31 - __builtin_clz(rows)) >> 2);
+          const uint32_t r8 = rows | (rows >> 8), xm = (r8 | (r8 >> 4)) & 0xFu;
+          const int bx0 = sx + __builtin_ctz(xm), bx1 = sx + 31 - __builtin_clz(xm);
           if (ci < cnt_round) {
             int cx, cy, cz;
             if (dotp) {
