@@ -626,8 +626,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         // read instead of a wave reduction).  Box of those voxels (lane =
         // x + 4y + 16z of the sub-brick) from the wave's mask with scalar bit
         // operations; a candidate that misses it cannot change any result.
-        const uint32_t first_key =
-            (uint32_t)__builtin_amdgcn_readfirstlane((int)(s_r2[c0] >> kshift));
+        // (lane 0 holds entry c0: no LDS round trip)
+        const uint32_t first_key = (uint32_t)__builtin_amdgcn_readfirstlane((int)cr) >> kshift;
         const uint32_t cmax = ((first_key + 1u) << gshift) - 1u;
         const unsigned long long open = __ballot(target && (m >> 16) < cmax);
         bool rel = false;
@@ -719,10 +719,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           done = true;
           break;
         }
-        const uint32_t last = min(c0 + 63, cnt_round - 1);
-        const uint32_t lastb = s_r2[last] >> kshift;
-        if (held == 0xFFFFFFFFu && wait == 0) held = lastb;
+        if (held == 0xFFFFFFFFu && wait == 0) {
+          // the chunk's last entry, held by lane last - c0
+          const uint32_t last = min(c0 + 63, cnt_round - 1);
+          held = (uint32_t)__builtin_amdgcn_readlane((int)cr, (int)(last - c0)) >> kshift;
+        }
         if (held != 0xFFFFFFFFu) {
+          const uint32_t last = min(c0 + 63, cnt_round - 1);
           const bool more = last + 1 < cnt_round && (s_r2[last + 1] >> kshift) == held;
           if (!more) {
             done = true;
