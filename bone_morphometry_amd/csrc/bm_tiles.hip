@@ -612,11 +612,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       for (uint32_t c0 = 0; c0 < cnt_round; c0 += 64) {
         ++scanned;
         const uint32_t ci = c0 + lane;
-        uint32_t cp = 0, cr = 0;
-        if (ci < cnt_round) {
-          cp = s_pos[ci];
-          cr = s_r2[ci];
-        }
+        // (lanes past the round read its last entry, no branch; the live
+        // lane test below keeps them out)
+        const bool live = ci < cnt_round;
+        const uint32_t cp = s_pos[min(ci, cnt_round - 1)], cr = s_r2[min(ci, cnt_round - 1)];
         // Voxels this chunk can still raise: targets whose maximum so far is
         // below the chunk's largest r^2 (those not covered yet among them).
         // With shift > 0 a bucket mixes radii in any order, so a voxel
@@ -640,22 +639,20 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           const int by0 = sy + (__builtin_ctz(rows) >> 2), by1 = sy + ((31 - __builtin_clz(rows)) >> 2);
           const uint32_t r8 = rows | (rows >> 8), xm = (r8 | (r8 >> 4)) & 0xFu;
           const int bx0 = sx + __builtin_ctz(xm), bx1 = sx + 31 - __builtin_clz(xm);
-          if (ci < cnt_round) {
-            int cx, cy, cz;
-            if (dotp) {
-              cx = (int)(cp & 255) - 64;
-              cy = (int)((cp >> 8) & 255) - 64;
-              cz = (int)((cp >> 16) & 255) - 64;
-            } else {
-              cx = (int)(cp & 1023) - 512;
-              cy = (int)((cp >> 10) & 1023) - 512;
-              cz = (int)(cp >> 20) - 512;
-            }
-            const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
-                      ez = max(max(bz0 - cz, cz - bz1), 0);
-            rel = gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= (cr >> 16);
-            if (held != 0xFFFFFFFFu && (cr >> kshift) != held) rel = false;
+          int cx, cy, cz;
+          if (dotp) {
+            cx = (int)(cp & 255) - 64;
+            cy = (int)((cp >> 8) & 255) - 64;
+            cz = (int)((cp >> 16) & 255) - 64;
+          } else {
+            cx = (int)(cp & 1023) - 512;
+            cy = (int)((cp >> 10) & 1023) - 512;
+            cz = (int)(cp >> 20) - 512;
           }
+          const int ex = max(max(bx0 - cx, cx - bx1), 0), ey = max(max(by0 - cy, cy - by1), 0),
+                    ez = max(max(bz0 - cz, cz - bz1), 0);
+          rel = live & (gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= (cr >> 16)) &
+                (held == 0xFFFFFFFFu || (cr >> kshift) == held);
         }
         unsigned long long rm = __ballot(rel);
         if (dotp) {
