@@ -208,8 +208,15 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
 // bits 0-9 (x), 10-19 (y) and 20-29 (z).
 // Squared length of a gap vector.  Gaps are below 2^15 (axes are at most 16384
 // voxels), so 24-bit multiplies are exact and run at the full vector rate.
+__device__ __forceinline__ uint32_t sq24(uint32_t v) {
+  // (written out: the compiler turns __umul24 of an operand it can bound
+  // into a full 32-bit multiply, a quarter-rate instruction)
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %1" : "=v"(r) : "v"(v));
+  return r;
+}
 __device__ __forceinline__ uint32_t gap2(uint32_t ex, uint32_t ey, uint32_t ez) {
-  return __umul24(ex, ex) + __umul24(ey, ey) + __umul24(ez, ez);
+  return sq24(ex) + sq24(ey) + sq24(ez);
 }
 
 // Number of entries of a tile list that can reach a box at squared distance
