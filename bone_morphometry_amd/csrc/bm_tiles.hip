@@ -64,7 +64,11 @@ __device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
 __global__ void __launch_bounds__(512)
 k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                uint32_t nitems) {
-  __shared__ uint32_t s_d2[10 * 10 * 10];
+  // The 10^3 block with rows 40 words apart and planes 448: the 32 lanes of a
+  // half-wavefront (x 0..7, y 0..3 of one plane) then read 32 distinct LDS
+  // banks (y * 40 mod 32 = 8y), where rows of 10 words give 2-way conflicts.
+  constexpr int kRy = 40, kRz = 448;  // (448 = 7 * 64: plane offsets pair in ds_read2st64)
+  __shared__ uint32_t s_d2[kRz * 10];
   __shared__ uint32_t s_cnt[kBuckets];
   __shared__ uint32_t s_pref[16];
   __shared__ uint32_t s_max, s_count;
@@ -101,8 +105,9 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
     const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + 512;
     const uint32_t v0 = halo_value(i0);
     const uint32_t v1 = i1 < 1000 ? halo_value(i1) : 0u;
-    s_d2[i0] = v0;
-    if (i1 < 1000) s_d2[i1] = v1;
+    auto lds_index = [](uint32_t i) { return i % 10 + kRy * ((i / 10) % 10) + kRz * (i / 100); };
+    s_d2[lds_index(i0)] = v0;
+    if (i1 < 1000) s_d2[lds_index(i1)] = v1;
   }
   static_assert(kBuckets == 512, "one bucket counter per thread");
   s_cnt[threadIdx.x] = 0;
@@ -111,7 +116,7 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
   const uint32_t li = threadIdx.x, lane = li & 63;
   const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
   const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
-  const int c = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
+  const int c = (lx + 1) + kRy * (ly + 1) + kRz * (lz + 1);
   const uint32_t r2 = valid ? s_d2[c] : 0;
   const bool sel = r2 != 0;
   // The 26 neighbours (always inside the staged 10^3 block) and the templates
@@ -122,7 +127,7 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
-    nb[q] = q == 13 ? 0u : s_d2[c + dx + 10 * dy + 100 * dz];
+    nb[q] = q == 13 ? 0u : s_d2[c + dx + kRy * dy + kRz * dz];
   }
   // fill target: selected, or in the image with a selected face neighbour
   const bool target = sel || (valid && (nb[4] | nb[10] | nb[12] | nb[14] | nb[16] | nb[22]) != 0);
