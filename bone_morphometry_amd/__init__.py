@@ -29,6 +29,8 @@ and two forms with no counterpart in the reference:
   Server                              request loop with HBM-budget admission and
                                       continuous batching
   autotune / tuning_load              per-shape-class kernel tunables (JSON cache)
+  set_stats_order / stats_order       exact device sums (default) or the
+                                      reference's sequential f64 order
 
 Every call runs on the GPU through libbm.so; errors raise BmError carrying the
 reference's message.
@@ -44,7 +46,7 @@ from .api import (Batch, BinaryVolume, BmError, DistributionStats, FeatureMode, 
                   read_roi_tiff_device, require_isotropic_spacing,
                   signed_distance_field_um, signed_distance_field_um_device,
                   smoothed_signed_distance_field_um, smoothed_signed_distance_field_um_device,
-                  tuning_load)
+                  StatsOrder, set_stats_order, stats_order, tuning_load)
 from . import synth  # noqa: F401
 
 __version__ = "0.1.0"

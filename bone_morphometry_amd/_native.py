@@ -24,7 +24,7 @@ EXPORTS = (
     "bm_euclidean_distance_transform_squared", "bm_local_thickness",
     "bm_analyze_thickness_spacing", "bm_euclidean_distance_transform_squared_device",
     "bm_local_thickness_device", "bm_analyze_thickness_spacing_device",
-    "bm_local_thickness_stages", "bm_profile_enable", "bm_profile_read",
+    "bm_local_thickness_stages", "bm_set_stats_order", "bm_profile_enable", "bm_profile_read",
     "bm_batch_create", "bm_batch_run", "bm_batch_device_bytes", "bm_batch_destroy",
     "bm_autotune", "bm_tuning_load",
     "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_stats_get",
@@ -149,6 +149,7 @@ def lib():
             ctypes.POINTER(BmThicknessSpacing), VP]
         L.bm_local_thickness_stages.argtypes = [VOL, I, P, P, P, P, ctypes.POINTER(U64),
                                                 ctypes.POINTER(U64)]
+        L.bm_set_stats_order.argtypes = [I, ctypes.POINTER(I)]
         ITEMS = ctypes.POINTER(BmBatchItem)
         L.bm_batch_create.argtypes = [ITEMS, U64, I, ctypes.POINTER(VP)]
         L.bm_batch_run.argtypes = [VP, ctypes.POINTER(BmBatchResult)]

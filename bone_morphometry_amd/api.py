@@ -108,6 +108,35 @@ def local_thickness_stages(volume, phase):
                 ridge_count=rc.value, fill_visits=fv.value)
 
 
+class StatsOrder(enum.IntEnum):
+    """Order of the f64 sums behind mean and SD (BM_STATS_* in bm.h)."""
+    Exact = 0       # exact device sums, rounded once
+    Sequential = 1  # the reference's order (src/metrics.rs:215-226), bit for bit
+
+
+def set_stats_order(order):
+    """Sets the process-wide statistics order; returns the previous one."""
+    prev = ctypes.c_int(0)
+    _check(_native.lib().bm_set_stats_order(int(order), ctypes.byref(prev)))
+    return StatsOrder(prev.value)
+
+
+class stats_order:
+    """Context manager: `with stats_order(StatsOrder.Sequential): ...`."""
+
+    def __init__(self, order):
+        self.order = order
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = set_stats_order(self.order)
+        return self
+
+    def __exit__(self, *exc):
+        set_stats_order(self.prev)
+        return False
+
+
 def require_isotropic_spacing(spacing):
     """src/sdf_curvature.rs:147-160 -> h (the common spacing) or BmError."""
     sp = (ctypes.c_double * 3)(*[float(v) for v in spacing])

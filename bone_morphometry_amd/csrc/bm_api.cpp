@@ -13,6 +13,8 @@
 // count*sum_of_squares - sum*sum is evaluated unfused, as Rust does.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -20,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/bm.h"
@@ -158,6 +161,13 @@ static std::string spacing_list(const double sp[3]) {
   return "[" + rust_f64(sp[0]) + ", " + rust_f64(sp[1]) + ", " + rust_f64(sp[2]) + "]";
 }
 
+// A ridge radius the tile lists cannot hold (status error bit 0): outside the
+// device path's limits.
+static void fail_list_range() {
+  throw std::range_error(
+      "a squared distance exceeds the device path's limit for ridge radii (r^2 < 2^23)");
+}
+
 static void fail_all_selected(Phase p) {
   throw Error(std::string("distance transform is undefined for an all-") + phase_name(p) +
               " volume unless an exterior convention is specified");
@@ -227,6 +237,7 @@ static const Plan& local_thickness_device(Engine& e, const uint8_t* d_data, cons
   if (stages) e.stage_outputs(pl, stages->ridge_flags, stages->propagated, s);
   pl.fetch(s);
   if (!pl.result(0).status.has_site) fail_all_selected(p);
+  if (pl.result(0).status.error) fail_list_range();
   if (stages) {
     BM_HIP(hipMemcpyAsync(&stages->status, pl.desc(0).status, sizeof(PhaseStatus),
                           hipMemcpyDeviceToHost, s));
@@ -292,7 +303,8 @@ void check_mode(int mode) {
 
 // The two items (bone, marrow) of one analyze call.
 void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims,
-                   const double spacing[3], int mode, std::vector<ItemSpec>& out) {
+                   const double spacing[3], int mode, std::vector<ItemSpec>& out,
+                   bool keep_maps) {
   const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
   const float scale_f32 = (float)spacing[0];
   for (int ph = 0; ph < 2; ++ph) {
@@ -304,7 +316,7 @@ void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims
     sp.tune = tuning_for(dims);
     sp.thickness = true;
     sp.quantiles = quantiles;
-    sp.want_map = quantiles;
+    sp.want_map = quantiles || keep_maps;
     sp.keep_d2 = false;
     sp.scale = scale_f32;
     sp.lsb = stats_lsb_exp(scale_f32);
@@ -315,7 +327,8 @@ void analyze_items(const uint8_t* d_bone, const uint8_t* d_roi, const Dims& dims
 // Results of analyze from the two items' results, with the error precedence
 // of src/metrics.rs:90-104.
 void analyze_finish(const ItemResult& bone, const ItemResult& marrow, uint64_t n,
-                    const double spacing[3], int mode, bm_thickness_spacing* out) {
+                    const double spacing[3], int mode, bm_thickness_spacing* out,
+                    const StatsResult* seq) {
   const bool quantiles = mode == BM_MODE_STANDARD || mode == BM_MODE_EXPERIMENTAL;
   const float scale_f32 = (float)spacing[0];
   const int lsb = stats_lsb_exp(scale_f32);
@@ -326,19 +339,92 @@ void analyze_finish(const ItemResult& bone, const ItemResult& marrow, uint64_t n
     throw Error("BoneJ compatibility mode requires an ROI mask with both inside and outside voxels");
   if (bone_voxels == 0) throw Error("ROI contains no bone voxels");
   if (marrow_voxels == 0) throw Error("ROI contains no marrow voxels");
-  if ((bone.status.error | marrow.status.error) != 0)
-    throw DeviceError("squared distance beyond the tile lists' range");
+  if ((bone.status.error | marrow.status.error) != 0) fail_list_range();
   QuantileResult q[2];
   if (quantiles) {
     q[0] = quantiles_from(bone.sel, bone.stats.count, scale_f32);
     q[1] = quantiles_from(marrow.sel, marrow.stats.count, scale_f32);
   }
-  out->thickness = distribution_stats(finish_stats(bone.stats, lsb), quantiles ? &q[0] : nullptr);
-  out->spacing = distribution_stats(finish_stats(marrow.stats, lsb), quantiles ? &q[1] : nullptr);
+  StatsResult sums[2] = {finish_stats(bone.stats, lsb), finish_stats(marrow.stats, lsb)};
+  if (seq) {
+    for (int i = 0; i < 2; ++i) {
+      if (sums[i].flags) throw DeviceError("statistics accumulator left its exact range");
+      if (seq[i].count != sums[i].count)
+        throw DeviceError("sequential statistics disagree with the device count");
+      sums[i] = seq[i];
+    }
+  }
+  out->thickness = distribution_stats(sums[0], quantiles ? &q[0] : nullptr);
+  out->spacing = distribution_stats(sums[1], quantiles ? &q[1] : nullptr);
   const double voxel_volume = spacing[0] * spacing[1] * spacing[2];
   out->bone_volume = (double)bone_voxels * voxel_volume;
   out->total_volume = (double)roi_voxels * voxel_volume;
   out->bone_volume_fraction = out->bone_volume / out->total_volume;
+}
+
+// ---------------------------------------------------------------------------
+// Order of the statistics sums (bm_set_stats_order).
+static std::atomic<int> g_stats_order{BM_STATS_EXACT};
+
+bool stats_sequential() { return g_stats_order.load() == BM_STATS_SEQUENTIAL; }
+
+// The reference's loop (src/metrics.rs:215-226) over a device map: chunks come
+// back through two pinned buffers, the next one in flight while the current
+// one is summed, one value at a time in index order (this file is built with
+// FMA contraction off, so sum_of_squares += value * value rounds twice).
+StatsResult sequential_sums(const float* d_map, uint64_t n, float scale_f32, hipStream_t s) {
+  constexpr size_t kChunk = (size_t)4 << 20;  // values per chunk (16 MiB)
+  static std::mutex mu;
+  static float* buf[2] = {nullptr, nullptr};
+  std::lock_guard<std::mutex> g(mu);
+  if (!buf[0]) {
+    BM_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf[0]), kChunk * sizeof(float)));
+    BM_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf[1]), kChunk * sizeof(float)));
+  }
+  uint64_t count = 0;
+  double sum = 0.0, sum_of_squares = 0.0, maximum = 0.0;
+  const uint64_t chunks = (n + kChunk - 1) / kChunk;
+  auto issue = [&](uint64_t k) {
+    const uint64_t off = k * kChunk, len = std::min<uint64_t>(kChunk, n - off);
+    BM_HIP(hipMemcpyAsync(buf[k & 1], d_map + off, len * sizeof(float), hipMemcpyDeviceToHost,
+                          s));
+  };
+  if (chunks) issue(0);
+  BM_HIP(hipStreamSynchronize(s));
+  for (uint64_t k = 0; k < chunks; ++k) {
+    if (k + 1 < chunks) issue(k + 1);
+    const float* b = buf[k & 1];
+    const uint64_t len = std::min<uint64_t>(kChunk, n - k * kChunk);
+    for (uint64_t i = 0; i < len; ++i) {
+      const float raw = b[i];
+      if (!std::isfinite(raw) || raw <= 0.0f) continue;
+      const double value = (double)(raw * scale_f32);
+      count += 1;
+      sum += value;
+      sum_of_squares += value * value;
+      maximum = std::max(maximum, value);
+    }
+    BM_HIP(hipStreamSynchronize(s));
+  }
+  StatsResult r;
+  r.count = count;
+  r.sum = sum;
+  r.sum_of_squares = sum_of_squares;
+  r.maximum = maximum;
+  return r;
+}
+
+bool replay_sums(const Plan& plan, size_t first, float scale_f32, hipStream_t s,
+                 StatsResult out[2]) {
+  for (int i = 0; i < 2; ++i) {
+    const ItemResult& r = plan.result(first + i);
+    if (r.status.selected == 0 || !r.status.has_site) return false;
+  }
+  for (int i = 0; i < 2; ++i) {
+    const ItemDesc& d = plan.desc(first + i);
+    out[i] = sequential_sums(d.map, d.n, scale_f32, s);
+  }
+  return true;
 }
 
 // Thickness/spacing part of analyze (src/metrics.rs:64-132) on device-resident
@@ -350,12 +436,16 @@ static void analyze_device(Engine& e, const uint8_t* d_bone, const uint8_t* d_ro
   check_spacing(spacing);
   check_mode(mode);
   std::vector<ItemSpec> specs;
-  analyze_items(d_bone, d_roi, dims, spacing, mode, specs);
+  const bool seq = stats_sequential();
+  analyze_items(d_bone, d_roi, dims, spacing, mode, specs, seq);
   Plan& pl = e.plain_plan();
   pl.build(specs, s);
   pl.launch(s);
   pl.fetch(s);
-  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, out);
+  StatsResult sums[2];
+  const bool replayed = seq && replay_sums(pl, 0, (float)spacing[0], s, sums);
+  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, out,
+                 replayed ? sums : nullptr);
 }
 
 void analyze_thickness_spacing(const BinaryVolume& bone, const BinaryVolume& roi,
@@ -426,15 +516,19 @@ static void analyze_full_device(Engine& e, const uint8_t* d_bone, const uint8_t*
   check_spacing(spacing);
   check_mode(mode);
   std::vector<ItemSpec> specs;
-  analyze_items(d_bone, d_roi, dims, spacing, mode, specs);
+  const bool seq = stats_sequential();
+  analyze_items(d_bone, d_roi, dims, spacing, mode, specs, seq);
   Plan& pl = e.plain_plan();
   pl.build(specs, s);
   pl.launch(s);
   TopologyCounts t;
   topology_device(e, d_bone, d_roi, dims, kTopoCubical | kTopoBonej | kTopoBetti, &t, s);
   pl.fetch(s);
+  StatsResult sums[2];
+  const bool replayed = seq && replay_sums(pl, 0, (float)spacing[0], s, sums);
   bm_thickness_spacing ts;
-  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, &ts);
+  analyze_finish(pl.result(0), pl.result(1), dims.n(), spacing, mode, &ts,
+                 replayed ? sums : nullptr);
   out->thickness = ts.thickness;
   out->spacing = ts.spacing;
   out->bone_volume = ts.bone_volume;
@@ -625,6 +719,15 @@ int bm_local_thickness_device(const uint8_t* d_data, uint64_t width, uint64_t he
     Engine& e = engine_for_device(current_device());
     std::lock_guard<std::mutex> g(e.mutex());
     local_thickness_device(e, d_data, dims, p, d_out, stream_or_default(e, stream));
+  });
+}
+
+int bm_set_stats_order(int order, int* previous) {
+  return guarded([&] {
+    if (order != BM_STATS_EXACT && order != BM_STATS_SEQUENTIAL)
+      throw Error(fmt("unknown statistics order %d", order));
+    const int prev = g_stats_order.exchange(order);
+    if (previous) *previous = prev;
   });
 }
 

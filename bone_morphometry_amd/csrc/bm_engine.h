@@ -59,10 +59,10 @@ struct Dims {
 struct PhaseStatus {
   unsigned long long selected;     // voxels of the phase (X pass)
   unsigned long long ridge_count;  // distance ridges found (stage outputs only)
-  unsigned long long fill_visits;  // unused (kept for the layout)
+  unsigned long long fill_visits;  // sphere-fill work (stage outputs only)
   uint32_t has_site;               // any non-selected voxel (X pass)
   uint32_t max_d2;                 // maximum squared distance (Z pass)
-  uint32_t error;                  // bit 0: r^2 too large for the tile lists
+  uint32_t error;                  // bit 0: r^2 at or above the item's r2_limit
   uint32_t overflow_tiles;         // tiles whose fill candidates exceeded LDS
   unsigned long long fill_diag16[16];  // fill work counters and clocks (diagnostics)
 };
@@ -158,6 +158,7 @@ struct ItemDesc {
   TileMeta* meta;      // per-tile list header
   unsigned long long* masks;  // per tile: 8 selected + 8 target words
   uint32_t* overflow;  // [count, tile ids...]
+  uint32_t* bigtiles;  // [count, tile ids...]: tiles holding ridges of r^2 > kGatherMaxR2
   uint32_t* tmpl;      // 3 x (bound+1) templates
   float* diam;         // (bound+1) diameters 2*sqrt(m)
   PhaseStatus* status;
@@ -178,6 +179,7 @@ struct ItemDesc {
   uint32_t zgx, zgy, zgz, zchunk, zhalo;
   uint32_t lblocks, fill_cap, cblocks, qblocks;
   uint32_t tile_order;  // Tuning::tile_order
+  uint32_t r2_limit;    // ridges of r^2 >= r2_limit set status error bit 0
   // invariant-divisor constants (d, m, s1, s2) of TX, TY, (TX + 3) / 4 and
   // (TY + 3) / 4 for the tile kernels' position -> tile mapping
   uint32_t divs[4][4];
@@ -247,9 +249,12 @@ class Plan {
   ~Plan();
   // Lays out and (re)allocates the buffers of these items and uploads the
   // descriptors (synchronously on s).  With ext, the buffers are laid out in
-  // that device region instead (at most ext_cap bytes; nothing is allocated).
+  // that device region instead (at most ext_cap bytes; nothing is allocated
+  // for them).  With ext_meta as well, the descriptors, start tables and
+  // results live at the end of that region too and results come back through
+  // pageable host memory: the plan then makes no HIP allocation at all.
   void build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext = nullptr,
-             size_t ext_cap = 0);
+             size_t ext_cap = 0, bool ext_meta = false);
   // Enqueues every stage for every item on s.  No allocation, no host
   // synchronisation, no memset nodes: safe under stream capture.
   void launch(hipStream_t s) const;
@@ -258,10 +263,13 @@ class Plan {
   void fetch(hipStream_t s);
   size_t size() const { return desc_.size(); }
   const ItemDesc& desc(size_t i) const { return desc_[i]; }
+  const ItemDesc* device_desc() const { return d_desc_; }
   const ItemResult& result(size_t i) const { return results_[i]; }
   size_t device_bytes() const { return bytes_; }
   // Device bytes a plan of these items needs (for admission control).
   static size_t bytes_for(const std::vector<ItemSpec>& specs);
+  // Device bytes of the descriptors, start tables and results of nitems items.
+  static size_t meta_bytes_for(size_t nitems);
   // Allocates the descriptor and result space of plans of up to nitems items
   // now, so that later builds of that size allocate nothing.
   void reserve_items(size_t nitems);
@@ -273,6 +281,7 @@ class Plan {
   uint32_t y_lds_ = 0, z_lds_ = 0;
   uint32_t fill_tiles_ = 1;  // fill variant of the whole plan
   uint32_t any_ = 0;  // union of item flags
+  bool ext_meta_ = false;  // descriptors and results in the caller's region
   size_t bytes_ = 0;
   Arena mem_;                         // item buffers
   Arena meta_;                        // descriptors, start tables, results
@@ -296,8 +305,9 @@ class Engine {
   // phase_inside (src/volume.rs:84-105) as a device map.
   void run_phase_inside(const uint8_t* bone, const uint8_t* roi, uint64_t n, bool select_bone,
                         uint8_t* out, hipStream_t s);
-  // Diagnostic products of a one-item plan that has run: ridge flags from the
-  // tile lists (and the ridge count into its status) and the propagated map.
+  // Diagnostic products of a one-item plan that has run: the sphere-fill work
+  // into its status, ridge flags from the tile lists (and the ridge count into
+  // its status) and the propagated map.
   void stage_outputs(const Plan& p, uint8_t* ridge_flags_out, uint32_t* propagated_out,
                      hipStream_t s);
 

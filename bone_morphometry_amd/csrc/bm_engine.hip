@@ -36,8 +36,8 @@ __global__ void k_fill_gather_diag(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather2(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_reduce_diag(const ItemDesc*);
 __global__ void k_fill_overflow(const ItemDesc*, const uint32_t*, uint32_t);
-__global__ void k_zero_if_scatter(const ItemDesc*, const uint32_t*, uint32_t);
-__global__ void k_fill_scatter_lists(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_scatter_big(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_fill_work(const ItemDesc*);
 __global__ void k_cleanup_tiled(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_count_ridges(const TileMeta*, uint32_t, PhaseStatus*);
 __global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
@@ -81,6 +81,17 @@ static uint32_t grid_for(uint64_t items, unsigned per_block, unsigned cap) {
 }
 
 static uint32_t round8(uint64_t v) { return (uint32_t)((v + kXcds - 1) / kXcds * kXcds); }
+
+// Largest ridge r^2 the tile lists hold, plus one (kListR2Limit).  Tests lower
+// it with BM_LIST_R2_LIMIT to reach the out-of-range error on small volumes.
+static uint32_t list_r2_limit() {
+  uint32_t lim = kListR2Limit;
+  if (const char* env = std::getenv("BM_LIST_R2_LIMIT")) {
+    const long x = std::strtol(env, nullptr, 10);
+    if (x >= 1 && x <= (long)kListR2Limit) lim = (uint32_t)x;
+  }
+  return lim;
+}
 
 // ---------------------------------------------------------------------------
 // Item layout: geometry, block counts and buffer offsets (relative to the
@@ -132,6 +143,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   d.lblocks = lt ? std::min<uint32_t>(std::max<uint32_t>((d.bound + 4) / 4, 1), 128) : 0;
   d.fill_cap = t.fill_cap ? std::min<uint32_t>(t.fill_cap, kFillCap) : kFillCap;
   d.tile_order = t.tile_order & 7u;
+  d.r2_limit = list_r2_limit();
   {
     const uint32_t dv[4] = {d.TX, d.TY, (d.TX + 3) / 4, (d.TY + 3) / 4};
     for (int i = 0; i < 4; ++i) {
@@ -180,6 +192,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
     d.meta = reinterpret_cast<TileMeta*>(take((size_t)d.ntiles * sizeof(TileMeta)));
     d.masks = reinterpret_cast<unsigned long long*>(take((size_t)d.ntiles * 16 * 8));
     d.overflow = reinterpret_cast<uint32_t*>(take(((size_t)d.ntiles + 1) * sizeof(uint32_t)));
+    d.bigtiles = reinterpret_cast<uint32_t*>(take(((size_t)d.ntiles + 1) * sizeof(uint32_t)));
     d.tmpl = reinterpret_cast<uint32_t*>(take(3 * ((size_t)d.bound + 1) * sizeof(uint32_t)));
     d.diam = reinterpret_cast<float*>(take(((size_t)d.bound + 1) * sizeof(float)));
     d.partials = reinterpret_cast<StatsPartial*>(take((size_t)d.cblocks * sizeof(StatsPartial)));
@@ -210,11 +223,18 @@ size_t Plan::bytes_for(const std::vector<ItemSpec>& specs) {
   return total;
 }
 
+size_t Plan::meta_bytes_for(size_t nit) {
+  const size_t o_starts = align_up(nit * sizeof(ItemDesc), 256);
+  const size_t o_res = align_up(o_starts + (size_t)kNumStages * (nit + 1) * sizeof(uint32_t), 256);
+  return align_up(o_res + nit * sizeof(ItemResult), 256);
+}
+
 Plan::~Plan() {
   if (h_results_) (void)hipHostFree(h_results_);
 }
 
-void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, size_t ext_cap) {
+void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, size_t ext_cap,
+                 bool ext_meta) {
   const size_t nit = specs.size();
   if (nit == 0) throw std::invalid_argument("a plan needs at least one item");
   desc_.assign(nit, ItemDesc{});
@@ -235,8 +255,11 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
   }
   if (y_lds_ > 65536 || z_lds_ > 65536) throw std::invalid_argument("EDT tile exceeds 64 KiB of LDS");
   char* base;
+  ext_meta_ = ext && ext_meta;
+  const size_t meta_bytes = ext_meta_ ? meta_bytes_for(nit) : 0;
   if (ext) {
-    if (total > ext_cap) throw std::range_error("plan exceeds the memory given to it");
+    if (total + meta_bytes > ext_cap || total + meta_bytes < total)
+      throw std::range_error("plan exceeds the memory given to it");
     base = static_cast<char*>(ext);
   } else {
     mem_.reserve(total);
@@ -260,8 +283,13 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
   const size_t o_desc = 0;
   const size_t o_starts = align_up(nit * sizeof(ItemDesc), 256);
   const size_t o_res = align_up(o_starts + starts_.size() * sizeof(uint32_t), 256);
-  reserve_items(nit);
-  char* mb = static_cast<char*>(meta_.base());
+  char* mb;
+  if (ext_meta_) {
+    mb = base + align_up(total, 256);
+  } else {
+    reserve_items(nit);
+    mb = static_cast<char*>(meta_.base());
+  }
   d_desc_ = reinterpret_cast<ItemDesc*>(mb + o_desc);
   d_starts_ = reinterpret_cast<uint32_t*>(mb + o_starts);
   d_results_ = reinterpret_cast<ItemResult*>(mb + o_res);
@@ -308,9 +336,7 @@ void Plan::launch(hipStream_t s) const {
     if (any_ & kItemDiag) k_reduce_diag<<<nit, 256, 0, s>>>(d_desc_);
     BM_LAUNCH("fill_overflow", k_fill_overflow, total_[kStOverflow], 512, 0, s, d_desc_,
               st(kStOverflow), nit);
-    BM_LAUNCH("fill_scatter_zero", k_zero_if_scatter, total_[kStOverflow], 256, 0, s, d_desc_,
-              st(kStOverflow), nit);
-    BM_LAUNCH("fill_scatter", k_fill_scatter_lists, total_[kStOverflow], 256, 0, s, d_desc_,
+    BM_LAUNCH("fill_scatter", k_fill_scatter_big, total_[kStOverflow], 256, 0, s, d_desc_,
               st(kStOverflow), nit);
     BM_LAUNCH("cleanup_stats", k_cleanup_tiled, total_[kStCleanup], 512, 0, s, d_desc_,
               st(kStCleanup), nit);
@@ -331,6 +357,12 @@ void Plan::launch(hipStream_t s) const {
 
 void Plan::fetch(hipStream_t s) {
   const size_t nit = desc_.size();
+  if (ext_meta_) {  // pageable destination: no pinned buffer to allocate
+    BM_HIP(hipMemcpyAsync(results_.data(), d_results_, nit * sizeof(ItemResult),
+                          hipMemcpyDeviceToHost, s));
+    BM_HIP(hipStreamSynchronize(s));
+    return;
+  }
   BM_HIP(hipMemcpyAsync(h_results_, d_results_, nit * sizeof(ItemResult), hipMemcpyDeviceToHost,
                         s));
   BM_HIP(hipStreamSynchronize(s));
@@ -363,6 +395,7 @@ void Engine::stage_outputs(const Plan& p, uint8_t* ridge_flags_out, uint32_t* pr
                            hipStream_t s) {
   const ItemDesc& d = p.desc(0);
   if (!(d.flags & kItemThickness)) return;
+  k_fill_work<<<4096, 256, 0, s>>>(p.device_desc());
   if (ridge_flags_out) {
     k_count_ridges<<<256, 256, 0, s>>>(d.meta, d.ntiles, d.status);
     k_fill_u8<<<grid_for(d.n, 256, 65536), 256, 0, s>>>(ridge_flags_out, d.n, 0);

@@ -74,7 +74,10 @@ __global__ void __launch_bounds__(256) k_prepare(const ItemDesc* __restrict__ it
   uint32_t* st = reinterpret_cast<uint32_t*>(I.status);
   for (uint32_t i = threadIdx.x; i < sizeof(PhaseStatus) / 4; i += blockDim.x) st[i] = 0;
   if (I.flags & kItemThickness) {
-    if (threadIdx.x == 0) gbl(I.overflow)[0] = 0;
+    if (threadIdx.x == 0) {
+      gbl(I.overflow)[0] = 0;
+      gbl(I.bigtiles)[0] = 0;
+    }
     if (I.flags & kItemQuantiles)
       for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) gbl(I.top16)[i] = 0;
   }

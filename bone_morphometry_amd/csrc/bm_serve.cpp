@@ -32,16 +32,17 @@ struct FinishCtx {
   const double* spacing;
   int mode;
   bm_thickness_spacing* out;
+  const StatsResult* seq;
 };
 void finish_volume(void* p) {
   FinishCtx* c = static_cast<FinishCtx*>(p);
-  analyze_finish(*c->bone, *c->marrow, c->n, c->spacing, c->mode, c->out);
+  analyze_finish(*c->bone, *c->marrow, c->n, c->spacing, c->mode, c->out, c->seq);
 }
 
 VolumeResult finish(const ItemResult& bone, const ItemResult& marrow, const Dims& dims,
-                    const double spacing[3], int mode) {
+                    const double spacing[3], int mode, const StatsResult* seq = nullptr) {
   VolumeResult r;
-  FinishCtx c{&bone, &marrow, dims.n(), spacing, mode, &r.metrics};
+  FinishCtx c{&bone, &marrow, dims.n(), spacing, mode, &r.metrics, seq};
   r.status = capture_errors(finish_volume, &c, &r.message);
   if (r.status != BM_OK) r.metrics = bm_thickness_spacing{};
   return r;
@@ -50,7 +51,8 @@ VolumeResult finish(const ItemResult& bone, const ItemResult& marrow, const Dims
 
 // ---------------------------------------------------------------------------
 Batch::Batch(const bm_batch_item* items, size_t count, bool use_graph)
-    : items_(items, items + count), first_item_(count, -1), early_(count) {
+    : items_(items, items + count), first_item_(count, -1), early_(count),
+      seq_(stats_sequential()) {
   std::vector<ItemSpec> specs;
   for (size_t i = 0; i < count; ++i) {
     const bm_batch_item& it = items_[i];
@@ -63,7 +65,7 @@ Batch::Batch(const bm_batch_item* items, size_t count, bool use_graph)
       continue;
     }
     first_item_[i] = (int)specs.size();
-    analyze_items(it.d_bone, it.d_roi, c.dims, it.voxel_spacing, it.feature_mode, specs);
+    analyze_items(it.d_bone, it.d_roi, c.dims, it.voxel_spacing, it.feature_mode, specs, seq_);
   }
   BM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   if (specs.empty()) return;
@@ -112,8 +114,11 @@ void Batch::run(std::vector<VolumeResult>& out) {
     dims.w = (uint32_t)it.width;
     dims.h = (uint32_t)it.height;
     dims.d = (uint32_t)it.depth;
+    StatsResult sums[2];
+    const bool replayed =
+        seq_ && replay_sums(plan_, (size_t)first_item_[i], (float)it.voxel_spacing[0], stream_, sums);
     out[i] = finish(plan_.result(first_item_[i]), plan_.result(first_item_[i] + 1), dims,
-                    it.voxel_spacing, it.feature_mode);
+                    it.voxel_spacing, it.feature_mode, replayed ? sums : nullptr);
   }
 }
 

@@ -52,6 +52,7 @@ class Batch {
   std::vector<bm_batch_item> items_;
   std::vector<int> first_item_;       // plan item of each volume's bone phase, or -1
   std::vector<VolumeResult> early_;   // volumes rejected before the launch
+  bool seq_ = false;                  // BM_STATS_SEQUENTIAL at creation
   Plan plan_;
   hipStream_t stream_ = nullptr;
   hipGraph_t graph_ = nullptr;

@@ -27,9 +27,15 @@ constexpr int kFillCap = 1024;              // LDS candidates per round of the f
 constexpr int kStageCap = 2048;             // candidates of a tile staged in LDS
 constexpr int kChunkMap = 2048;             // 64-entry list chunks mapped to their tile in LDS
 constexpr int kMaxNeighbours = 512;         // tiles in reach kept per tile
-constexpr uint32_t kGatherMaxRadius = 128;  // beyond this the scatter fill is used
+// Ridges of radius above kGatherMaxRadius (r^2 > kGatherMaxR2) are left out
+// of the gather and scattered afterwards (atomic maxima); the gather's reach
+// and candidate keys assume radii up to this bound.
+constexpr uint32_t kGatherMaxRadius = 128;
+constexpr uint32_t kGatherMaxR2 = kGatherMaxRadius * kGatherMaxRadius;
+// A list entry is r^2 << 9 | local index: r^2 must stay below 2^23.
+constexpr uint32_t kListR2Limit = 1u << 23;
 constexpr int kDiamLds = 2048;              // diameters the clean-up keeps in LDS
-constexpr uint32_t kOverflowBlocks = 64;    // blocks per item of the fill fallbacks
+constexpr uint32_t kOverflowBlocks = 64;    // blocks per item of the fill fallbacks and the scatter
 
 // Radius thresholds of the prefix counts: (2j + 1)^2.
 inline __host__ __device__ uint32_t kPrefR2(int j) { return (uint32_t)(2 * j + 1) * (2 * j + 1); }

@@ -204,7 +204,11 @@ k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ 
     m.maxr2 = s_max;
     for (int q = 0; q < 16; ++q) m.pref[q] = (uint16_t)s_pref[q];
     *(TileMeta*)&meta[t] = m;
-    if (m.maxr2 >= (1u << 23)) atomicOr((uint32_t*)&st->error, 1u);
+    if (m.maxr2 >= I.r2_limit) atomicOr((uint32_t*)&st->error, 1u);
+    if (m.maxr2 > kGatherMaxR2) {  // balls the gather leaves to the scatter
+      BM_GLOBAL uint32_t* __restrict__ big = gbl(I.bigtiles);
+      big[1 + atomicAdd((uint32_t*)&big[0], 1u)] = t;
+    }
   }
 }
 
@@ -331,8 +335,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
             hz = min(oz + 7, (int)D - 1);
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t max_r2 = st->max_d2;
-  const int R = (int)ceil_sqrt_u32(max_r2);
-  if (R > (int)kGatherMaxRadius) return;  // the scatter fill handles this volume
+  // Ridges of radius above kGatherMaxRadius are left to the scatter pass
+  // (k_fill_scatter_big): the gather takes r^2 <= r2cap only.
+  const uint32_t r2cap = min(max_r2, kGatherMaxR2);
+  const int R = (int)ceil_sqrt_u32(r2cap);
   if (threadIdx.x < 8 * TXN)
     s_tgt[threadIdx.x] = (threadIdx.x >> 3) < ntx
                              ? masks[(size_t)(t + (threadIdx.x >> 3)) * 16 + 8 + (threadIdx.x & 7)]
@@ -491,7 +497,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
                   ez = max(max(512 - pz, pz - wbz), 0);
         const uint32_t r2 = en >> 9;
         // (no short circuit: every lane computes the gap, no branch around it)
-        const bool keep = (en != 0) & (gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= r2);
+        const bool keep = (en != 0) & (gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= r2) &
+                          (r2 <= r2cap);
         const uint32_t pos = (uint32_t)px | ((uint32_t)py << 10) | ((uint32_t)pz << 20);
         use(keep, pos, r2);
       }
@@ -826,7 +833,7 @@ k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   if (!st->has_site) return;
   const uint32_t count = overflow[0];
   const uint32_t max_r2 = st->max_d2;
-  const int R = (int)ceil_sqrt_u32(max_r2);
+  const int R = (int)ceil_sqrt_u32(min(max_r2, kGatherMaxR2));  // as the gather
   const int K = (R + 7) / 8;
   const uint32_t gshift = bucket_shift(max_r2);
   for (uint32_t oi = vb; oi < count; oi += kOverflowBlocks) {
@@ -845,6 +852,7 @@ k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           for (uint32_t e = 0; e < mt.count; ++e) {
             const uint32_t ent = lists[(size_t)u * 512 + e];
             const uint32_t r2 = ent >> 9;
+            if (r2 > kGatherMaxR2) continue;  // scattered by k_fill_scatter_big
             if (r2 <= m) {
               if ((r2 >> gshift) < (m >> gshift)) break;  // sorted: the rest is smaller
               continue;
@@ -859,58 +867,91 @@ k_fill_overflow(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   }
 }
 
-// Scatter fallback for very large radii: one wavefront per ridge, lanes over
-// the (2R+1)^2 (y, x) plane of each z, atomic max.
+// Balls of radius above kGatherMaxRadius, after the gather (which leaves them
+// out) and its overflow pass: atomic maxima into M.  The ridge pass lists the
+// tiles that hold such ridges.  Every block visits every such ridge and takes
+// the z-planes -R + vb, -R + vb + kOverflowBlocks, ... of its ball, so one
+// large ball is spread over all blocks of the item; a wavefront takes a
+// (z, y) row and its lanes consecutive x (coalesced atomics).  Each plane,
+// row and run is clipped exactly: |dz| <= isqrt(r^2), |dy| <= isqrt(r^2 -
+// dz^2), |dx| <= isqrt(r^2 - dz^2 - dy^2), and to the array.
 __global__ void __launch_bounds__(256)
-k_fill_scatter_lists(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
-                     uint32_t nitems) {
+k_fill_scatter_big(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                   uint32_t nitems) {
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, ntiles = I.ntiles;
+  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
+  if (!st->has_site || st->max_d2 <= kGatherMaxR2) return;
+  const int W = (int)I.W, H = (int)I.H, D = (int)I.D;
+  const uint32_t TX = I.TX, TY = I.TY;
   BM_GLOBAL const uint32_t* __restrict__ lists = gbl(I.lists);
   BM_GLOBAL const TileMeta* __restrict__ meta = gbl(I.meta);
-  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
+  BM_GLOBAL const uint32_t* __restrict__ big = gbl(I.bigtiles);
   BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);
-  if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t gw = (vb * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nw = (kOverflowBlocks * blockDim.x) >> 6;
-  for (uint32_t t = gw; t < ntiles; t += nw) {
-    const TileMeta mt = *(const TileMeta*)&meta[t];
+  const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6),
+            nwv = (int)(blockDim.x >> 6);
+  const uint32_t nbig = big[0];
+  for (uint32_t bi = 0; bi < nbig; ++bi) {
+    const uint32_t t = big[1 + bi];
+    const uint32_t count = meta[t].count;
     const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
-    for (uint32_t e = 0; e < mt.count; ++e) {
+    for (uint32_t e = 0; e < count; ++e) {
       const uint32_t ent = lists[(size_t)t * 512 + e];
       const uint32_t r2 = ent >> 9, l = ent & 511;
+      if (r2 <= kGatherMaxR2) continue;
       const int x = ox + (int)(l & 7), y = oy + (int)((l >> 3) & 7), z = oz + (int)(l >> 6);
-      const int R = (int)ceil_sqrt_u32(r2);
-      const int side = 2 * R + 1, plane = side * side;
-      for (int dz = -R; dz <= R; ++dz) {
+      const int R = (int)isqrt_u32(r2);
+      for (int dz = -R + (int)vb; dz <= R; dz += (int)kOverflowBlocks) {
         const int nz = z + dz;
-        const uint32_t dz2 = (uint32_t)(dz * dz);
-        if (nz < 0 || nz >= (int)D || dz2 > r2) continue;
-        for (int k = (int)lane; k < plane; k += 64) {
-          const int dy = k / side - R, dx = k % side - R;
-          const int ny = y + dy, nx = x + dx;
-          if (ny < 0 || ny >= (int)H || nx < 0 || nx >= (int)W) continue;
-          if (dz2 + (uint32_t)(dy * dy) + (uint32_t)(dx * dx) > r2) continue;
-          atomicMax((uint32_t*)&M[((size_t)nz * H + ny) * W + nx], r2);
+        if (nz < 0 || nz >= D) continue;
+        const uint32_t rz = r2 - (uint32_t)(dz * dz);
+        const int Ry = (int)isqrt_u32(rz);
+        for (int dy = -Ry + wv; dy <= Ry; dy += nwv) {
+          const int ny = y + dy;
+          if (ny < 0 || ny >= H) continue;
+          const int Rx = (int)isqrt_u32(rz - (uint32_t)(dy * dy));
+          const int x0 = max(x - Rx, 0), x1 = min(x + Rx, W - 1);
+          BM_GLOBAL uint32_t* __restrict__ row = M + ((size_t)nz * H + ny) * W;
+          for (int nx = x0 + lane; nx <= x1; nx += 64) atomicMax((uint32_t*)&row[nx], r2);
         }
       }
     }
   }
 }
 
+// Work of the sphere fill as the reference's propagation does it
+// (src/local_thickness.rs:182-201): the sum over ridges of the number of
+// voxels of the ball within the array.  One thread per list entry, exact
+// integer row extents (diagnostics: runs after the plan).
 __global__ void __launch_bounds__(256)
-k_zero_if_scatter(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
-                  uint32_t nitems) {
-  uint32_t vb;
-  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
-  BM_GLOBAL const PhaseStatus* __restrict__ st = gbl(I.status);
-  if (!st->has_site || ceil_sqrt_u32(st->max_d2) <= kGatherMaxRadius) return;
-  BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);
-  const uint64_t n = I.n;
-  const uint64_t stride = (uint64_t)kOverflowBlocks * blockDim.x;
-  for (uint64_t i = (uint64_t)vb * blockDim.x + threadIdx.x; i < n; i += stride) M[i] = 0;
+k_fill_work(const ItemDesc* __restrict__ items) {
+  const ItemDesc& I = items[0];
+  const int W = (int)I.W, H = (int)I.H, D = (int)I.D;
+  const uint32_t TX = I.TX, TY = I.TY, ntiles = I.ntiles;
+  BM_GLOBAL const uint32_t* __restrict__ lists = gbl(I.lists);
+  BM_GLOBAL const TileMeta* __restrict__ meta = gbl(I.meta);
+  BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
+  unsigned long long visits = 0;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t count = meta[t].count;
+    const int ox = (int)(t % TX) * 8, oy = (int)((t / TX) % TY) * 8, oz = (int)(t / (TX * TY)) * 8;
+    for (uint32_t e = threadIdx.x; e < count; e += blockDim.x) {
+      const uint32_t ent = lists[(size_t)t * 512 + e];
+      const uint32_t r2 = ent >> 9, l = ent & 511;
+      const int x = ox + (int)(l & 7), y = oy + (int)((l >> 3) & 7), z = oz + (int)(l >> 6);
+      const int R = (int)isqrt_u32(r2);
+      for (int nz = max(z - R, 0); nz <= min(z + R, D - 1); ++nz) {
+        const uint32_t rz = r2 - (uint32_t)((nz - z) * (nz - z));
+        const int Ry = (int)isqrt_u32(rz);
+        for (int ny = max(y - Ry, 0); ny <= min(y + Ry, H - 1); ++ny) {
+          const int Rx = (int)isqrt_u32(rz - (uint32_t)((ny - y) * (ny - y)));
+          visits += (unsigned long long)(min(x + Rx, W - 1) - max(x - Rx, 0) + 1);
+        }
+      }
+    }
+  }
+  visits = wave_sum_u64(visits);
+  if ((threadIdx.x & 63) == 0 && visits) atomicAdd((unsigned long long*)&st->fill_visits, visits);
 }
 
 // Total ridge count (diagnostics) from the tile headers.

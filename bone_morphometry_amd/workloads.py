@@ -92,6 +92,49 @@ class Request:
         return self.width * self.height * self.depth
 
 
+def serving_arrivals(rate=150.0, duration=60.0, seed=5):
+    """Poisson arrival times (seconds) at `rate` per second over `duration`."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=int(rate * duration * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    return arrivals[arrivals < duration]
+
+
+def serving_pool_params(count, seed, median_side=192, sigma_ln=0.7, lo=64, hi=512,
+                        jitter=0.2):
+    """Parameters of `count` serving requests, each its own generated volume:
+    voxel count lognormal (median median_side^3, log-sigma sigma_ln, clipped
+    to [lo^3, hi^3]) with near-cubic axes (each jittered by U[-jitter,
+    jitter], clipped to [lo, hi]); sigma U[2, 6], BV/TV U[0.08, 0.35], a
+    centred z-cylinder ROI of diameter U[0.70, 0.95]*min(w, h), spacing
+    U[6, 20] um, and a generator seed."""
+    rng = np.random.default_rng(seed)
+    mu = np.log(float(median_side) ** 3)
+    out = []
+    for _ in range(count):
+        nvox = float(np.clip(np.exp(rng.normal(mu, sigma_ln)), float(lo) ** 3, float(hi) ** 3))
+        side = nvox ** (1.0 / 3.0)
+        w, h, d = (int(np.clip(round(side * (1.0 + rng.uniform(-jitter, jitter))), lo, hi))
+                   for _ in range(3))
+        sigma = float(rng.uniform(2.0, 6.0))
+        bvtv = float(rng.uniform(0.08, 0.35))
+        frac = float(rng.uniform(0.70, 0.95))
+        sp = float(rng.uniform(6.0, 20.0))
+        out.append(dict(width=w, height=h, depth=d, sigma=sigma, bv_tv=bvtv,
+                        diameter=frac * min(w, h), spacing=sp,
+                        seed=int(rng.integers(1, 2**62))))
+    return out
+
+
+def serving_pool(count, seed, threads=0, **kw):
+    """The pool of serving_pool_params(count, seed, **kw) as Request volumes."""
+    reqs = []
+    for p in serving_pool_params(count, seed, **kw):
+        wl = make_volume(p, threads)
+        reqs.append(Request(wl.width, wl.height, wl.depth, wl.bone, wl.roi, wl.spacing))
+    return reqs
+
+
 def serving_stream(rate=150.0, duration=60.0, seed=5, pool=256, median_side=192,
                    sigma_ln=0.7, lo=64, hi=512, jitter=0.2, bases=4, threads=0):
     """The serving workload: Poisson arrival times at `rate` per second over

@@ -116,6 +116,24 @@ int bm_analyze_thickness_spacing(const bm_volume *bone, const bm_volume *roi,
                                  const double voxel_spacing[3], int feature_mode,
                                  bm_thickness_spacing *out);
 
+/* Order of the f64 sums behind the mean and standard deviation of the
+ * thickness statistics (no counterpart in the reference, whose order is
+ * BM_STATS_SEQUENTIAL: src/metrics.rs:215-226 sums the map in f64, value after
+ * value, in linear index order).
+ *   BM_STATS_EXACT (default): exact fixed-point sums on the device, rounded
+ *     once to f64.  Mean and SD differ from the reference's only by the
+ *     rounding of its sequential sums.
+ *   BM_STATS_SEQUENTIAL: the finished maps are copied to the host and summed
+ *     in the reference's order, so mean, SD and CV equal the reference's bit
+ *     for bit (slower: one map copy and a serial pass per phase).
+ * Process-wide; applies to the analyze calls (host and device forms, and
+ * bm_analyze) and to batches created afterwards (the server and the
+ * no-allocation entry point always use BM_STATS_EXACT).  *previous (may be
+ * NULL) receives the order that was in force. */
+#define BM_STATS_EXACT 0
+#define BM_STATS_SEQUENTIAL 1
+int bm_set_stats_order(int order, int *previous);
+
 /* Device-resident forms of the three calls above: d_* are device pointers on
  * the current HIP device, stream is a hipStream_t or NULL.  Each returns once
  * its result is complete (the statistics are copied to *out). */

@@ -465,19 +465,43 @@ typedef struct {
   uint64_t count;
 } oracle_stats;
 
-static int cmp_f32(const void *a, const void *b) {
-  float x = *(const float *)a, y = *(const float *)b;
-  return (x > y) - (x < y);
+/* select_value, src/metrics.rs:295-299: the index-th smallest value (the
+ * values here are finite and positive, so total_cmp is the plain order).  Like
+ * the reference's select_nth_unstable_by it leaves the slice partitioned about
+ * that position; the linear-time selection (Hoare partition about a median of
+ * three, recursing into the side that holds the index) returns the same order
+ * statistic whatever the partition order. */
+static float select_value(float *v, size_t len, size_t index) {
+  size_t lo = 0, hi = len - 1;
+  while (hi > lo) {
+    size_t mid = lo + (hi - lo) / 2;
+    float a = v[lo], b = v[mid], c = v[hi];
+    float pivot = a < b ? (b < c ? b : (a < c ? c : a)) : (a < c ? a : (b < c ? c : b));
+    size_t i = lo, j = hi;
+    for (;;) {
+      while (v[i] < pivot) ++i;
+      while (v[j] > pivot) --j;
+      if (i >= j) break;
+      float t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+      ++i;
+      --j;
+    }
+    /* v[lo..j] <= pivot <= v[j+1..hi] */
+    if (index <= j) hi = j;
+    else lo = j + 1;
+  }
+  return v[index];
 }
 
-/* linear_quantile_unstable, src/metrics.rs:271-291, over values sorted
- * ascending (selection by sort gives the same order statistics). */
-static double linear_quantile_sorted(const float *sorted, size_t len, double q, float scale) {
+/* linear_quantile_unstable, src/metrics.rs:271-293. */
+static double linear_quantile(float *values, size_t len, double q, float scale) {
   double position = q * (double)(len - 1);
   size_t lower = (size_t)floor(position), upper = (size_t)ceil(position);
   double fraction = position - (double)lower;
-  double lo = (double)sorted[lower];
-  double hi = upper == lower ? lo : (double)sorted[upper];
+  double lo = (double)select_value(values, len, lower);
+  double hi = upper == lower ? lo : (double)select_value(values, len, upper);
   double qv = lo + fraction * (hi - lo);
   return (double)((float)qv * scale);
 }
@@ -485,8 +509,7 @@ static double linear_quantile_sorted(const float *sorted, size_t len, double q, 
 double oracle_quantile(const float *values, size_t len, double q, float scale) {
   float *s = (float *)malloc(len * sizeof(float));
   memcpy(s, values, len * sizeof(float));
-  qsort(s, len, sizeof(float), cmp_f32);
-  double r = linear_quantile_sorted(s, len, q, scale);
+  double r = linear_quantile(s, len, q, scale);
   free(s);
   return r;
 }
@@ -517,12 +540,12 @@ int oracle_nonzero_stats(const float *values, size_t n, double scale, int includ
     size_t k = 0;
     for (size_t i = 0; i < n; ++i)
       if (isfinite(values[i]) && values[i] > 0.0f) kept[k++] = values[i];
-    qsort(kept, k, sizeof(float), cmp_f32);
-    med = linear_quantile_sorted(kept, k, 0.5, scale_f32);
-    p10 = linear_quantile_sorted(kept, k, 0.1, scale_f32);
-    p25 = linear_quantile_sorted(kept, k, 0.25, scale_f32);
-    p75 = linear_quantile_sorted(kept, k, 0.75, scale_f32);
-    p90 = linear_quantile_sorted(kept, k, 0.9, scale_f32);
+    /* the reference's call order: median, P10, P25, P75, P90 (:245-251) */
+    med = linear_quantile(kept, k, 0.5, scale_f32);
+    p10 = linear_quantile(kept, k, 0.1, scale_f32);
+    p25 = linear_quantile(kept, k, 0.25, scale_f32);
+    p75 = linear_quantile(kept, k, 0.75, scale_f32);
+    p90 = linear_quantile(kept, k, 0.9, scale_f32);
     free(kept);
   }
   st->mean = mean;

@@ -5,14 +5,34 @@ requests, and whatever the tunables; per-volume errors are the one-volume
 call's errors."""
 import json
 import math
+import os
+import time
 
 import numpy as np
 import pytest
 
+import _oracle as orc
 import bone_morphometry_amd as bm
-from bone_morphometry_amd import _native, hip, synth
+from _stats import check_stats
+from bone_morphometry_amd import _native, hip, synth, workloads
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def _check_golden(got, g, sequential=False):
+    """A ThicknessSpacing against an oracle record of gen_workload_golden.py."""
+    assert not isinstance(got, bm.BmError), str(got)
+    check_stats(got.thickness, g["thickness"], sequential)
+    check_stats(got.spacing, g["spacing_stats"], sequential)
+    assert (got.bone_volume, got.total_volume, got.bone_volume_fraction) == \
+        (g["bone_volume"], g["total_volume"], g["bone_volume_fraction"])
 
 
 def _volumes(seed, count, lo=24, hi=72):
@@ -82,6 +102,74 @@ def test_batch_matches_one_volume_calls(use_graph):
         got = batch.run()
         assert all(_same(g, r) for g, r in zip(got, ref))
     batch.close()
+    # and each against the oracle
+    for v, it, g in zip(vols, items, got):
+        o = orc.analyze_thickness_spacing(v["bone"], v["roi"], v["w"], v["h"], v["d"],
+                                          v["spacing"],
+                                          include_quantiles=it[6] == bm.FeatureMode.Standard)
+        check_stats(g.thickness, o.thickness)
+        check_stats(g.spacing, o.spacing)
+
+
+def test_batched_workload_matches_oracle():
+    """The 96-volume batched workload (axes 64..384, sigma 2..6, BV/TV
+    0.08..0.35) as one graph-replayed batch: every volume's statistics against
+    the oracle's (tests/golden/batched_oracle.json), in both summation
+    orders."""
+    golden = _golden("batched_oracle.json")["volumes"]
+    params = workloads.batched_params()
+    assert len(golden) == len(params) == 96
+    vols, bufs, items = [], [], []
+    for p in params:
+        wl = workloads.make_volume(p)
+        b, r = hip.DeviceBuffer(wl.n), hip.DeviceBuffer(wl.n)
+        b.upload(wl.bone)
+        r.upload(wl.roi)
+        bufs.append((b, r))
+        items.append((b.value, r.value, wl.width, wl.height, wl.depth, wl.spacing,
+                      bm.FeatureMode.Standard))
+    batch = bm.Batch(items, use_graph=True)
+    for _ in range(2):
+        got = batch.run()
+        for g, gold in zip(got, golden):
+            _check_golden(g, gold)
+    batch.close()
+    with bm.stats_order(bm.StatsOrder.Sequential):
+        batch = bm.Batch(items, use_graph=True)
+    got = batch.run()
+    batch.close()
+    for g, gold in zip(got, golden):
+        _check_golden(g, gold, sequential=True)
+
+
+def test_serving_stream_matches_oracle():
+    """A time-compressed serving stream: Poisson arrivals at 150/s for 4 s
+    (about 600 requests) over the 24-volume request pool (lognormal sizes of
+    median 192^3, each volume its own sigma and BV/TV), submitted at their
+    arrival times: every result equals the oracle's for its volume
+    (tests/golden/serving_oracle.json)."""
+    g = _golden("serving_oracle.json")
+    pool = workloads.serving_pool(**g["pool"])
+    golden = g["volumes"]
+    arrivals = workloads.serving_arrivals(rate=150.0, duration=4.0, seed=7)
+    srv = bm.Server(budget_bytes=48 << 30, max_batch=32, slots=2)
+    t0 = time.perf_counter()
+    tickets = []
+    for i, at in enumerate(arrivals):
+        delay = at - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        q = pool[i % len(pool)]
+        tickets.append(srv.submit(q.bone, q.roi, q.width, q.height, q.depth, q.spacing))
+    sizes = []
+    for i, t in enumerate(tickets):
+        res, tim = srv.poll(t, wait=True)
+        sizes.append(tim["batch_size"])
+        _check_golden(res, golden[i % len(pool)])
+    st = srv.stats()
+    srv.close()
+    assert st["requests_done"] == len(tickets) > 400
+    assert max(sizes) > 1  # requests that arrived together shared a batch
 
 
 def test_batch_reports_per_volume_errors():

@@ -1,0 +1,104 @@
+"""Parity at the benchmark sizes.
+
+plain_512 (512^3, sigma 4, BV/TV 0.15, cylinder ROI of diameter 460, 10.5 um):
+the squared EDT, ridge flags, propagated r^2, thickness map, ridge count and
+fill work of both analysis phases (bone and marrow inside the ROI) equal the
+oracle's, computed here on the same seeded volume.
+
+large (1024x1024x1536, sigma (5, 5, 9), BV/TV 0.12, ellipsoid ROI, 8 um): the
+same products of both phases against the oracle's digests (xxh3-128 of each
+map) in tests/golden/large_oracle.json, which the oracle needs about an hour
+and 50 GB of host memory to produce (tests/golden/gen_workload_golden.py);
+analyze's statistics against the oracle's, in both summation orders."""
+import json
+import os
+
+import numpy as np
+import pytest
+import xxhash
+
+import _oracle as orc
+import bone_morphometry_amd as bm
+from _stats import check_stats
+from bone_morphometry_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LARGE_GOLDEN = os.path.join(HERE, "golden", "large_oracle.json")
+MAPS = ("squared_edt", "ridge_flags", "propagated", "thickness")
+
+
+def _phase(wl, name):
+    """phase_inside(roi, select_bone) of the analysis (src/volume.rs:84-105)."""
+    return ((wl.roi != 0) & ((wl.bone != 0) == (name == "bone"))).astype(np.uint8)
+
+
+def _digest(a):
+    return xxhash.xxh3_128_hexdigest(memoryview(np.ascontiguousarray(a)).cast("B"))
+
+
+@pytest.fixture(scope="module")
+def plain512():
+    return workloads.plain_512()
+
+
+@pytest.mark.parametrize("phase", ["bone", "marrow"])
+def test_plain_512_stage_maps_match_oracle(plain512, phase):
+    wl = plain512
+    sel = _phase(wl, phase)
+    ref = orc.local_thickness_stages(sel, wl.width, wl.height, wl.depth, 0)
+    got = bm.local_thickness_stages(bm.BinaryVolume(sel, wl.width, wl.height, wl.depth),
+                                    bm.Phase.Foreground)
+    for k in MAPS:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    assert got["ridge_count"] == ref["ridge_count"]
+    assert got["fill_visits"] == ref["fill_visits"]
+
+
+def _large_golden(phase):
+    if not os.path.exists(LARGE_GOLDEN):
+        pytest.skip("tests/golden/large_oracle.json not generated")
+    with open(LARGE_GOLDEN) as f:
+        g = json.load(f)
+    if phase not in g:
+        pytest.skip(f"large_oracle.json has no {phase} phase")
+    return g[phase]
+
+
+@pytest.fixture(scope="module")
+def large():
+    return workloads.large()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("phase", ["bone", "marrow"])
+def test_large_stage_maps_match_oracle(large, phase):
+    g = _large_golden(phase)
+    wl = large
+    sel = _phase(wl, phase)
+    assert int(np.count_nonzero(sel)) == g["selected"]
+    got = bm.local_thickness_stages(bm.BinaryVolume(sel, wl.width, wl.height, wl.depth),
+                                    bm.Phase.Foreground)
+    del sel
+    assert int(got["squared_edt"].max()) == g["max_d2"]
+    for k in MAPS:
+        assert _digest(got[k]) == g[k], k
+    assert got["ridge_count"] == g["ridge_count"]
+    assert got["fill_visits"] == g["fill_visits"]
+
+
+@pytest.mark.slow
+def test_large_analyze_matches_oracle(large):
+    gb, gm = _large_golden("bone"), _large_golden("marrow")
+    wl = large
+    b = bm.BinaryVolume(wl.bone, wl.width, wl.height, wl.depth)
+    r = bm.BinaryVolume(wl.roi, wl.width, wl.height, wl.depth)
+    got = bm.analyze_thickness_spacing(b, r, wl.spacing)
+    check_stats(got.thickness, gb["stats"])
+    check_stats(got.spacing, gm["stats"])
+    assert got.bone_volume == gb["selected"] * 8.0 ** 3
+    with bm.stats_order(bm.StatsOrder.Sequential):
+        got = bm.analyze_thickness_spacing(b, r, wl.spacing)
+    check_stats(got.thickness, gb["stats"], sequential=True)
+    check_stats(got.spacing, gm["stats"], sequential=True)

@@ -10,6 +10,7 @@ import pytest
 
 import _oracle as orc
 import bone_morphometry_amd as bm
+from _stats import check_stats
 from bone_morphometry_amd import synth, workloads
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +119,7 @@ def test_local_thickness_stages_match_oracle(seed, shape, frac):
     np.testing.assert_array_equal(got["squared_edt"], ref["squared_edt"])
     np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
     assert got["ridge_count"] == ref["ridge_count"]
+    assert got["fill_visits"] == ref["fill_visits"]
     np.testing.assert_array_equal(got["propagated"], ref["propagated"])
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
@@ -264,42 +266,24 @@ def _close(a, b):
     return abs(a - b) <= REL * abs(b)
 
 
-def sd_tolerance(n, cv):
-    """Relative SD difference the reference's sequential f64 sum of squares can
-    cause: |error of sum x^2| <= (n-1) 2^-53 sum x^2, amplified by the
-    cancellation in n*sum x^2 - (sum x)^2 by (1 + 1/cv^2), halved by the sqrt.
-    The device sums are exact (fixed point), so this is the whole gap."""
-    return max(REL, (n - 1) * 2.0 ** -53 * (1.0 + 1.0 / cv ** 2) / 2.0)
+def _check_stats(g, r, sequential=False):
+    check_stats(g, r, sequential)
 
 
-def _check_stats(g, r):
-    assert g.count == r.count
-    assert g.maximum == r.maximum
-    # the device sum is exact; the reference's sequential f64 sum of n positive
-    # values is within (n - 1) 2^-53 of it, relatively (exact at small sizes)
-    mtol = max(REL, (r.count - 1) * 2.0 ** -53)
-    assert abs(g.mean - r.mean) <= mtol * abs(r.mean), ("mean", g.mean, r.mean, mtol)
-    tol = sd_tolerance(r.count, r.coefficient_of_variation)
-    for f in ("standard_deviation", "coefficient_of_variation"):
-        gv, rv = getattr(g, f), getattr(r, f)
-        assert abs(gv - rv) <= tol * abs(rv), (f, gv, rv, tol)
-    for f in ("median", "percentile_10", "percentile_25", "percentile_75", "percentile_90",
-              "interquartile_range"):
-        gv, rv = getattr(g, f), getattr(r, f)
-        assert (math.isnan(gv) and math.isnan(rv)) or gv == rv, (f, gv, rv)
-
-
+@pytest.mark.parametrize("order", [bm.StatsOrder.Exact, bm.StatsOrder.Sequential])
 @pytest.mark.parametrize("mode", [bm.FeatureMode.Standard, bm.FeatureMode.Classic])
-def test_analyze_plain_small_matches_oracle(mode):
+def test_analyze_plain_small_matches_oracle(mode, order):
     wl = workloads.plain_small()
     b = vol(wl.bone, wl.width, wl.height, wl.depth)
     r = vol(wl.roi, wl.width, wl.height, wl.depth)
-    got = bm.analyze_thickness_spacing(b, r, wl.spacing, mode)
+    with bm.stats_order(order):
+        got = bm.analyze_thickness_spacing(b, r, wl.spacing, mode)
     q = mode in (bm.FeatureMode.Standard, bm.FeatureMode.Experimental)
     ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, wl.depth,
                                         wl.spacing, include_quantiles=q)
-    _check_stats(got.thickness, ref.thickness)
-    _check_stats(got.spacing, ref.spacing)
+    seq = order == bm.StatsOrder.Sequential
+    _check_stats(got.thickness, ref.thickness, seq)
+    _check_stats(got.spacing, ref.spacing, seq)
     assert got.bone_volume == ref.bone_volume and got.total_volume == ref.total_volume
     assert got.bone_volume_fraction == ref.bone_volume_fraction
 
@@ -309,10 +293,92 @@ def test_analyze_cylinder_roi_matches_oracle():
     from bone_morphometry_amd import synth
     bone = synth.volume(w, h, d, 2.5, 0.2, 11)
     roi = synth.roi_cylinder(w, h, d, 70.0)
-    got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d), (7.25, 7.25, 7.25))
     ref = orc.analyze_thickness_spacing(bone, roi, w, h, d, (7.25, 7.25, 7.25))
+    got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d), (7.25, 7.25, 7.25))
     _check_stats(got.thickness, ref.thickness)
     _check_stats(got.spacing, ref.spacing)
+    with bm.stats_order(bm.StatsOrder.Sequential):
+        got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d),
+                                           (7.25, 7.25, 7.25))
+    _check_stats(got.thickness, ref.thickness, True)
+    _check_stats(got.spacing, ref.spacing, True)
+
+
+def test_stats_order_rejects_unknown_values():
+    with pytest.raises(bm.BmError, match="unknown statistics order 7"):
+        bm.set_stats_order(7)
+    assert bm.set_stats_order(bm.StatsOrder.Exact) == bm.StatsOrder.Exact
+
+
+# --- radii beyond the gather: the scatter of the largest balls -------------------
+@pytest.mark.parametrize("radius", [129, 160])
+@pytest.mark.parametrize("axis", [0, 2])
+def test_fill_scatter_beyond_gather_radius_matches_oracle(axis, radius):
+    """Two planes of sites 2R apart: every ridge of the mid-plane has radius R
+    above the gather's 128, so its ball is scattered; the voxels near the
+    planes take the gathered smaller balls."""
+    dims = [11, 10, 9]
+    dims[axis] = 2 * radius + 1
+    w, h, d = dims
+    a = np.ones((d, h, w), np.uint8)
+    ends = [slice(None)] * 3
+    for pos in (0, 2 * radius):
+        ends[2 - axis] = pos
+        a[tuple(ends)] = 0
+    ref = orc.local_thickness_stages(a, w, h, d, 0)
+    assert int(ref["squared_edt"].max()) == radius * radius
+    got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+    assert got["fill_visits"] == ref["fill_visits"]
+
+
+def _cavity_volume(n, radius, seed):
+    """A spherical cavity of the given radius at the centre of an n^3 volume
+    plus small struts of the generator outside it (radii of a few voxels), as
+    the selected phase."""
+    c = (n - 1) / 2.0
+    z, y, x = np.ogrid[0:n, 0:n, 0:n]
+    sphere = ((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) <= radius * radius
+    struts = synth.volume(n, n, n, 2.0, 0.1, seed).reshape(n, n, n) != 0
+    return (sphere | struts).astype(np.uint8).reshape(-1)
+
+
+def test_fill_large_cavity_matches_oracle():
+    """A marrow cavity of radius 140 (balls above the gather's radius, so the
+    scatter pass adds them after the gather) among small struts."""
+    n = 296
+    a = _cavity_volume(n, 140.0, 31)
+    ref = orc.local_thickness_stages(a, n, n, n, 0)
+    assert int(ref["squared_edt"].max()) > 128 * 128
+    got = bm.local_thickness_stages(vol(a, n, n, n), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
+    np.testing.assert_array_equal(got["propagated"], ref["propagated"])
+    np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+
+
+@pytest.mark.parametrize("call", ["local_thickness", "stages", "analyze"])
+def test_ridge_radius_beyond_list_range_raises(call, monkeypatch):
+    """Ridges whose r^2 the tile lists cannot hold (2^23 and above, lowered
+    here to 50 by the test override) fail every thickness path with
+    BM_ERR_UNSUPPORTED instead of returning maps built from cut radii."""
+    monkeypatch.setenv("BM_LIST_R2_LIMIT", "50")
+    w, h, d = 40, 36, 44
+    a = _structure(5, (d, h, w), 0.85)
+    assert orc.edt_squared(a, w, h, d, 0).max() >= 50
+    with pytest.raises(bm.BmError, match="limit for ridge radii") as e:
+        if call == "local_thickness":
+            bm.local_thickness(vol(a, w, h, d), bm.Phase.Foreground)
+        elif call == "stages":
+            bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+        else:
+            bm.analyze_thickness_spacing(vol(a, w, h, d), vol(np.ones_like(a), w, h, d),
+                                         (1.0, 1.0, 1.0))
+    assert e.value.code == 3
+    monkeypatch.delenv("BM_LIST_R2_LIMIT")
+    got = bm.local_thickness(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got, orc.local_thickness(a, w, h, d, 0))
 
 
 @pytest.mark.parametrize("bone,roi,mode,msg", [
