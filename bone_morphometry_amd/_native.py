@@ -27,7 +27,8 @@ EXPORTS = (
     "bm_local_thickness_stages", "bm_set_stats_order", "bm_profile_enable", "bm_profile_read",
     "bm_batch_create", "bm_batch_run", "bm_batch_device_bytes", "bm_batch_destroy",
     "bm_autotune", "bm_tuning_load",
-    "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_stats_get",
+    "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_cancel",
+    "bm_server_stats_get",
     "bm_server_destroy",
     "bm_require_isotropic_spacing", "bm_gaussian_kernel", "bm_signed_distance_field_um",
     "bm_gaussian_smooth_nearest_in_place", "bm_smoothed_signed_distance_field_um",
@@ -162,6 +163,7 @@ def lib():
                                        I, ctypes.POINTER(U64)]
         L.bm_server_poll.argtypes = [VP, U64, I, ctypes.POINTER(BmBatchResult),
                                      ctypes.POINTER(BmRequestTiming), ctypes.POINTER(I)]
+        L.bm_server_cancel.argtypes = [VP, U64, ctypes.POINTER(I)]
         L.bm_server_stats_get.argtypes = [VP, ctypes.POINTER(BmServerStats)]
         L.bm_server_destroy.argtypes = [VP]
         D = ctypes.c_double

@@ -538,6 +538,12 @@ class Server:
         return _result(res), dict(submit_s=tim.submit_s, start_s=tim.start_s, done_s=tim.done_s,
                                   batch_size=tim.batch_size)
 
+    def cancel(self, ticket):
+        """Withdraws a still-queued request (True); False once it has launched."""
+        c = ctypes.c_int(0)
+        _check(_native.lib().bm_server_cancel(self._h, int(ticket), ctypes.byref(c)))
+        return bool(c.value)
+
     def stats(self):
         s = _native.BmServerStats()
         _check(_native.lib().bm_server_stats_get(self._h, ctypes.byref(s)))

@@ -1143,6 +1143,14 @@ int bm_server_poll(bm_server* server, uint64_t ticket, int wait, bm_batch_result
   });
 }
 
+int bm_server_cancel(bm_server* server, uint64_t ticket, int* cancelled) {
+  return guarded([&] {
+    if (!server || !server->impl) throw Error("null argument");
+    const bool c = server->impl->cancel(ticket);
+    if (cancelled) *cancelled = c ? 1 : 0;
+  });
+}
+
 int bm_server_stats_get(bm_server* server, bm_server_stats* out) {
   return guarded([&] {
     if (!server || !server->impl || !out) throw Error("null argument");

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "bm_analyze.h"
@@ -132,6 +133,7 @@ Server::Server(uint64_t budget_bytes, int max_batch, int slots) {
   size_t free_b = 0, total_b = 0;
   BM_HIP(hipMemGetInfo(&free_b, &total_b));
   slot_cap_ = std::min<uint64_t>(budget_ / n, (uint64_t)(free_b / 10 * 9) / n);
+  if (const char* env = std::getenv("BM_SERVER_FAIL_BATCH")) fail_batch_ = std::strtol(env, nullptr, 10);
   for (int i = 0; i < n; ++i) {
     auto s = std::make_unique<Slot>();
     BM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -210,6 +212,22 @@ uint64_t Server::submit(const uint8_t* bone, const uint8_t* roi, uint64_t w, uin
   return t;
 }
 
+bool Server::cancel(uint64_t ticket) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = reqs_.find(ticket);
+  if (it == reqs_.end()) throw Error("unknown ticket");
+  Request* r = it->second.get();
+  auto q = std::find(queue_.begin(), queue_.end(), r);
+  if (r->done || q == queue_.end()) return false;  // launched or finished
+  queue_.erase(q);
+  r->result.status = BM_ERR_INVALID;
+  r->result.message = "request cancelled";
+  r->t_start = r->t_done = now();
+  r->done = true;
+  cv_done_.notify_all();
+  return true;
+}
+
 bool Server::poll(uint64_t ticket, bool wait, VolumeResult* out, bm_request_timing* timing) {
   std::unique_lock<std::mutex> lk(mu_);
   auto it = reqs_.find(ticket);
@@ -272,6 +290,8 @@ bool Server::launch_slot(Slot& s, std::unique_lock<std::mutex>& lk) {
   std::string err;
   int code = BM_OK;
   try {
+    if (fail_batch_ >= 0 && launched_++ == fail_batch_)
+      throw DeviceError("injected device allocation failure (BM_SERVER_FAIL_BATCH)");
     s.mem.reserve(slot_cap_);
     char* base = static_cast<char*>(s.mem.base());
     size_t off = 0;

@@ -69,6 +69,9 @@ class Server {
                   const double spacing[3], int mode);
   // Returns true when the request is done (its record is then released).
   bool poll(uint64_t ticket, bool wait, VolumeResult* out, bm_request_timing* timing);
+  // Withdraws a request that is still queued (true); a request already
+  // launched or done is left alone (false).
+  bool cancel(uint64_t ticket);
   void stats(bm_server_stats* out);
 
  private:
@@ -109,6 +112,10 @@ class Server {
   std::map<uint64_t, std::unique_ptr<Request>> reqs_;
   uint64_t next_ticket_ = 1;
   bool stop_ = false;
+  // Test hook (BM_SERVER_FAIL_BATCH=k): the k-th batch launched (from 0)
+  // fails as a device allocation failure would, before anything is uploaded.
+  long fail_batch_ = -1;
+  long launched_ = 0;
   std::thread thread_;
   std::string worker_error_;
   // statistics

@@ -443,8 +443,24 @@ void decompress(const Page& p, const uint8_t* src, uint64_t n, std::vector<uint8
   out.resize(want);
 }
 
-// Decodes the page's samples (width*height bytes) into dst.
+// The tiff crate's default decoder limits (Limits::default of tiff 0.11):
+// the decoded buffer of one page, and the compressed data of one strip or
+// tile.  Larger pages fail to decode, as they do in the reference.
+constexpr uint64_t kDecodingBufferLimit = (uint64_t)256 << 20;
+constexpr uint64_t kIntermediateBufferLimit = (uint64_t)128 << 20;
+
+uint64_t checked_mul(uint64_t a, uint64_t b) {
+  const unsigned __int128 r = (unsigned __int128)a * b;
+  if (r >> 64) throw TiffFormatError("size overflow");
+  return (uint64_t)r;
+}
+
+// Decodes the page's samples (width*height bytes) into dst.  Every size is
+// checked before anything is allocated or written: the page and each strip
+// or tile within the decoder limits, products without wrap-around.
 void decode_page(const TiffFile& f, const Page& p, uint8_t* dst) {
+  if (checked_mul(p.width, p.height) > kDecodingBufferLimit)
+    throw TiffFormatError("page exceeds the decoding buffer limit");
   if (p.sample_format != 1) throw TiffFormatError("signed/float samples");
   if (p.predictor != 1 && p.predictor != 2) throw TiffFormatError("unsupported predictor");
   std::vector<uint8_t> buf;
@@ -463,20 +479,30 @@ void decode_page(const TiffFile& f, const Page& p, uint8_t* dst) {
     for (uint64_t s = 0; s < strips; ++s) {
       const uint64_t rows = std::min(rps, p.height - s * rps);
       const uint64_t want = rows * p.width;
+      if (p.counts[s] > kIntermediateBufferLimit)
+        throw TiffFormatError("strip exceeds the intermediate buffer limit");
       decompress(p, f.bytes(p.offsets[s], p.counts[s]), p.counts[s], buf, want);
       unpredict(buf.data(), rows, p.width);
       std::memcpy(dst + s * rps * p.width, buf.data(), want);
     }
   } else {
     if (!p.tile_w || !p.tile_h) throw TiffFormatError("tile dimensions missing");
+    if (p.tile_w > 0xFFFFFFFFull || p.tile_h > 0xFFFFFFFFull)
+      throw TiffFormatError("tile dimensions out of range");
+    const uint64_t tile_len = checked_mul(p.tile_w, p.tile_h);
+    if (tile_len > kDecodingBufferLimit)
+      throw TiffFormatError("tile exceeds the decoding buffer limit");
     const uint64_t across = (p.width + p.tile_w - 1) / p.tile_w;
     const uint64_t down = (p.height + p.tile_h - 1) / p.tile_h;
-    if (p.offsets.size() < across * down || p.counts.size() < across * down)
+    const uint64_t tiles = checked_mul(across, down);
+    if (p.offsets.size() < tiles || p.counts.size() < tiles)
       throw TiffFormatError("missing tile offsets or byte counts");
     for (uint64_t ty = 0; ty < down; ++ty)
       for (uint64_t tx = 0; tx < across; ++tx) {
         const uint64_t t = ty * across + tx;
-        decompress(p, f.bytes(p.offsets[t], p.counts[t]), p.counts[t], buf, p.tile_w * p.tile_h);
+        if (p.counts[t] > kIntermediateBufferLimit)
+          throw TiffFormatError("tile exceeds the intermediate buffer limit");
+        decompress(p, f.bytes(p.offsets[t], p.counts[t]), p.counts[t], buf, tile_len);
         unpredict(buf.data(), p.tile_h, p.tile_w);
         const uint64_t x0 = tx * p.tile_w, y0 = ty * p.tile_h;
         const uint64_t cw = std::min(p.tile_w, p.width - x0), ch = std::min(p.tile_h, p.height - y0);
@@ -541,7 +567,7 @@ std::string process_page(const TiffFile& f, const Page& p, uint64_t index, const
     if (p.sample_format != 1)
       return path + " decoded to an unsupported pixel type; expected unsigned 8-bit samples";
     decode_page(f, p, raw);
-  } catch (const TiffFormatError&) {
+  } catch (const std::exception&) {  // format errors, and allocation failures of the decoder
     return fmtf("decoding page %llu of %s", (unsigned long long)(index + 1), path.c_str());
   }
   if (accepted != TiffSamples::Any) {
@@ -604,6 +630,18 @@ std::string process_pages(const TiffFile& f, const Directory& dir, uint64_t lo, 
   return bad < hi ? errors[bad - lo] : std::string();
 }
 
+// The output buffer must hold page_len * depth bytes (compared without
+// wrap-around: the product can exceed 2^64 for a crafted directory).
+void check_capacity(uint64_t capacity, uint64_t page_len, uint64_t depth) {
+  const unsigned __int128 need = (unsigned __int128)page_len * depth;
+  if (need > capacity) {
+    const std::string voxels = (need >> 64) ? std::string("more than 2^64")
+                                            : std::to_string((unsigned long long)need);
+    throw Error(fmtf("output buffer holds %llu bytes, the volume has %s voxels",
+                     (unsigned long long)capacity, voxels.c_str()));
+  }
+}
+
 // Width and height of the first page and the number of pages.
 TiffVolumeInfo volume_info(const Directory& dir) {
   TiffVolumeInfo info;
@@ -629,10 +667,8 @@ TiffVolumeInfo read_binary_tiff(const std::string& path, uint8_t threshold, Tiff
   const Directory dir = read_directory(*f, path);
   TiffVolumeInfo info = volume_info(dir);
   if (!out) return info;  // the size only
-  const uint64_t page_len = info.width * info.height;
-  if (capacity < page_len * info.depth)
-    throw Error(fmtf("output buffer holds %llu bytes, the volume has %llu voxels",
-                     (unsigned long long)capacity, (unsigned long long)(page_len * info.depth)));
+  const uint64_t page_len = info.width * info.height;  // fits: both below 2^32
+  check_capacity(capacity, page_len, info.depth);
   const std::string err = process_pages(*f, dir, 0, dir.pages.size(), path, threshold, accepted,
                                         threads, [&](uint64_t k) { return out + k * page_len; });
   if (!err.empty()) throw Error(err);
@@ -653,15 +689,18 @@ TiffVolumeInfo read_binary_tiff_device(Engine& e, const std::string& path, uint8
   }
   const Directory dir = read_directory(*f, path);
   TiffVolumeInfo info = volume_info(dir);
-  const uint64_t page_len = info.width * info.height;
-  if (capacity < page_len * info.depth)
-    throw Error(fmtf("output buffer holds %llu bytes, the volume has %llu voxels",
-                     (unsigned long long)capacity, (unsigned long long)(page_len * info.depth)));
+  const uint64_t page_len = info.width * info.height;  // fits: both below 2^32
+  check_capacity(capacity, page_len, info.depth);
   // Two pinned staging buffers of up to 64 MiB (whole pages): the pool decodes
   // a batch into one while the other's copy to HBM runs.
+  // (pages above the decoder's buffer limit fail before they are written:
+  // their batches stage nothing)
+  const bool decodable = page_len <= kDecodingBufferLimit;
   const uint64_t batch_pages =
-      std::max<uint64_t>(1, std::min<uint64_t>(dir.pages.size(), (64ull << 20) / std::max<uint64_t>(page_len, 1)));
-  const uint64_t stage_bytes = std::max<uint64_t>(1, batch_pages * page_len);
+      decodable ? std::max<uint64_t>(1, std::min<uint64_t>(dir.pages.size(), (64ull << 20) /
+                                                            std::max<uint64_t>(page_len, 1)))
+                : 1;
+  const uint64_t stage_bytes = decodable ? std::max<uint64_t>(1, batch_pages * page_len) : 1;
   uint8_t* pinned[2] = {nullptr, nullptr};
   hipEvent_t done[2] = {nullptr, nullptr};
   std::string err;
@@ -677,7 +716,7 @@ TiffVolumeInfo read_binary_tiff_device(Engine& e, const std::string& path, uint8
       if (used[slot]) BM_HIP(hipEventSynchronize(done[slot]));  // its last copy has landed
       uint8_t* stage = pinned[slot];
       err = process_pages(*f, dir, lo, hi, path, threshold, accepted, threads,
-                          [&](uint64_t k) { return stage + (k - lo) * page_len; });
+                          [&](uint64_t k) { return decodable ? stage + (k - lo) * page_len : stage; });
       if (!err.empty()) break;
       BM_HIP(hipMemcpyAsync(d_out + lo * page_len, stage, (hi - lo) * page_len,
                             hipMemcpyHostToDevice, s));

@@ -367,6 +367,11 @@ int bm_server_submit(bm_server *server, const uint8_t *bone, const uint8_t *roi,
                      int feature_mode, uint64_t *ticket);
 int bm_server_poll(bm_server *server, uint64_t ticket, int wait, bm_batch_result *result,
                    bm_request_timing *timing, int *done);
+/* Withdraws a request that is still queued: *cancelled = 1, and its poll then
+ * returns BM_ERR_INVALID with "request cancelled"; a request already launched
+ * or done is left alone (*cancelled = 0).  Other requests are unaffected (the
+ * reference's per-file isolation, src/main.rs:254-268). */
+int bm_server_cancel(bm_server *server, uint64_t ticket, int *cancelled);
 int bm_server_stats_get(bm_server *server, bm_server_stats *out);
 int bm_server_destroy(bm_server *server);
 

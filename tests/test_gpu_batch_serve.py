@@ -29,8 +29,8 @@ def _golden(name):
 def _check_golden(got, g, sequential=False):
     """A ThicknessSpacing against an oracle record of gen_workload_golden.py."""
     assert not isinstance(got, bm.BmError), str(got)
-    check_stats(got.thickness, g["thickness"], sequential)
-    check_stats(got.spacing, g["spacing_stats"], sequential)
+    check_stats(got.thickness, g["thickness"], sequential, g["spacing"])
+    check_stats(got.spacing, g["spacing_stats"], sequential, g["spacing"])
     assert (got.bone_volume, got.total_volume, got.bone_volume_fraction) == \
         (g["bone_volume"], g["total_volume"], g["bone_volume_fraction"])
 
@@ -107,8 +107,8 @@ def test_batch_matches_one_volume_calls(use_graph):
         o = orc.analyze_thickness_spacing(v["bone"], v["roi"], v["w"], v["h"], v["d"],
                                           v["spacing"],
                                           include_quantiles=it[6] == bm.FeatureMode.Standard)
-        check_stats(g.thickness, o.thickness)
-        check_stats(g.spacing, o.spacing)
+        check_stats(g.thickness, o.thickness, scale=v["spacing"][0])
+        check_stats(g.spacing, o.spacing, scale=v["spacing"][0])
 
 
 def test_batched_workload_matches_oracle():
@@ -271,3 +271,43 @@ def test_tile_orders_and_fill_shapes_keep_results(tmp_path, tile_order, fill_til
     finally:
         bm.tuning_load(tmp_path / "absent.json")
     assert all(_same(a, b) for a, b in zip(after, before))
+
+
+def test_server_failures_and_cancellation_leave_other_requests_unchanged(monkeypatch):
+    """Per-request isolation in the server (the reference analyzes file after
+    file and reports each failure on its own, src/main.rs:254-268): a batch
+    whose device allocation fails (forced on the third batch by the test
+    hook), a request with an empty ROI and a cancelled request each fail
+    alone; every other request equals the one-volume call, and requests after
+    the failures are served."""
+    monkeypatch.setenv("BM_SERVER_FAIL_BATCH", "2")
+    big = _volumes(17, 3, 150, 170)
+    vols = _volumes(16, 7)
+    empty = dict(vols[3])
+    empty["roi"] = np.zeros_like(empty["roi"])
+    reqs = big + vols[:3] + [empty] + vols[3:]
+    srv = bm.Server(budget_bytes=8 << 30, max_batch=1, slots=1)
+    tickets = [srv.submit(v["bone"], v["roi"], v["w"], v["h"], v["d"], v["spacing"])
+               for v in reqs]
+    assert srv.cancel(tickets[-1])  # queued behind the large volumes
+    res = [srv.poll(t, wait=True)[0] for t in tickets]
+    st = srv.stats()
+    srv.close()
+    bufs = _upload(reqs)
+    ref = [_plain(it) for it in _items(reqs, bufs)]
+    for k, (g, r) in enumerate(zip(res, ref)):
+        if k == 2:
+            assert isinstance(g, bm.BmError) and g.code == _native.BM_ERR_DEVICE
+            assert "injected device allocation failure" in str(g)
+        elif k == len(reqs) - 1:
+            assert isinstance(g, bm.BmError) and str(g) == "request cancelled"
+        else:
+            assert _same(g, r), k
+    assert isinstance(res[6], bm.BmError) and "ROI mask contains no included voxels" in str(res[6])
+    assert st["requests_done"] == len(reqs) - 1  # the cancelled one never ran
+    with pytest.raises(bm.BmError, match="unknown ticket"):
+        s2 = bm.Server(budget_bytes=1 << 30, max_batch=1, slots=1)
+        try:
+            s2.cancel(12345)
+        finally:
+            s2.close()

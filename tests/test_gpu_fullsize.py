@@ -95,8 +95,8 @@ def test_large_analyze_matches_oracle(large):
     b = bm.BinaryVolume(wl.bone, wl.width, wl.height, wl.depth)
     r = bm.BinaryVolume(wl.roi, wl.width, wl.height, wl.depth)
     got = bm.analyze_thickness_spacing(b, r, wl.spacing)
-    check_stats(got.thickness, gb["stats"])
-    check_stats(got.spacing, gm["stats"])
+    check_stats(got.thickness, gb["stats"], scale=8.0)
+    check_stats(got.spacing, gm["stats"], scale=8.0)
     assert got.bone_volume == gb["selected"] * 8.0 ** 3
     with bm.stats_order(bm.StatsOrder.Sequential):
         got = bm.analyze_thickness_spacing(b, r, wl.spacing)

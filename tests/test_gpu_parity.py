@@ -225,8 +225,8 @@ def test_analyze_large_radius_volume_matches_oracle():
                                         wl.spacing)
     got = bm.analyze_thickness_spacing(vol(wl.bone, wl.width, wl.height, wl.depth),
                                        vol(wl.roi, wl.width, wl.height, wl.depth), wl.spacing)
-    _check_stats(got.thickness, ref.thickness)
-    _check_stats(got.spacing, ref.spacing)
+    _check_stats(got.thickness, ref.thickness, scale=wl.spacing[0])
+    _check_stats(got.spacing, ref.spacing, scale=wl.spacing[0])
 
 
 def test_analyze_large_workload_generator_matches_oracle():
@@ -239,8 +239,8 @@ def test_analyze_large_workload_generator_matches_oracle():
     ref = orc.analyze_thickness_spacing(bone, roi, w, h, d, (8.0, 8.0, 8.0))
     got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d), (8.0, 8.0, 8.0))
     assert ref.spacing.maximum / 8.0 > 40.0
-    _check_stats(got.thickness, ref.thickness)
-    _check_stats(got.spacing, ref.spacing)
+    _check_stats(got.thickness, ref.thickness, scale=8.0)
+    _check_stats(got.spacing, ref.spacing, scale=8.0)
 
 
 def test_local_thickness_plain_small_matches_oracle():
@@ -266,8 +266,8 @@ def _close(a, b):
     return abs(a - b) <= REL * abs(b)
 
 
-def _check_stats(g, r, sequential=False):
-    check_stats(g, r, sequential)
+def _check_stats(g, r, sequential=False, scale=None):
+    check_stats(g, r, sequential, scale)
 
 
 @pytest.mark.parametrize("order", [bm.StatsOrder.Exact, bm.StatsOrder.Sequential])
@@ -282,8 +282,8 @@ def test_analyze_plain_small_matches_oracle(mode, order):
     ref = orc.analyze_thickness_spacing(wl.bone, wl.roi, wl.width, wl.height, wl.depth,
                                         wl.spacing, include_quantiles=q)
     seq = order == bm.StatsOrder.Sequential
-    _check_stats(got.thickness, ref.thickness, seq)
-    _check_stats(got.spacing, ref.spacing, seq)
+    _check_stats(got.thickness, ref.thickness, seq, wl.spacing[0])
+    _check_stats(got.spacing, ref.spacing, seq, wl.spacing[0])
     assert got.bone_volume == ref.bone_volume and got.total_volume == ref.total_volume
     assert got.bone_volume_fraction == ref.bone_volume_fraction
 
@@ -295,8 +295,8 @@ def test_analyze_cylinder_roi_matches_oracle():
     roi = synth.roi_cylinder(w, h, d, 70.0)
     ref = orc.analyze_thickness_spacing(bone, roi, w, h, d, (7.25, 7.25, 7.25))
     got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d), (7.25, 7.25, 7.25))
-    _check_stats(got.thickness, ref.thickness)
-    _check_stats(got.spacing, ref.spacing)
+    _check_stats(got.thickness, ref.thickness, scale=7.25)
+    _check_stats(got.spacing, ref.spacing, scale=7.25)
     with bm.stats_order(bm.StatsOrder.Sequential):
         got = bm.analyze_thickness_spacing(vol(bone, w, h, d), vol(roi, w, h, d),
                                            (7.25, 7.25, 7.25))
@@ -334,39 +334,36 @@ def test_fill_scatter_beyond_gather_radius_matches_oracle(axis, radius):
     assert got["fill_visits"] == ref["fill_visits"]
 
 
-def _cavity_volume(n, radius, seed):
-    """A spherical cavity of the given radius at the centre of an n^3 volume
-    plus small struts of the generator outside it (radii of a few voxels), as
-    the selected phase."""
+def test_fill_large_tube_cavity_matches_oracle():
+    """A tube-shaped cavity of radius 130.5 across a 12-voxel slab (sites
+    outside the tube only): the distance field does not change along z, so
+    the ridges are the digital disk's medial set in every slice, with radii
+    from 1 to above 128.  Balls up to radius 128 are gathered, the larger
+    ones scattered afterwards; both meet in every voxel near the axis."""
+    n, d = 264, 12
     c = (n - 1) / 2.0
-    z, y, x = np.ogrid[0:n, 0:n, 0:n]
-    sphere = ((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) <= radius * radius
-    struts = synth.volume(n, n, n, 2.0, 0.1, seed).reshape(n, n, n) != 0
-    return (sphere | struts).astype(np.uint8).reshape(-1)
-
-
-def test_fill_large_cavity_matches_oracle():
-    """A marrow cavity of radius 140 (balls above the gather's radius, so the
-    scatter pass adds them after the gather) among small struts."""
-    n = 296
-    a = _cavity_volume(n, 140.0, 31)
-    ref = orc.local_thickness_stages(a, n, n, n, 0)
-    assert int(ref["squared_edt"].max()) > 128 * 128
-    got = bm.local_thickness_stages(vol(a, n, n, n), bm.Phase.Foreground)
+    y, x = np.ogrid[0:n, 0:n]
+    disk = ((x - c) ** 2 + (y - c) ** 2) <= 130.5 ** 2
+    a = np.ascontiguousarray(np.broadcast_to(disk, (d, n, n)), np.uint8).reshape(-1)
+    ref = orc.local_thickness_stages(a, n, n, d, 0)
+    big = (ref["ridge_flags"] > 0) & (ref["squared_edt"] > 128 * 128)
+    assert 0 < int(big.sum()) < int(ref["ridge_count"])
+    got = bm.local_thickness_stages(vol(a, n, n, d), bm.Phase.Foreground)
     np.testing.assert_array_equal(got["ridge_flags"], ref["ridge_flags"])
     np.testing.assert_array_equal(got["propagated"], ref["propagated"])
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
+    assert got["fill_visits"] == ref["fill_visits"]
 
 
 @pytest.mark.parametrize("call", ["local_thickness", "stages", "analyze"])
 def test_ridge_radius_beyond_list_range_raises(call, monkeypatch):
     """Ridges whose r^2 the tile lists cannot hold (2^23 and above, lowered
-    here to 50 by the test override) fail every thickness path with
+    here to 12 by the test override) fail every thickness path with
     BM_ERR_UNSUPPORTED instead of returning maps built from cut radii."""
-    monkeypatch.setenv("BM_LIST_R2_LIMIT", "50")
+    monkeypatch.setenv("BM_LIST_R2_LIMIT", "12")
     w, h, d = 40, 36, 44
     a = _structure(5, (d, h, w), 0.85)
-    assert orc.edt_squared(a, w, h, d, 0).max() >= 50
+    assert orc.edt_squared(a, w, h, d, 0).max() >= 12
     with pytest.raises(bm.BmError, match="limit for ridge radii") as e:
         if call == "local_thickness":
             bm.local_thickness(vol(a, w, h, d), bm.Phase.Foreground)

@@ -156,7 +156,7 @@ def check_metrics(m, bone, roi, w, h, d, spacing, mode):
     ts = orc.analyze_thickness_spacing(bone, roi, w, h, d, spacing,
                                        include_quantiles=(mode == bm.FeatureMode.Standard))
     for got, ref in ((m.thickness, ts.thickness), (m.spacing, ts.spacing)):
-        check_stats(got, ref)
+        check_stats(got, ref, scale=spacing[0])
     return ts
 
 

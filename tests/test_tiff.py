@@ -238,3 +238,92 @@ def test_capacity_is_checked(tmp_path):
                                            dims)
     assert rc == _native.BM_ERR_INVALID
     assert _native.lib().bm_last_error() == b"output buffer holds 100 bytes, the volume has 11550 voxels"
+
+
+# --- crafted headers: sizes that a decoder must reject, not act on ---------------
+def _two_pages(path, **kw):
+    tw.write(path, [np.zeros((64, 64), np.uint8), np.zeros((64, 64), np.uint8)], **kw)
+
+
+@pytest.mark.parametrize("case", ["bigtiff_tile_product_wraps", "huge_tiles",
+                                  "tile_dims_beyond_32_bits", "zero_tile_width"])
+def test_crafted_tile_sizes_fail_to_decode(tmp_path, case):
+    """Tile dimensions whose product wraps to 0 in 64 bits (BigTIFF LONG8), tiles
+    of 2^20 x 2^20 on a 64 x 64 page, dimensions past 32 bits and a zero width:
+    each page fails with the decoding error instead of reading past a buffer
+    or allocating without bound."""
+    path = str(tmp_path / f"{case}.tif")
+    if case == "bigtiff_tile_product_wraps":
+        _two_pages(path, big=True, tile=(64, 64),
+                   extra={322: ("LONG8", [1 << 32]), 323: ("LONG8", [1 << 32])})
+    elif case == "huge_tiles":
+        _two_pages(path, tile=(64, 64), extra={322: ("LONG", [1 << 20]), 323: ("LONG", [1 << 20])})
+    elif case == "tile_dims_beyond_32_bits":
+        _two_pages(path, big=True, tile=(64, 64),
+                   extra={322: ("LONG8", [(1 << 33) + 64]), 323: ("LONG8", [64])})
+    else:
+        _two_pages(path, tile=(64, 64), extra={322: ("LONG", [0])})
+    assert err(bm.read_binary_tiff, path) == f"decoding page 1 of {path}"
+
+
+def test_page_beyond_the_decoding_limit_fails_to_decode(tmp_path):
+    """A page of 20000 x 20000 samples (400 MB, above the decoder's 256 MiB
+    buffer) declared over a one-byte strip: the decoding error, before any
+    page-sized buffer is allocated or written."""
+    path = str(tmp_path / "huge_page.tif")
+    tw.write(path, [np.zeros((2, 2), np.uint8), np.zeros((2, 2), np.uint8)],
+             extra={256: ("LONG", [20000]), 257: ("LONG", [20000]), 278: ("LONG", [20000])})
+    import ctypes
+    from bone_morphometry_amd import _native
+    dims = (ctypes.c_uint64 * 3)()
+    rc = _native.lib().bm_read_binary_tiff(os.fsencode(path), 0, 0, 0, None, 0, dims)
+    assert rc == _native.BM_OK and tuple(dims) == (20000, 20000, 2)
+    buf = np.empty(1, np.uint8)
+    rc = _native.lib().bm_read_binary_tiff(os.fsencode(path), 0, 0, 0, buf.ctypes.data,
+                                           (1 << 63), dims)  # a capacity the call must not use
+    assert rc == _native.BM_ERR_INVALID
+    assert _native.lib().bm_last_error().decode() == f"decoding page 1 of {path}"
+
+
+def test_capacity_product_does_not_wrap(tmp_path):
+    """Pages of 2^32-1 x 2^32-1 samples: page size times page count exceeds
+    2^64, and the capacity check compares without wrap-around."""
+    path = str(tmp_path / "wrap.tif")
+    tw.write(path, [np.zeros((2, 2), np.uint8)] * 3, big=True,
+             extra={256: ("LONG8", [0xFFFFFFFF]), 257: ("LONG8", [0xFFFFFFFF])})
+    import ctypes
+    from bone_morphometry_amd import _native
+    dims = (ctypes.c_uint64 * 3)()
+    buf = np.empty(16, np.uint8)
+    rc = _native.lib().bm_read_binary_tiff(os.fsencode(path), 0, 0, 0, buf.ctypes.data, buf.size,
+                                           dims)
+    assert rc == _native.BM_ERR_INVALID
+    assert _native.lib().bm_last_error() == (
+        b"output buffer holds 16 bytes, the volume has more than 2^64 voxels")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_truncated_and_corrupted_files_fail_cleanly(tmp_path, seed):
+    """Files cut short at random points and files with random bytes
+    overwritten (every compression): every read ends in a result or in one of
+    the reference's messages, never in a crash."""
+    rng = np.random.default_rng(seed)
+    vol = volume(seed, d=4, h=40, w=48)
+    comp = ["raw", "tiff_lzw", "tiff_adobe_deflate", "packbits", "tiff_deflate", "raw"][seed]
+    path = str(tmp_path / "src.tif")
+    pil_write(path, vol, compression=comp)
+    data = open(path, "rb").read()
+    for k in range(24):
+        b = bytearray(data)
+        if k % 2 == 0:
+            b = b[:int(rng.integers(8, len(b)))]
+        else:
+            for _ in range(int(rng.integers(1, 16))):
+                b[int(rng.integers(0, len(b)))] = int(rng.integers(0, 256))
+        p = str(tmp_path / f"case{k}.tif")
+        open(p, "wb").write(bytes(b))
+        try:
+            got = bm.read_binary_tiff(p)
+            assert got.data.size == got.width * got.height * got.depth
+        except bm.BmError as e:
+            assert str(e)
