@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbm.so")
+# BM_LIBRARY names another build of the same library (the host-sanitized
+# build of the CPU tests); by default the one built in this directory.
+LIB_PATH = os.environ.get("BM_LIBRARY") or os.path.join(HERE, "libbm.so")
 SYNTH_PATH = os.path.join(HERE, "libbm_synth.so")
 # The autotuner's cache shipped with the package (per architecture and shape
 # class); BM_TUNE_CACHE, when set, takes precedence.

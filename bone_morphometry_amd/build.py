@@ -77,6 +77,58 @@ def build_libbm(jobs=8):
     return out
 
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build of the whole
+# library (device code is built as usual: each -fsanitize= applies to the host
+# pass only).  Loaded through BM_LIBRARY with the sanitizer runtime preloaded,
+# it runs the CPU tests of the host code (the TIFF decoder, argument checks).
+SANITIZE_FLAGS = ("-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined "
+                  "-Xarch_host -fno-sanitize-recover=undefined "
+                  "-Xarch_host -fno-omit-frame-pointer -Xarch_host -shared-libasan").split()
+SANITIZED_LIB = os.path.join(HERE, "_build", "sanitized", "libbm.so")
+
+
+def sanitizer_runtime():
+    """The clang ASan runtime the sanitized library links against."""
+    hipcc = _hipcc()
+    r = subprocess.run([os.path.join(os.path.dirname(os.path.realpath(hipcc)), "..", "llvm",
+                                     "bin", "clang++"), "--print-file-name=libclang_rt.asan-x86_64.so"],
+                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    path = r.stdout.strip()
+    if r.returncode != 0 or not os.path.isabs(path) or not os.path.exists(path):
+        import glob
+        hits = glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so")
+        path = hits[0] if hits else ""
+    return path
+
+
+def build_sanitized(jobs=8):
+    hipcc = _hipcc()
+    objdir = os.path.join(HERE, "_build", "sanitized")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(objdir, src + ".o")
+        objs.append(obj)
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [hipcc] + HIPCC_FLAGS + SANITIZE_FLAGS + ["-O1"] + lang + ["-c", path, "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(cmd[-3])
+    if failed:
+        raise RuntimeError("hipcc (sanitized) failed on: " + ", ".join(failed))
+    tmp = SANITIZED_LIB + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + SANITIZE_FLAGS + objs +
+         ["-lz"], CSRC)
+    os.replace(tmp, SANITIZED_LIB)
+    return SANITIZED_LIB
+
+
 def build_synth():
     out = os.path.join(HERE, "libbm_synth.so")
     tmp = out + ".tmp"

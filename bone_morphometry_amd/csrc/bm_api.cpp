@@ -941,7 +941,7 @@ int bm_gaussian_kernel(double sigma, double* out, uint64_t cap, uint64_t* len) {
     if (!len || (!out && cap)) throw Error("null argument");
     const std::vector<double> k = gaussian_kernel(sigma);
     *len = k.size();
-    std::memcpy(out, k.data(), std::min<uint64_t>(cap, k.size()) * sizeof(double));
+    if (cap) std::memcpy(out, k.data(), std::min<uint64_t>(cap, k.size()) * sizeof(double));
   });
 }
 

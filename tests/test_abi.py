@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_version_and_empty_error():
     assert _native.lib().bm_version() == 100
-    assert _native.lib().bm_last_error() == b""
+    assert bm.require_isotropic_spacing((2.0, 2.0, 2.0)) == 2.0
+    assert _native.lib().bm_last_error() == b""  # a call that succeeds clears it
 
 
 # BinaryVolume::new, src/volume.rs:12-41
