@@ -26,6 +26,7 @@ EXPORTS = (
     "bm_euclidean_distance_transform_squared", "bm_local_thickness",
     "bm_analyze_thickness_spacing", "bm_euclidean_distance_transform_squared_device",
     "bm_local_thickness_device", "bm_analyze_thickness_spacing_device",
+    "bm_analyze_thickness_spacing_device_noalloc",
     "bm_local_thickness_stages", "bm_set_stats_order", "bm_profile_enable", "bm_profile_read",
     "bm_batch_create", "bm_batch_run", "bm_batch_device_bytes", "bm_batch_destroy",
     "bm_autotune", "bm_tuning_load",
@@ -148,6 +149,9 @@ def lib():
         L.bm_euclidean_distance_transform_squared_device.argtypes = [P, U64, U64, U64, I, P, VP]
         L.bm_local_thickness_device.argtypes = [P, U64, U64, U64, I, P, VP]
         L.bm_analyze_thickness_spacing_device.argtypes = [
+            P, P, U64, U64, U64, ctypes.POINTER(ctypes.c_double), I,
+            ctypes.POINTER(BmThicknessSpacing), VP]
+        L.bm_analyze_thickness_spacing_device_noalloc.argtypes = [
             P, P, U64, U64, U64, ctypes.POINTER(ctypes.c_double), I,
             ctypes.POINTER(BmThicknessSpacing), VP]
         L.bm_local_thickness_stages.argtypes = [VOL, I, P, P, P, P, ctypes.POINTER(U64),

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["bm_kernels.hip", "bm_tiles.hip", "bm_engine.hip", "bm_quantiles.hip", "bm_smooth.hip",
-               "bm_topology.hip",
+               "bm_topology.hip", "bm_noalloc.hip",
                "bm_tune.cpp", "bm_serve.cpp", "bm_tiff.cpp", "bm_api.cpp"]
 
 HIPCC_FLAGS = [

@@ -147,6 +147,22 @@ int bm_analyze_thickness_spacing_device(const uint8_t *d_bone, const uint8_t *d_
                                         const double voxel_spacing[3], int feature_mode,
                                         bm_thickness_spacing *out, void *stream);
 
+/* No counterpart in the reference: bm_analyze_thickness_spacing_device for
+ * callers that cannot allocate memory (code that must not call hipMalloc or
+ * hipHostMalloc: tight serving loops, harnesses that forbid allocation).
+ * Same arguments and results, bit for bit; every device buffer of the call
+ * lives in a device array of BM_NOALLOC_SCRATCH_BYTES that the library holds
+ * on each GPU it runs on until the process ends (about 9 million voxels, e.g.
+ * 208^3, fit; larger volumes fail with BM_ERR_UNSUPPORTED before any device
+ * work).  The statistics use BM_STATS_EXACT.  Calls on one GPU share that
+ * array: the library serialises them, and each returns when its results are
+ * complete. */
+#define BM_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int bm_analyze_thickness_spacing_device_noalloc(const uint8_t *d_bone, const uint8_t *d_roi,
+                                                uint64_t width, uint64_t height, uint64_t depth,
+                                                const double voxel_spacing[3], int feature_mode,
+                                                bm_thickness_spacing *out, void *stream);
+
 /* Intermediate products of local_thickness for parity tests and diagnostics
  * (no counterpart in the reference's public API; these are the values of
  * src/local_thickness.rs:17,20-26,28 before the clean-up).  Any output pointer
