@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-host-call", action="store_true",
                     help="skip timing the host-pointer entry point")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-fill-work", action="store_true",
+                    help="skip counting the fill work (two extra stage calls; profiling runs)")
     ap.add_argument("--cpu-runs", type=int, default=3,
                     help="runs of the CPU baseline (the median is reported)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -229,7 +231,7 @@ def plain_form(args, bm, native, hip, workloads, cpu_threads):
     alg_bytes = per_vox * wl.n * 2
     hbm_gbs = alg_bytes / (avg_ms * 1e-3) / 1e9 if per_vox else None
     traffic = read_traffic()
-    if name == "fill_gather":
+    if name == "fill_gather" and not args.no_fill_work:
         # The sphere fill is bound by vector/scalar instruction issue: its
         # algorithmic work is W_fill, the reference propagation's voxel visits,
         # against the vector lanes' rate at one 32-bit operation per visit.

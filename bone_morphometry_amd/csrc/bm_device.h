@@ -83,10 +83,14 @@ __device__ __forceinline__ void acc_value(float v, int lsb_exp, unsigned long lo
     flags |= 1u;  // outside the exact fixed-point range
     return;
   }
-  const u128 k = (u128)mant << shift;
+  // (the shifts are 64-bit ones: mant << shift < 2^48, and mant^2 < 2^48 is
+  // split at bit 64 - 2 shift; a 128-bit variable shift costs far more)
   cnt += 1;
-  s += k;
-  s2 += (u128)((unsigned long long)mant * mant) << (2 * shift);
+  s += (u128)((unsigned long long)mant << shift);
+  const unsigned long long q = (unsigned long long)mant * mant;
+  const int n = 2 * shift;
+  const unsigned long long lo = q << n, hi = n ? q >> (64 - n) : 0ull;
+  s2 += ((u128)hi << 64) | lo;
   mxb = max(mxb, bits);
 }
 

@@ -38,7 +38,7 @@ __global__ void k_reduce_diag(const ItemDesc*);
 __global__ void k_fill_overflow(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_scatter_big(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_work(const ItemDesc*);
-__global__ void k_cleanup_tiled(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_cleanup_stream(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_count_ridges(const TileMeta*, uint32_t, PhaseStatus*);
 __global__ void k_lists_to_flags(uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
                                  const uint32_t*, const TileMeta*, uint8_t*);
@@ -154,8 +154,9 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
       d.divs[i][3] = f.s2;
     }
   }
-  d.cblocks = lt ? std::min<uint32_t>(std::max<uint32_t>(round8((d.ntiles + 7) / 8), 8),
-                                      kStatsBlocks)
+  d.cblocks = lt ? std::min<uint32_t>(
+                       std::max<uint32_t>(round8(cleanup_work_items(d.W, d.H, d.D)), 8),
+                       kStatsBlocks)
                  : 0;
   d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 32768), 8), 1024) : 0;
   if (blocks) {
@@ -338,7 +339,7 @@ void Plan::launch(hipStream_t s) const {
               st(kStOverflow), nit);
     BM_LAUNCH("fill_scatter", k_fill_scatter_big, total_[kStOverflow], 256, 0, s, d_desc_,
               st(kStOverflow), nit);
-    BM_LAUNCH("cleanup_stats", k_cleanup_tiled, total_[kStCleanup], 512, 0, s, d_desc_,
+    BM_LAUNCH("cleanup_stats", k_cleanup_stream, total_[kStCleanup], 512, 0, s, d_desc_,
               st(kStCleanup), nit);
     BM_LAUNCH("stats_final", k_stats_final, nit, 256, 0, s, d_desc_);
     if (any_ & kItemQuantiles) {

@@ -66,6 +66,7 @@ __device__ __forceinline__ uint32_t site_bits8(BM_GLOBAL const uint8_t* __restri
 
 constexpr int kNoneL = -(1 << 20);
 constexpr int kNoneR = (1 << 21);
+constexpr uint32_t kXSegs = 4;  // rows of up to 2048 voxels take one sweep
 
 // Zeroes the status (and the counters the later stages accumulate into) of
 // each item; one block per item.  A kernel, not a memset: graph-safe.
@@ -96,76 +97,108 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t nw = (I.xblocks * blockDim.x) >> 6;
   const uint32_t nseg = (W + 511) / 512;
   uint32_t selcnt = 0, anysite = 0;
+  __shared__ int s_carry[4][kXSegs];  // per wavefront: nearest site right of each segment
+  // One segment (512 voxels, 8 per lane) of a row: each voxel's distance to
+  // the nearest site on its left (in lower lanes of the segment, else carryL)
+  // and on its right (in higher lanes, else carryR), as u16.
+  auto segment_out = [&](size_t rb, uint32_t x0, uint32_t bits, int& carryL, int carryR) {
+    const unsigned long long has = __ballot(bits != 0);
+    const int last = bits ? (int)(x0 + 31 - __clz(bits)) : kNoneL;
+    const int first = bits ? (int)(x0 + __ffs(bits) - 1) : kNoneR;
+    const unsigned long long lower = has & ((1ull << lane) - 1ull);
+    const int fromL = __shfl(last, lower ? 63 - __clzll(lower) : 0);
+    const int Lc = lower ? fromL : carryL;
+    const unsigned long long higher = lane == 63 ? 0ull : (has & (~0ull << (lane + 1)));
+    const int fromR = __shfl(first, higher ? __ffsll(higher) - 1 : 0);
+    const int Rc = higher ? fromR : carryR;
+    if (has) carryL = __shfl(last, 63 - __clzll(has));
+    if (x0 < W) {
+      uint16_t o[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int x = (int)x0 + j;
+        const uint32_t mL = bits & ((2u << j) - 1u);
+        const int L = mL ? (int)(x0 + 31 - __clz(mL)) : Lc;
+        const uint32_t mR = bits & (~0u << j);
+        const int R = mR ? (int)(x0 + __ffs(mR) - 1) : Rc;
+        const int dist = min(x - L, R - x);
+        o[j] = dist > 65534 ? (uint16_t)kInf16 : (uint16_t)dist;
+      }
+      auto p = out + rb + x0;
+      if (x0 + 8 <= W && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+        v4u32 w;
+        w.x = o[0] | ((uint32_t)o[1] << 16);
+        w.y = o[2] | ((uint32_t)o[3] << 16);
+        w.z = o[4] | ((uint32_t)o[5] << 16);
+        w.w = o[6] | ((uint32_t)o[7] << 16);
+        *(BM_GLOBAL v4u32*)(p) = w;
+      } else {
+        for (int j = 0; j < 8; ++j)
+          if (x0 + j < W) p[j] = o[j];
+      }
+    }
+  };
   for (uint32_t r = gw; r < rows; r += nw) {
     const size_t rb = (size_t)r * W;
     int carryL = kNoneL;
+    if (nseg > 1 && nseg <= kXSegs) {
+      // Rows of up to kXSegs segments: the nearest site right of each segment
+      // comes from a sweep from the right end that only reads (the first
+      // site of each segment, a wave-uniform value), then each voxel is
+      // written once; the second read of the row's bytes hits the cache.
+      int* carryR = s_carry[threadIdx.x >> 6];
+      int cr = kNoneR;
+#pragma unroll 1
+      for (uint32_t s = nseg; s-- > 0;) {
+        if (lane == 0) carryR[s] = cr;
+        uint32_t dummy = 0;
+        const uint32_t x0 = s * 512 + lane * 8;
+        const uint32_t bits = site_bits8(data, roi, want, rb, x0, W, dummy);
+        const unsigned long long has = __ballot(bits != 0);
+        const int first = bits ? (int)(x0 + __ffs(bits) - 1) : kNoneR;
+        if (has) cr = __builtin_amdgcn_readlane(first, __ffsll(has) - 1);
+      }
+      // (carryR is written and read by this wavefront only)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+      for (uint32_t s = 0; s < nseg; ++s) {
+        const uint32_t x0 = s * 512 + lane * 8;
+        const uint32_t bits = site_bits8(data, roi, want, rb, x0, W, selcnt);
+        anysite |= bits;
+        segment_out(rb, x0, bits, carryL, carryR[s]);
+      }
+      continue;
+    }
     for (uint32_t s = 0; s < nseg; ++s) {
       const uint32_t x0 = s * 512 + lane * 8;
       const uint32_t bits = site_bits8(data, roi, want, rb, x0, W, selcnt);
       anysite |= bits;
+      segment_out(rb, x0, bits, carryL, kNoneR);
+    }
+    if (nseg == 1) continue;
+    // Longer rows: sites right of each segment from a second sweep from the
+    // right end.
+    int carryR = kNoneR;
+    for (uint32_t s = nseg; s-- > 0;) {
+      const uint32_t x0 = s * 512 + lane * 8;
+      uint32_t dummy = 0;
+      const uint32_t bits = site_bits8(data, roi, want, rb, x0, W, dummy);
       const unsigned long long has = __ballot(bits != 0);
-      const int last = bits ? (int)(x0 + 31 - __clz(bits)) : kNoneL;
       const int first = bits ? (int)(x0 + __ffs(bits) - 1) : kNoneR;
-      const unsigned long long lower = has & ((1ull << lane) - 1ull);
-      const int fromL = __shfl(last, lower ? 63 - __clzll(lower) : 0);
-      const int Lc = lower ? fromL : carryL;
       const unsigned long long higher = lane == 63 ? 0ull : (has & (~0ull << (lane + 1)));
       const int fromR = __shfl(first, higher ? __ffsll(higher) - 1 : 0);
-      const int Rc = higher ? fromR : kNoneR;
-      if (has) carryL = __shfl(last, 63 - __clzll(has));
+      const int Rc = higher ? fromR : carryR;
+      if (has) carryR = __shfl(first, __ffsll(has) - 1);
       if (x0 < W) {
-        uint16_t o[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int x = (int)x0 + j;
-          const uint32_t mL = bits & ((2u << j) - 1u);
-          const int L = mL ? (int)(x0 + 31 - __clz(mL)) : Lc;
-          const uint32_t mR = bits & (~0u << j);
-          const int R = mR ? (int)(x0 + __ffs(mR) - 1) : Rc;
-          const int dist = min(x - L, R - x);
-          o[j] = dist > 65534 ? (uint16_t)kInf16 : (uint16_t)dist;
-        }
         auto p = out + rb + x0;
-        if (x0 + 8 <= W && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
-          uint4 v;
-          v.x = o[0] | ((uint32_t)o[1] << 16);
-          v.y = o[2] | ((uint32_t)o[3] << 16);
-          v.z = o[4] | ((uint32_t)o[5] << 16);
-          v.w = o[6] | ((uint32_t)o[7] << 16);
-          v4u32 w;
-          w.x = v.x;
-          w.y = v.y;
-          w.z = v.z;
-          w.w = v.w;
-          *(BM_GLOBAL v4u32*)(p) = w;
-        } else {
-          for (int j = 0; j < 8; ++j)
-            if (x0 + j < W) p[j] = o[j];
-        }
-      }
-    }
-    if (nseg > 1) {
-      // Sites right of each segment: second sweep from the right end.
-      int carryR = kNoneR;
-      for (uint32_t s = nseg; s-- > 0;) {
-        const uint32_t x0 = s * 512 + lane * 8;
-        uint32_t dummy = 0;
-        const uint32_t bits = site_bits8(data, roi, want, rb, x0, W, dummy);
-        const unsigned long long has = __ballot(bits != 0);
-        const int first = bits ? (int)(x0 + __ffs(bits) - 1) : kNoneR;
-        const unsigned long long higher = lane == 63 ? 0ull : (has & (~0ull << (lane + 1)));
-        const int fromR = __shfl(first, higher ? __ffsll(higher) - 1 : 0);
-        const int Rc = higher ? fromR : carryR;
-        if (has) carryR = __shfl(first, __ffsll(has) - 1);
-        if (x0 < W) {
-          auto p = out + rb + x0;
-          for (int j = 0; j < 8; ++j) {
-            if (x0 + j >= W) break;
-            const uint32_t mR = bits & (~0u << j);
-            if (mR) continue;  // a site inside the segment is already accounted for
-            const int dist = Rc - (int)(x0 + j);
-            if (dist <= 65534 && (uint32_t)dist < p[j]) p[j] = (uint16_t)dist;
-          }
+        for (int j = 0; j < 8; ++j) {
+          if (x0 + j >= W) break;
+          const uint32_t mR = bits & (~0u << j);
+          if (mR) continue;  // a site inside the segment is already accounted for
+          const int dist = Rc - (int)(x0 + j);
+          if (dist <= 65534 && (uint32_t)dist < p[j]) p[j] = (uint16_t)dist;
         }
       }
     }

@@ -981,20 +981,52 @@ k_lists_to_flags(uint32_t W, uint32_t H, uint32_t D, uint32_t TX, uint32_t TY, u
 }
 
 // ---------------------------------------------------------------------------
-// Clean-up + phase mask + statistics over 8x8x8 tiles (persistent blocks).
+// Clean-up + phase mask + statistics, streamed along z.  A work item is a
+// 64 x 8 (x, y) footprint over a chunk of kCleanZ planes; a block walks work
+// items (persistent, XCD-contiguous ranges) and, for each, a rolling window
+// of planes in LDS:
+//   plane s arrives: its 68 x 12 region of M (footprint + 2-voxel halo) and
+//     the bit array of its zero voxels;
+//   zxy(s): per row of the 66 x 10 one-halo region, the 3 x 3 in-plane
+//     dilation of the zeros (bit operations, one thread per 32-bit word);
+//     Z3(s-1) = zxy(s-2) | zxy(s-1) | zxy(s): the voxels of plane s-1 with a
+//     zero among their 26 neighbours (out-of-array voxels hold kOut, never
+//     zero: the array border is not background, src/local_thickness.rs:325);
+//   c(s-1): what each one-halo voxel of plane s-1 adds to a neighbour's mean
+//     (its diameter if it is positive, inside the array and not on the
+//     surface, else 0; src/local_thickness.rs:214-238, 255-268);
+//   output plane s-2: the diameter, or the f32 mean of the positive
+//     non-surface neighbours in NEIGHBORS_26 order (src/local_thickness.rs:
+//     243-277), the phase mask (:31-43), exact sums and the value histogram.
+// The rings (M 4 planes, Z3 2, c 3) are sized so that one iteration's output
+// phase may overlap the next iteration's loads: three barriers per 512
+// outputs, and every M value is read from HBM once per work item (halos
+// shared by neighbouring footprints come from L2).
 constexpr uint32_t kOut = 0xFFFFFFFFu;  // outside the array: never zero, never summed
+constexpr int kRegX = 68, kRegY = 12, kReg = kRegX * kRegY;  // footprint + 2-voxel halo
+constexpr int kHalX = 66, kHalY = 10, kHal = kHalX * kHalY;  // footprint + 1-voxel halo
+constexpr int kZWords = 30;  // 10 rows x 3 words of one-halo bits
 
-__global__ void __launch_bounds__(512, 6)
-k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
-                uint32_t nitems) {
-  __shared__ uint32_t s_m[12 * 12 * 12];  // propagated r^2 with a 2-voxel halo
-  __shared__ uint32_t s_zb[54];           // bit i set where halo value i is 0 (1728 bits)
-  __shared__ uint32_t s_dil[10 * 10];     // per row of the 1-halo box: zeros dilated by 3x3x3
-  __shared__ float s_c[10 * 10 * 10];     // per voxel of the 1-halo box: its part of a mean
-  __shared__ float s_diam[kDiamLds];      // diameters 2*sqrt(m) for small m
-  __shared__ uint32_t s_top[kTopBins];    // top-16-bit histogram of the values (quantiles)
-  uint32_t vb;
-  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+// kLdsDiam: every diameter of the item is in the LDS table (the item's
+// largest r^2 is below kDiamLds), so a lookup needs no bounds test.
+struct CleanupLds {
+  uint32_t m[4][kReg];       // M ring
+  unsigned long long zb[14];  // zero bits of the newest plane (816 used, padded)
+  uint32_t z3[2][kZWords];   // Z3 ring
+  float c[3][kHal];          // contribution ring
+  float diam[kDiamLds];      // diameters 2*sqrt(m) for small m
+  uint32_t top[kTopBins];    // top-16-bit histogram of the values (quantiles)
+};
+
+template <bool kLdsDiam>
+__device__ __forceinline__ void
+cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
+  auto& s_m = L.m;
+  auto& s_zb = L.zb;
+  auto& s_z3 = L.z3;
+  auto& s_c = L.c;
+  auto& s_diam = L.diam;
+  auto& s_top = L.top;
   BM_GLOBAL const uint32_t* __restrict__ M = gbl(I.h);
   BM_GLOBAL const unsigned long long* __restrict__ masks = gbl(I.masks);
   BM_GLOBAL const float* __restrict__ diam = gbl(I.diam);
@@ -1006,197 +1038,178 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   uint32_t* __restrict__ top16 = (I.flags & kItemQuantiles) ? I.top16 : nullptr;
   if (top16)
     for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_top[i] = 0;
+  if (threadIdx.x < 14) s_zb[threadIdx.x] = 0;  // (the padding word stays 0)
   unsigned long long cnt = 0;
   u128 s = 0, s2 = 0;
   uint32_t mxb = 0, flags = 0;
   const bool valid_phase = st->has_site != 0;
   const uint32_t nd = valid_phase ? min(st->max_d2 + 1, (uint32_t)kDiamLds) : 0u;
   for (uint32_t i = threadIdx.x; i < nd; i += blockDim.x) s_diam[i] = diam[i];
-  auto dia = [&](uint32_t m) { return m < (uint32_t)kDiamLds ? s_diam[m] : diam[m]; };
-  const uint32_t li = threadIdx.x;
-  const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
-  // The 12^3 halo region of a tile, 4 values per thread (1728 <= 4 * 512),
-  // is loaded one tile ahead into registers to hide the HBM latency.
-  uint32_t pre[4];
-  unsigned long long pre_sel = 0;
-  // When 12 planes of the volume span less than 4 GiB, a thread's voxels are
-  // at fixed 32-bit byte offsets from a tile's first voxel (map) and from its
-  // halo's first voxel (the 4 halo values; halo values past 1728 are never
-  // read).  Interior tiles then load and store with no index arithmetic.
-  const bool near = (unsigned long long)H * W * 12 * 4 < (1ull << 32);
-  uint32_t hoff[4], loff = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t i = min(threadIdx.x + 512u * k, 1727u);
-    hoff[k] = near ? 4 * (((i / 144) * H + (i / 12) % 12) * W + i % 12) : 0u;
-  }
-  if (near) loff = 4 * (((uint32_t)lz * H + (uint32_t)ly) * W + (uint32_t)lx);
-  auto fetch = [&](uint32_t tx, uint32_t ty, uint32_t tz) {
-    const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
-    pre_sel = masks[((size_t)(tz * TY + ty) * TX + tx) * 16 + (threadIdx.x >> 6)];
-    const bool interior = near && ox >= 2 && oy >= 2 && oz >= 2 && ox + 10 <= (int)W &&
-                          oy + 10 <= (int)H && oz + 10 <= (int)D;
-    if (interior) {
-      const char* hb = (const char*)(M + (((size_t)(oz - 2) * H + (oy - 2)) * W + (ox - 2)));
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        pre[k] = (k < 3 || threadIdx.x < 1728 - 3 * 512) ? *(const uint32_t*)(hb + hoff[k]) : kOut;
-      return;
-    }
-    // (the empty asm keeps the coordinates below from being hoisted out of the
-    // tile loop, which would spill them)
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t i = tid + 512u * k;
-      uint32_t v = kOut;
-      if (i < 1728) {
-        const int hx = (int)(i % 12), hy = (int)((i / 12) % 12), hz = (int)(i / 144);
-        const int gx = ox + hx - 2, gy = oy + hy - 2, gz = oz + hz - 2;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
-          v = M[((size_t)gz * H + gy) * W + gx];
-      }
-      pre[k] = v;
-    }
+  auto dia = [&](uint32_t m) {
+    return kLdsDiam ? s_diam[m] : m < (uint32_t)kDiamLds ? s_diam[m] : diam[m];
   };
-  // Block b walks positions j = b/8, b/8 + G/8, ... of its XCD's contiguous
-  // range of the tile order (G blocks, a multiple of 8), so neighbouring
-  // tiles share that XCD's L2.
-  const bool grouped = I.tile_order & 4u;
-  const uint32_t npos = tile_positions(TX, I.TY, I.TZ, grouped);
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const uint32_t nfx = (W + 63) / 64, nfy = (H + 7) / 8;
+  const uint32_t npos = valid_phase ? cleanup_work_items(W, H, D) : 0u;
+  // Block vb walks positions j = xbase + vb / 8, + G / 8, ... of its XCD's
+  // contiguous range (G blocks, a multiple of 8): the blocks of one XCD
+  // work on neighbouring footprints at once and share their halos in L2.
   const uint32_t per = (npos + kXcds - 1) / kXcds;
   const uint32_t xbase = (vb % kXcds) * per, xend = min(xbase + per, npos);
   const uint32_t jstep = I.cblocks / kXcds;
-  const TileWalk walk = walk_of(I, grouped);
-  // next position at or after j that holds a tile (uniform over the block)
-  auto next_tile = [&](uint32_t j, uint32_t& tx, uint32_t& ty, uint32_t& tz) {
-    while (j < xend && !walk.at(j, tx, ty, tz)) j += jstep;
-    return j;
-  };
-  uint32_t tx = 0, ty = 0, tz = 0, nx = 0, ny = 0, nz = 0;
-  uint32_t j = valid_phase ? next_tile(xbase + vb / kXcds, tx, ty, tz) : xend;
-  if (j < xend) fetch(tx, ty, tz);
-  while (j < xend) {
-    const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
-    const unsigned long long selw = pre_sel;
-    const bool full = ox + 8 <= (int)W && oy + 8 <= (int)H && oz + 8 <= (int)D;
-    const bool valid = full || (ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D);
-    const size_t tbase = ((size_t)oz * H + oy) * W + ox;  // the tile's first voxel
-    auto store = [&](float v) {
-      if (!map_out || !valid) return;
-      if (near) *(float*)((char*)(map_out + tbase) + loff) = v;
-      else map_out[tbase + ((size_t)lz * H + ly) * W + lx] = v;
+  // Region elements of this thread: e0 = t and e1 = t + 512 (< 816); one-halo
+  // elements t and t + 512 (< 660); output voxel (lane, wv) of the footprint.
+  const bool has1 = t + 512 < (uint32_t)kReg;
+  const bool hh1 = t + 512 < (uint32_t)kHal;
+  const int ox = (int)lane, oy = (int)wv;
+  __syncthreads();
+  for (uint32_t j = xbase + vb / kXcds; j < xend; j += jstep) {
+    const uint32_t fy = j % nfy, fx = (j / nfy) % nfx, zc = j / (nfy * nfx);
+    const int x0 = (int)fx * 64, y0 = (int)fy * 8;
+    const int za = (int)(zc * kCleanZ), zb = min(za + (int)kCleanZ, (int)D);
+    const int gx = x0 + ox, gy = y0 + oy;
+    const bool out_valid = gx < (int)W && gy < (int)H;
+    // (the empty asm keeps the element coordinates from being hoisted out of
+    // the loops and held in registers)
+    auto load = [&](uint32_t e, int z) -> uint32_t {
+      asm volatile("" : "+v"(e));
+      const int qx = x0 - 2 + (int)(e % kRegX), qy = y0 - 2 + (int)(e / kRegX);
+      if (z < 0 || z >= (int)D || qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H) return kOut;
+      return M[((size_t)z * H + qy) * W + qx];
     };
-    // (this barrier also ends every read of the previous tile's arrays)
-    const bool any_sel = __syncthreads_or(selw != 0);
-    if (any_sel) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t i = threadIdx.x + 512u * k;
-        if (i < 1728) s_m[i] = pre[k];
-        // zero bits of the halo from a ballot: 64 values per wavefront and k
-        const unsigned long long zb = __ballot(i < 1728 && pre[k] == 0);
-        const uint32_t w = 2 * (8 * k + (threadIdx.x >> 6));  // word holding bit i - lane
-        const uint32_t lane = threadIdx.x & 63;
-        if (lane < 2 && w + lane < 54) s_zb[w + lane] = lane ? (uint32_t)(zb >> 32) : (uint32_t)zb;
-      }
+    // Work items whose region lies inside the array in x and y (the common
+    // case): each element is at a fixed offset within a plane, so a load is
+    // one add; planes outside [0, D) read kOut.
+    const bool inner = x0 >= 2 && y0 >= 2 && x0 + 66 <= (int)W && y0 + 10 <= (int)H;
+    const size_t plane = (size_t)W * H;
+    uint32_t off0 = 0, off1 = 0;
+    if (inner) {
+      off0 = (uint32_t)((y0 - 2 + (int)(t / kRegX)) * (int)W + x0 - 2 + (int)(t % kRegX));
+      const uint32_t e1 = has1 ? t + 512 : t;
+      off1 = (uint32_t)((y0 - 2 + (int)(e1 / kRegX)) * (int)W + x0 - 2 + (int)(e1 % kRegX));
     }
-    const uint32_t jn = next_tile(j + jstep, nx, ny, nz);
-    if (jn < xend) fetch(nx, ny, nz);  // next tile, in flight meanwhile
-    if (!any_sel) {  // no selected voxel in this tile
-      store(0.0f);
-      j = jn;
-      tx = nx;
-      ty = ny;
-      tz = nz;
-      continue;
-    }
-    __syncthreads();
-    if (threadIdx.x < 500) {
-      // surface (pass 1, src/local_thickness.rs:214-238): M != 0 with a zero
-      // among the 26 neighbours; the neighbourhood test is a 3x3x3 dilation of
-      // the zero bits (out-of-array voxels hold kOut, never zero).  Five
-      // threads per row (rz, ry) of the 1-halo box, two voxels each; each
-      // dilates its row.  (The empty asm keeps the row's addresses from being
-      // hoisted out of the tile loop and spilled.)
-      uint32_t t5 = threadIdx.x;
-      asm volatile("" : "+v"(t5));
-      const uint32_t row = t5 / 5, x0 = 2 * (t5 - 5 * row) + 1;
-      const int ry = (int)(row % 10) + 1, rz = (int)(row / 10) + 1;
-      uint32_t dil = 0;
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy) {
-          const uint32_t b = 12u * (uint32_t)((rz + dz) * 12 + (ry + dy));  // row's first bit
-          const uint32_t w = b >> 5;
-          // bits b .. b+11 of the two words (a 32-bit funnel shift)
-          const uint32_t z =
-              __builtin_amdgcn_alignbit(s_zb[min(w + 1, 53u)], s_zb[w], b & 31) & 0xFFFu;
-          dil |= z | (z << 1) | (z >> 1);
+    auto load2 = [&](int z, uint32_t& v0, uint32_t& v1) {
+      if (inner) {
+        if (z < 0 || z >= (int)D) {
+          v0 = v1 = kOut;
+        } else {
+          BM_GLOBAL const uint32_t* pz = M + (size_t)z * plane;
+          v0 = pz[off0];
+          v1 = has1 ? pz[off1] : kOut;
         }
-      if (x0 == 1) s_dil[row] = dil;
-      // Also what each voxel of the row adds to the mean of a surface
-      // neighbour (pass 2, src/local_thickness.rs:243-277): its diameter when
-      // it is positive, inside the array and not on the surface, else 0
-      // (diameters of positive r^2 are positive).  Tabled once per tile, so
-      // the 26 reads of a mean below are independent of each other.
-#pragma unroll
-      for (uint32_t k = 0; k < 2; ++k) {
-        const uint32_t x = x0 + k;
-        const uint32_t v = s_m[(rz * 12 + ry) * 12 + x];
-        const bool nonzero = v != 0 && v != kOut;
-        const bool part = nonzero && !((dil >> x) & 1u);
-        s_c[row * 10 + (x - 1)] = part ? dia(part ? v : 0u) : 0.0f;
-      }
-    }
-    __syncthreads();
-    const bool sel = (selw >> (li & 63)) & 1ull;
-    float tv = 0.0f;
-    if (sel) {
-      const int c12 = (lx + 2) + 12 * (ly + 2) + 144 * (lz + 2);
-      const uint32_t m = s_m[c12];
-      // on the surface: nonzero with a zero among the 26 neighbours
-      const bool surface = m != 0 && ((s_dil[(lz + 1) * 10 + (ly + 1)] >> (lx + 2)) & 1u);
-      if (!surface) {
-        tv = dia(m);
       } else {
-        // pass 2 (src/local_thickness.rs:243-277): f32 mean of the positive,
-        // non-surface neighbours in NEIGHBORS_26 order
-        // (two groups of 13 reads in flight, kept apart so that the 26 do not
-        // all hold registers at once)
-        const int c10 = (lx + 1) + 10 * (ly + 1) + 100 * (lz + 1);
-        float sum = 0.0f;
-        uint32_t count = 0;
-#pragma unroll
-        for (int g = 0; g < 26; g += 13) {
-          float c[13];
-#pragma unroll
-          for (int k = 0; k < 13; ++k)
-            c[k] = s_c[c10 + kNb26[g + k][0] + 10 * kNb26[g + k][1] + 100 * kNb26[g + k][2]];
-          // (non-contributors hold +0.0f and the sum starts at +0.0f, so adding
-          // them leaves the f32 sum bit for bit as skipping them would)
-#pragma unroll
-          for (int k = 0; k < 13; ++k) {
-            sum += c[k];
-            count += c[k] > 0.0f ? 1u : 0u;
-          }
-          __builtin_amdgcn_sched_barrier(0);
-        }
-        tv = count > 0 ? sum / (float)count : dia(m);
+        v0 = load(t, z);
+        v1 = has1 ? load(t + 512, z) : kOut;
       }
-      if (tv > 0.0f && tv < INFINITY) acc_value(tv * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
+    };
+    uint32_t pre0, pre1;
+    load2(za - 2, pre0, pre1);
+    uint32_t zxy_a = 0, zxy_b = 0;  // zxy(s-2), zxy(s-1) of this thread's word (t < 30)
+    for (int sp = za - 2; sp <= zb + 1; ++sp) {
+      const int rel = sp - (za - 2);
+      // plane sp: M values and zero bits
+      uint32_t* sm = s_m[rel & 3];
+      sm[t] = pre0;
+      if (has1) sm[t + 512] = pre1;
+      {
+        const unsigned long long b0 = __ballot(pre0 == 0u);
+        const unsigned long long b1 = __ballot(has1 && pre1 == 0u);
+        if (lane == 0) {
+          s_zb[wv] = b0;
+          if (wv < 5) s_zb[8 + wv] = b1;  // words 8..12 (elements 512..831)
+        }
+      }
+      if (sp + 1 <= zb + 1) load2(sp + 1, pre0, pre1);  // the next plane, in flight meanwhile
+      __syncthreads();
+      if (t < (uint32_t)kZWords) {
+        // zxy(sp): bit hx of word k of row hy = a zero among region x in
+        // [32k + hx', +2], rows hy .. hy+2 (hx' = bit index)
+        const uint32_t hy = t / 3, k = t % 3;
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t dy = 0; dy < 3; ++dy) {
+          const uint32_t p = 68u * (hy + dy) + 32u * k;  // first bit
+          const unsigned long long lo = s_zb[p >> 6], hi = s_zb[(p >> 6) + 1];
+          const uint32_t sh = p & 63;
+          const unsigned long long v = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+          const uint32_t v32 = (uint32_t)v, v33 = (uint32_t)(v >> 1), v34 = (uint32_t)(v >> 2);
+          acc |= v32 | v33 | v34;
+        }
+        if (k == 2) acc &= 3u;  // one-halo x < 66
+        const uint32_t z3 = zxy_a | zxy_b | acc;  // Z3(sp - 1)
+        s_z3[(rel + 1) & 1][t] = z3;              // slot of plane sp - 1
+        zxy_a = zxy_b;
+        zxy_b = acc;
+      }
+      __syncthreads();
+      if (sp - 1 >= za - 1) {
+        // c(sp - 1) over the one-halo region
+        const uint32_t* m1 = s_m[(rel - 1) & 3];
+        const uint32_t* z3 = s_z3[(rel + 1) & 1];
+        float* c1 = s_c[(rel + 2) % 3];  // slot of plane sp - 1
+        auto contrib = [&](uint32_t e) {
+          asm volatile("" : "+v"(e));
+          const int hx = (int)(e % kHalX), hy = (int)(e / kHalX);
+          const uint32_t v = m1[(hy + 1) * kRegX + hx + 1];
+          const bool surf = (z3[hy * 3 + (hx >> 5)] >> (hx & 31)) & 1u;
+          const bool part = v != 0 && v != kOut && !surf;
+          return part ? dia(part ? v : 0u) : 0.0f;
+        };
+        c1[t] = contrib(t);
+        if (hh1) c1[t + 512] = contrib(t + 512);
+      }
+      __syncthreads();
+      const int pz = sp - 2;  // output plane
+      if (pz < za) continue;
+      // selected voxels of the plane (this row of the footprint: 8 tiles)
+      const unsigned long long selw =
+          out_valid ? masks[((size_t)((uint32_t)pz / 8 * TY + (uint32_t)gy / 8) * TX +
+                             (uint32_t)gx / 8) * 16 + ((uint32_t)pz & 7)]
+                    : 0ull;
+      const bool sel = (selw >> (((uint32_t)gx & 7) + 8 * ((uint32_t)gy & 7))) & 1ull;
+      const uint32_t m = s_m[(rel - 2) & 3][(oy + 2) * kRegX + ox + 2];
+      float tv = 0.0f;
+      if (sel) {
+        const uint32_t* z3p = s_z3[rel & 1];  // slot of plane sp - 2
+        const int hx = ox + 1, hy = oy + 1;
+        const bool surface = m != 0 && ((z3p[hy * 3 + (hx >> 5)] >> (hx & 31)) & 1u);
+        if (!surface) {
+          tv = dia(m);
+        } else {
+          // pass 2 (src/local_thickness.rs:243-277): f32 mean of the positive,
+          // non-surface neighbours in NEIGHBORS_26 order (two groups of 13
+          // reads in flight; non-contributors hold +0.0f, which leaves the f32
+          // sum bit for bit as skipping them would)
+          const float* cz[3] = {s_c[(rel + 0) % 3], s_c[(rel + 1) % 3], s_c[(rel + 2) % 3]};
+          const int c0 = hy * kHalX + hx;
+          float sum = 0.0f;
+          uint32_t count = 0;
+#pragma unroll
+          for (int g = 0; g < 26; g += 13) {
+            float c[13];
+#pragma unroll
+            for (int k = 0; k < 13; ++k)
+              c[k] = cz[1 + kNb26[g + k][2]][c0 + kNb26[g + k][0] + kHalX * kNb26[g + k][1]];
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+              sum += c[k];
+              count += c[k] > 0.0f ? 1u : 0u;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          tv = count > 0 ? sum / (float)count : dia(m);
+        }
+        if (tv > 0.0f && tv < INFINITY) acc_value(tv * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
+      }
+      if (top16) {
+        const uint32_t bin = (__float_as_uint(tv) >> 16) - kTopBase;
+        const bool in = tv > 0.0f && bin < (uint32_t)kTopBins;
+        if (tv > 0.0f && !in) flags |= 2u;  // outside the histogram's range
+        wave_hist_add_runs(s_top, in, in ? bin : 0u);
+      }
+      if (map_out && out_valid) map_out[((size_t)pz * H + gy) * W + gx] = tv;
     }
-    if (top16) {
-      const uint32_t bin = (__float_as_uint(tv) >> 16) - kTopBase;
-      const bool in = tv > 0.0f && bin < (uint32_t)kTopBins;
-      if (tv > 0.0f && !in) flags |= 2u;  // outside the histogram's range
-      wave_hist_add_runs(s_top, in, in ? bin : 0u);
-    }
-    store(tv);
-    j = jn;
-    tx = nx;
-    ty = ny;
-    tz = nz;
+    __syncthreads();  // the rings are reused by the next work item
   }
   if (top16) {
     __syncthreads();
@@ -1204,6 +1217,16 @@ k_cleanup_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       if (s_top[i]) atomicAdd(&top16[i], s_top[i]);
   }
   block_reduce_store(cnt, s, s2, mxb, flags, I.partials + vb);
+}
+
+__global__ void __launch_bounds__(512, 6)
+k_cleanup_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                 uint32_t nitems) {
+  __shared__ CleanupLds lds;
+  uint32_t vb;
+  const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
+  if (gbl(I.status)->max_d2 < (uint32_t)kDiamLds) cleanup_stream<true>(I, vb, lds);
+  else cleanup_stream<false>(I, vb, lds);
 }
 
 }  // namespace bm
