@@ -30,7 +30,7 @@ __global__ void k_fill_u8(uint8_t*, uint64_t, uint8_t);
 __global__ void k_phase_inside(const uint8_t*, const uint8_t*, uint64_t, uint32_t, uint8_t*);
 __global__ void k_collect(const ItemDesc*, ItemResult*);
 // Tile kernels (bm_tiles.hip).
-__global__ void k_ridges_tiled(const ItemDesc*, const uint32_t*, uint32_t);
+__global__ void k_ridges_stream(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather_diag(const ItemDesc*, const uint32_t*, uint32_t);
 __global__ void k_fill_gather2(const ItemDesc*, const uint32_t*, uint32_t);
@@ -165,7 +165,7 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
     blocks[kStEdtZ] = d.zgx * d.zgy * d.zgz;
     blocks[kStLuts] = d.lblocks;
     // one block per position of the ridge and fill orders (the larger of the two)
-    blocks[kStRidges] = lt ? round8(tile_positions(d.TX, d.TY, d.TZ, t.tile_order & 1u)) : 0;
+    blocks[kStRidges] = lt ? round8(ridge_work_items(d.W, d.H, d.D)) : 0;
     blocks[kStTiles] = !lt ? 0
                        : fill_tiles == 2
                            ? round8((uint64_t)((d.TX + 1) / 2) * d.TY * d.TZ)
@@ -323,7 +323,7 @@ void Plan::launch(hipStream_t s) const {
   BM_LAUNCH("reduce_status", k_reduce_status, nit, 1024, 0, s, d_desc_);
   if (any_ & kItemThickness) {
     BM_LAUNCH("luts", k_luts, total_[kStLuts], 256, 0, s, d_desc_, st(kStLuts), nit);
-    BM_LAUNCH("ridges", k_ridges_tiled, total_[kStRidges], 512, 0, s, d_desc_, st(kStRidges),
+    BM_LAUNCH("ridges", k_ridges_stream, total_[kStRidges], 512, 0, s, d_desc_, st(kStRidges),
               nit);
     if (any_ & kItemDiag)
       BM_LAUNCH("fill_gather", k_fill_gather_diag, total_[kStTiles], 512, 0, s, d_desc_,

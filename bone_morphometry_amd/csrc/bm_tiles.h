@@ -42,6 +42,13 @@ inline __host__ __device__ uint32_t kPrefR2(int j) { return (uint32_t)(2 * j + 1
 
 inline uint32_t tiles_of(uint32_t n) { return (n + 7) / 8; }
 
+// Work items of the ridge pass: 64 x 8 (x, y) footprints (8 tiles along x)
+// over chunks of kRidgeZ planes (whole tile layers).
+constexpr uint32_t kRidgeZ = 64;
+inline __host__ __device__ uint32_t ridge_work_items(uint32_t W, uint32_t H, uint32_t D) {
+  return ((W + 63) / 64) * ((H + 7) / 8) * ((D + kRidgeZ - 1) / kRidgeZ);
+}
+
 // Work items of the clean-up: 64 x 8 (x, y) footprints over chunks of
 // kCleanZ planes.
 constexpr uint32_t kCleanZ = 64;

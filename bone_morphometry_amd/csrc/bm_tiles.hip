@@ -1,15 +1,17 @@
 // bm_tiles.hip -- tile kernels of local thickness: ridges, sphere fill, clean-up.
 //
-// The volume is cut into 8x8x8 tiles (one 512-thread workgroup per tile, one
-// wavefront per 8x8 z-plane or per 4x4x4 sub-brick).
+// The volume is cut into 8x8x8 tiles.  The ridge and clean-up passes stream
+// 64 x 8 (x, y) footprints along z (rolling windows of planes in LDS); the
+// sphere fill works per tile (one 512-thread workgroup, one wavefront per
+// 4x4x4 sub-brick).
 //
-// Ridges (src/local_thickness.rs:104-167).  A tile stages its squared EDT with
-// a 1-voxel halo in LDS and applies the 26-neighbour template rule.  Each
-// tile's ridges are written as a list sorted by decreasing r^2 (counting sort
-// on r^2 >> shift), entry = r^2 << 9 | local index, at a fixed slot of 512
-// entries per tile.  Two bit masks per tile are written too: the selected
-// voxels (d2 > 0) and the fill targets (selected voxels and non-selected voxels
-// with a selected face neighbour).
+// Ridges (src/local_thickness.rs:104-167).  Each voxel's 26 neighbours are
+// tested against the templates of its squared distance.  Each tile's ridges
+// are written as a list sorted by decreasing r^2 (counting sort on r^2 >>
+// shift), entry = r^2 << 9 | local index, at a fixed slot of 512 entries per
+// tile.  Two bit masks per tile are written too: the selected voxels (d2 > 0)
+// and the fill targets (selected voxels and non-selected voxels with a
+// selected face neighbour).
 //
 // Sphere fill (src/local_thickness.rs:169-208).  M[v] = max r^2(c) over ridges
 // c with |v - c|^2 <= r^2(c).  Instead of scattering every ball, each tile
@@ -27,12 +29,12 @@
 //     i.e. c = s -- impossible.  If s has a selected face neighbour u, then
 //     d2(u) = 1 and the unit ball of u contains s, so M[s] >= 1.
 // Hence targets are exactly the voxels with M > 0, every target is covered by
-// some candidate, and all other voxels have M = 0.
+// some candidate, and all other voxels have M = 0.  Balls of radius above
+// kGatherMaxRadius are scattered after the gather (k_fill_scatter_big).
 //
 // Clean-up (src/local_thickness.rs:210-281) + phase mask (:31-43) + the value
-// part of nonzero_stats (src/metrics.rs:215-226): a tile of M with a 2-voxel
-// halo in LDS; surface marks for the 1-halo region, then the f32 neighbour
-// means in NEIGHBORS_26 order.
+// part of nonzero_stats (src/metrics.rs:215-226): surface marks from the zero
+// voxels of three planes, then the f32 neighbour means in NEIGHBORS_26 order.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -61,154 +63,215 @@ __device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
 }
 
 // ---------------------------------------------------------------------------
+// Ridges streamed along z (src/local_thickness.rs:104-167).  A work item is a 64 x 8 (x, y) footprint --
+// eight tiles along x -- over kRidgeZ planes (whole tile layers).  Planes of
+// d^2 with a one-voxel halo pass through a ring of four LDS slots: while
+// plane s is evaluated against planes s-1 and s+1, plane s+2 is in flight and
+// plane s+1's successor takes the free slot, so one barrier per plane.  A
+// ridge voxel's entry (r^2 << 9 | local index) is appended to its tile's LDS
+// list (one LDS atomic per eight-lane group) and counted in the tile's bucket
+// histogram; when a tile layer is complete, wavefront k sorts tile k's
+// entries by bucket into the tile's list slot and writes its header, and the
+// tile masks of each plane are written once the plane is done.
 __global__ void __launch_bounds__(512)
-k_ridges_tiled(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
-               uint32_t nitems) {
-  // The 10^3 block with rows 40 words apart and planes 448: the 32 lanes of a
-  // half-wavefront (x 0..7, y 0..3 of one plane) then read 32 distinct LDS
-  // banks (y * 40 mod 32 = 8y), where rows of 10 words give 2-way conflicts.
-  constexpr int kRy = 40, kRz = 448;  // (448 = 7 * 64: plane offsets pair in ds_read2st64)
-  __shared__ uint32_t s_d2[kRz * 10];
-  __shared__ uint32_t s_cnt[kBuckets];
-  __shared__ uint32_t s_pref[16];
-  __shared__ uint32_t s_max, s_count;
+k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
+                uint32_t nitems) {
+  constexpr int kRX = 66, kRY = 10, kR = kRX * kRY;
+  __shared__ uint32_t s_d2[4][kR];
+  __shared__ uint32_t s_ent[8][512];
+  __shared__ uint32_t s_cnt[8][kBuckets];
+  __shared__ unsigned long long s_msk[2][8][2];  // plane parity, tile, (selected, target)
+  __shared__ uint32_t s_n[8], s_mx[8];
+  __shared__ uint16_t s_pref[8][16];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const uint32_t* __restrict__ d2 = gbl(I.d2);
-  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, TZ = I.TZ, bound = I.bound;
+  const uint32_t W = I.W, H = I.H, D = I.D, TX = I.TX, TY = I.TY, bound = I.bound;
   BM_GLOBAL const uint32_t* __restrict__ tmpl = gbl(I.tmpl);
   BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
   BM_GLOBAL uint32_t* __restrict__ lists = gbl(I.lists);
   BM_GLOBAL TileMeta* __restrict__ meta = gbl(I.meta);
   BM_GLOBAL unsigned long long* __restrict__ masks = gbl(I.masks);
   if (!st->has_site) return;
-  const bool grouped = I.tile_order & 1u;
-  const uint32_t npos = tile_positions(TX, TY, TZ, grouped);
+  const uint32_t nfx = (W + 63) / 64, nfy = (H + 7) / 8;
+  const uint32_t npos = ridge_work_items(W, H, D);
   const uint32_t j = xcd_tile(vb, npos);
   if (j >= npos) return;
-  uint32_t tx, ty, tz;
-  if (!walk_of(I, grouped).at(j, tx, ty, tz)) return;
-  const uint32_t t = (tz * TY + ty) * TX + tx;
-  const int ox = (int)tx * 8, oy = (int)ty * 8, oz = (int)tz * 8;
-  // stage the 10^3 block (tile + 1 halo): both of a thread's loads in flight
-  // before either is stored
-  auto halo_value = [&](uint32_t i) -> uint32_t {
-    const int hx = (int)(i % 10), hy = (int)((i / 10) % 10), hz = (int)(i / 100);
-    const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
-    uint32_t v = 0;  // outside the array: never >= a template, never selected
-    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)W && gy < (int)H && gz < (int)D)
-      v = d2[((size_t)gz * H + gy) * W + gx];
-    return v;
-  };
-  static_assert(2 * 512 >= 1000, "two staging loads per thread");
-  {
-    const uint32_t i0 = threadIdx.x, i1 = threadIdx.x + 512;
-    const uint32_t v0 = halo_value(i0);
-    const uint32_t v1 = i1 < 1000 ? halo_value(i1) : 0u;
-    auto lds_index = [](uint32_t i) { return i % 10 + kRy * ((i / 10) % 10) + kRz * (i / 100); };
-    s_d2[lds_index(i0)] = v0;
-    if (i1 < 1000) s_d2[lds_index(i1)] = v1;
-  }
-  static_assert(kBuckets == 512, "one bucket counter per thread");
-  s_cnt[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_max = 0;
-  __syncthreads();
-  const uint32_t li = threadIdx.x, lane = li & 63;
-  const int lx = (int)(li & 7), ly = (int)((li >> 3) & 7), lz = (int)(li >> 6);
-  const bool valid = ox + lx < (int)W && oy + ly < (int)H && oz + lz < (int)D;
-  const int c = (lx + 1) + kRy * (ly + 1) + kRz * (lz + 1);
-  const uint32_t r2 = valid ? s_d2[c] : 0;
-  const bool sel = r2 != 0;
-  // The 26 neighbours (always inside the staged 10^3 block) and the templates
-  // are read by every lane, selected or not (r2 = 0 reads the tables' first
-  // entries), so no branch separates the reads.
-  const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
-  uint32_t nb[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
-    nb[q] = q == 13 ? 0u : s_d2[c + dx + kRy * dy + kRz * dz];
-  }
-  // fill target: selected, or in the image with a selected face neighbour
-  const bool target = sel || (valid && (nb[4] | nb[10] | nb[12] | nb[14] | nb[16] | nb[22]) != 0);
-  // A ridge unless a neighbour's squared distance reaches the template of its
-  // offset class (1: face, 2: edge, 3: corner neighbour), i.e. unless the
-  // largest value of a class reaches that class's template.
-  uint32_t mx[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const int k = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
-    mx[k] = max(mx[k], nb[q]);
-  }
-  const bool ridge = sel && !(mx[1] >= t1 || mx[2] >= t2 || mx[3] >= t3);
-  const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
-  if (lane == 0) {
-    masks[(size_t)t * 16 + (li >> 6)] = msel;
-    masks[(size_t)t * 16 + 8 + (li >> 6)] = mtgt;
-  }
-  // largest ridge r^2 and bucket counts: LDS atomics (a tile holds few ridges)
+  // neighbouring footprints along y run on one XCD at once (shared halos)
+  const uint32_t fy = j % nfy, fx = (j / nfy) % nfx, zc = j / (nfy * nfx);
+  const int x0 = (int)fx * 64, y0 = (int)fy * 8, za = (int)(zc * kRidgeZ),
+            zb = min(za + (int)kRidgeZ, (int)D);
+  const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const bool valid = x0 + (int)lane < (int)W && y0 + (int)wv < (int)H;
   const uint32_t shift = bucket_shift(st->max_d2);  // one key for every tile
   const uint32_t top = st->max_d2 >> shift;
-  uint32_t bucket = 0;
-  if (ridge) {
-    bucket = top - (r2 >> shift);  // 0 = largest radii
-    atomicMax(&s_max, r2);
-    atomicAdd(&s_cnt[bucket], 1u);
+  for (uint32_t i = t; i < 8u * kBuckets; i += blockDim.x) (&s_cnt[0][0])[i] = 0;
+  if (t < 8) {
+    s_n[t] = 0;
+    s_mx[t] = 0;
   }
-  __syncthreads();
-  if (li < 64) {
-    // exclusive scan of the counts by one wavefront, 8 buckets per lane
-    uint32_t local[8], sum = 0;
+  const bool has1 = t + 512 < (uint32_t)kR;
+  // (the empty asm keeps the element coordinates from being hoisted and held)
+  auto load = [&](uint32_t e, int z) -> uint32_t {
+    asm volatile("" : "+v"(e));
+    const int qx = x0 - 1 + (int)(e % kRX), qy = y0 - 1 + (int)(e / kRX);
+    // outside the array: never >= a template, never selected
+    if (z < 0 || z >= (int)D || qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H) return 0u;
+    return d2[((size_t)z * H + qy) * W + qx];
+  };
+  s_d2[(za - 1) & 3][t] = load(t, za - 1);
+  if (has1) s_d2[(za - 1) & 3][t + 512] = load(t + 512, za - 1);
+  s_d2[za & 3][t] = load(t, za);
+  if (has1) s_d2[za & 3][t + 512] = load(t + 512, za);
+  uint32_t pre0 = load(t, za + 1), pre1 = has1 ? load(t + 512, za + 1) : 0u;
+  const uint32_t ci = (wv + 1) * kRX + lane + 1;  // this voxel in a plane slot
+  const uint32_t k = lane >> 3;                   // its tile in the footprint
+  auto write_masks = [&](int z) {                 // plane z's tile mask words
+    if (t < 16) {
+      const uint32_t kk = t >> 1, which = t & 1, tx = fx * 8 + kk;
+      if (tx < TX)
+        masks[((size_t)((uint32_t)z / 8 * TY + fy) * TX + tx) * 16 + which * 8 + ((uint32_t)z & 7)] =
+            s_msk[z & 1][kk][which];
+    }
+  };
+  for (int s = za; s < zb; ++s) {
+    s_d2[(s + 1) & 3][t] = pre0;
+    if (has1) s_d2[(s + 1) & 3][t + 512] = pre1;
+    if (s + 2 <= zb) {  // the plane after next, in flight meanwhile
+      pre0 = load(t, s + 2);
+      pre1 = has1 ? load(t + 512, s + 2) : 0u;
+    }
+    __syncthreads();
+    if (s > za) write_masks(s - 1);
+    const uint32_t* pm = s_d2[(s - 1) & 3];
+    const uint32_t* pc = s_d2[s & 3];
+    const uint32_t* pp = s_d2[(s + 1) & 3];
+    const uint32_t r2 = pc[ci];  // 0 outside the array
+    const bool sel = r2 != 0;
+    // the neighbours and templates are read by every lane, selected or not
+    // (r2 = 0 reads the tables' first entries), so no branch separates reads
+    const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
+    uint32_t nb[27];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      local[i] = s_cnt[lane * 8 + i];
-      sum += local[i];
+    for (int q = 0; q < 27; ++q) {
+      const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+      const uint32_t* pl = dz < 0 ? pm : dz > 0 ? pp : pc;
+      nb[q] = q == 13 ? 0u : pl[(int)ci + dx + kRX * dy];
     }
-    uint32_t x = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(x, off);
-      if (lane >= (uint32_t)off) x += y;
-    }
-    uint32_t run = x - sum;
+    // fill target: selected, or in the image with a selected face neighbour
+    const bool target =
+        sel || (valid && (nb[4] | nb[10] | nb[12] | nb[14] | nb[16] | nb[22]) != 0);
+    // A ridge unless a neighbour's squared distance reaches the template of
+    // its offset class (1: face, 2: edge, 3: corner neighbour).
+    uint32_t mx[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      s_cnt[lane * 8 + i] = run;
-      run += local[i];
+    for (int q = 0; q < 27; ++q) {
+      const int kc = (q % 3 != 1) + ((q / 3) % 3 != 1) + (q / 9 != 1);
+      mx[kc] = max(mx[kc], nb[q]);
     }
-    const uint32_t count = (uint32_t)__shfl((int)x, 63);
-    // (the starts written above are read by other lanes of this wavefront)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // prefix counts: entries with key >= kPrefR2(j) >> shift are the buckets
-    // 0 .. top - (kPrefR2(j) >> shift), i.e. the start of the next bucket
-    if (lane < 16) {
-      const uint32_t kj = kPrefR2((int)lane) >> shift;
-      uint32_t p = 0;
-      if (kj <= top) {
-        const uint32_t nb = top - kj + 1;  // first bucket below the threshold
-        p = nb < (uint32_t)kBuckets ? s_cnt[nb] : count;
+    const bool ridge = sel && !(mx[1] >= t1 || mx[2] >= t2 || mx[3] >= t3);
+    // mask bytes: row wv of each of the eight tiles (x + 8y within a plane)
+    const unsigned long long msel = __ballot(sel), mtgt = __ballot(target);
+    if (lane < 8) {
+      reinterpret_cast<uint8_t*>(&s_msk[s & 1][lane][0])[wv] = (uint8_t)(msel >> (8 * lane));
+      reinterpret_cast<uint8_t*>(&s_msk[s & 1][lane][1])[wv] = (uint8_t)(mtgt >> (8 * lane));
+    }
+    // the ridge's entry, appended to its tile's list
+    const unsigned long long rb = __ballot(ridge);
+    const uint32_t grp = (uint32_t)(rb >> (8 * k)) & 0xFFu;
+    uint32_t base = 0;
+    if ((lane & 7) == 0 && grp) base = atomicAdd(&s_n[k], (uint32_t)__popc(grp));
+    base = (uint32_t)__shfl((int)base, (int)(lane & ~7u));
+    if (ridge) {
+      const uint32_t li = (lane & 7) | (wv << 3) | (((uint32_t)s & 7) << 6);
+      s_ent[k][base + __popc(grp & ((1u << (lane & 7)) - 1u))] = (r2 << 9) | li;
+      atomicAdd(&s_cnt[k][top - (r2 >> shift)], 1u);
+      atomicMax(&s_mx[k], r2);
+    }
+    if ((s & 7) != 7 && s != zb - 1) continue;
+    // Tile layer complete: wavefront wv sorts tile wv.
+    __syncthreads();
+    {
+      const uint32_t kk = wv, tx = fx * 8 + kk;
+      if (tx < TX) {
+        const uint32_t tile = ((uint32_t)s / 8 * TY + fy) * TX + tx;
+        uint32_t* cnt = s_cnt[kk];
+        // exclusive scan of the counts, 8 buckets per lane
+        uint32_t local[8], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          local[i] = cnt[lane * 8 + i];
+          sum += local[i];
+        }
+        uint32_t x = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t y = __shfl_up(x, off);
+          if (lane >= (uint32_t)off) x += y;
+        }
+        uint32_t run = x - sum;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          cnt[lane * 8 + i] = run;
+          run += local[i];
+        }
+        const uint32_t count = (uint32_t)__shfl((int)x, 63);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // prefix counts: entries with key >= kPrefR2(j) >> shift are the
+        // buckets 0 .. top - (kPrefR2(j) >> shift): the next bucket's start
+        if (lane < 16) {
+          const uint32_t kj = kPrefR2((int)lane) >> shift;
+          uint32_t p = 0;
+          if (kj <= top) {
+            const uint32_t nbk = top - kj + 1;
+            p = nbk < (uint32_t)kBuckets ? cnt[nbk] : count;
+          }
+          s_pref[kk][lane] = (uint16_t)p;
+        }
+        const uint32_t mxr = s_mx[kk];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+          TileMeta m;
+          m.count = count;
+          m.maxr2 = mxr;
+          for (int q = 0; q < 16; ++q) m.pref[q] = s_pref[kk][q];
+          *(TileMeta*)&meta[tile] = m;
+          if (mxr >= I.r2_limit) atomicOr((uint32_t*)&st->error, 1u);
+          if (mxr > kGatherMaxR2) {  // balls the gather leaves to the scatter
+            BM_GLOBAL uint32_t* __restrict__ big = gbl(I.bigtiles);
+            big[1 + atomicAdd((uint32_t*)&big[0], 1u)] = tile;
+          }
+        }
+        // the entries at their sorted positions (any order inside a bucket)
+        for (uint32_t e = lane; e < count; e += 64) {
+          const uint32_t ent = s_ent[kk][e];
+          const uint32_t pos = atomicAdd(&cnt[top - ((ent >> 9) >> shift)], 1u);
+          lists[(size_t)tile * 512 + pos] = ent;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      s_pref[lane] = p;
+      // cleared for the next layer
+      for (uint32_t i = lane; i < (uint32_t)kBuckets; i += 64) s_cnt[kk][i] = 0;
+      if (lane == 0) {
+        s_n[kk] = 0;
+        s_mx[kk] = 0;
+      }
     }
-    if (lane == 0) s_count = count;
+    __syncthreads();
   }
   __syncthreads();
-  if (ridge) {
-    const uint32_t pos = atomicAdd(&s_cnt[bucket], 1u);
-    lists[(size_t)t * 512 + pos] = (r2 << 9) | li;
-  }
-  if (threadIdx.x == 0) {
-    TileMeta m;
-    m.count = s_count;
-    m.maxr2 = s_max;
-    for (int q = 0; q < 16; ++q) m.pref[q] = (uint16_t)s_pref[q];
-    *(TileMeta*)&meta[t] = m;
-    if (m.maxr2 >= I.r2_limit) atomicOr((uint32_t*)&st->error, 1u);
-    if (m.maxr2 > kGatherMaxR2) {  // balls the gather leaves to the scatter
-      BM_GLOBAL uint32_t* __restrict__ big = gbl(I.bigtiles);
-      big[1 + atomicAdd((uint32_t*)&big[0], 1u)] = t;
-    }
+  write_masks(zb - 1);
+  // planes of the last tile layer beyond the array: empty mask words
+  if (zb == (int)D && t < 16) {
+    const uint32_t kk = t >> 1, which = t & 1, tx = fx * 8 + kk;
+    if (tx < TX)
+      for (uint32_t z = D; z < (D + 7) / 8 * 8; ++z)
+        masks[((size_t)(z / 8 * TY + fy) * TX + tx) * 16 + which * 8 + (z & 7)] = 0ull;
   }
 }
 
