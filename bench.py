@@ -561,6 +561,7 @@ def serving_form(args, bm, workloads, pool, rate, seconds):
     # warm-up: one request of each of a few sizes
     for r in pool[:4]:
         srv.poll(srv.submit(r.bone, r.roi, r.width, r.height, r.depth, r.spacing), wait=True)
+    warm = srv.stats()  # subtracted below: the stream's own batches and busy time
     tickets = []
     done = []
     lock = threading.Condition()
@@ -605,6 +606,8 @@ def serving_form(args, bm, workloads, pool, rate, seconds):
     th.join()
     stats = srv.stats()
     srv.close()
+    for k in ("batches", "sum_batch", "busy_s"):
+        stats[k] -= warm[k]
     # latency = completion - arrival: the server's submit-to-done time plus
     # the time from the scheduled arrival to the submit call
     lat = np.array([tim["done_s"] - tim["submit_s"] + rq[1] for rq, _, tim in done]) * 1e3
