@@ -311,3 +311,47 @@ def test_server_failures_and_cancellation_leave_other_requests_unchanged(monkeyp
             s2.cancel(12345)
         finally:
             s2.close()
+
+
+def test_server_concurrent_clients_match_one_volume_calls():
+    """Four client threads submit, poll and cancel on one server at once (the
+    C calls release the GIL, so the server's queue, worker thread and result
+    table are driven concurrently): every request that was not cancelled
+    equals the one-volume call on its volume."""
+    import threading
+    vols = _volumes(18, 12)
+    srv = bm.Server(budget_bytes=4 << 30, max_batch=4, slots=2)
+    results = {}
+    errors = []
+
+    def client(k):
+        try:
+            mine = list(range(k, len(vols), 4))
+            tickets = [(i, srv.submit(vols[i]["bone"], vols[i]["roi"], vols[i]["w"],
+                                      vols[i]["h"], vols[i]["d"], vols[i]["spacing"]))
+                       for i in mine]
+            for i, t in tickets:
+                results[i] = srv.poll(t, wait=True)[0]
+            # a request cancelled right after its submission, and one after its result
+            v = vols[mine[0]]
+            t = srv.submit(v["bone"], v["roi"], v["w"], v["h"], v["d"], v["spacing"])
+            srv.cancel(t)
+            res = srv.poll(t, wait=True)[0]
+            if isinstance(res, bm.BmError):
+                assert str(res) == "request cancelled"
+            else:
+                assert _same(res, results[mine[0]])
+            srv.stats()
+        except BaseException as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    srv.close()
+    assert not errors, errors
+    bufs = _upload(vols)
+    ref = [_plain(it) for it in _items(vols, bufs)]
+    assert all(_same(results[i], ref[i]) for i in range(len(vols)))
