@@ -207,6 +207,33 @@ def test_ridge_set_is_the_template_rule(seed, frac):
             assert ridge
 
 
+def test_fill_visits_of_single_voxels():
+    """W_fill of one selected voxel: its ball r^2 = 1 is the voxel and its six
+    face neighbours, cut by the array at a corner (src/local_thickness.rs:182-201)."""
+    for pos, want in (((2, 2, 2), 7), ((0, 0, 0), 4), ((0, 2, 4), 5)):
+        v = np.zeros((5, 5, 5), np.uint8)
+        v[pos] = 1
+        st = orc.local_thickness_stages(v.reshape(-1), 5, 5, 5, 0)
+        assert st["ridge_count"] == 1 and st["fill_visits"] == want, pos
+
+
+@pytest.mark.parametrize("seed,frac", [(7, 0.3), (8, 0.75)])
+def test_fill_visits_equal_brute_force_ball_counts(seed, frac):
+    """W_fill = sum over ridge voxels of the lattice points within r^2 = d2 of
+    the centre inside the array, counted here over the whole grid."""
+    sel = _random_structure(seed, (11, 12, 13), frac)
+    d, h, w = sel.shape
+    st = orc.local_thickness_stages(sel.astype(np.uint8).reshape(-1), w, h, d, 0)
+    d2 = st["squared_edt"].reshape(d, h, w).astype(np.int64)
+    ridges = np.argwhere(st["ridge_flags"].reshape(d, h, w) != 0)
+    Z, Y, X = np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")
+    total = 0
+    for z, y, x in ridges:
+        total += int(((Z - z) ** 2 + (Y - y) ** 2 + (X - x) ** 2 <= d2[z, y, x]).sum())
+    assert st["ridge_count"] == len(ridges) > 0
+    assert st["fill_visits"] == total
+
+
 @pytest.mark.parametrize("seed,frac", [(5, 0.35), (6, 0.8)])
 def test_cleanup_is_order_independent_and_bounded(seed, frac):
     sel = _random_structure(seed, (14, 15, 16), frac)
