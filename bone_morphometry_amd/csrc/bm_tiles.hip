@@ -354,9 +354,14 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // a tile in reach: its index, and its offset (ix, iy, iz) in the reach box
   // (6 bits each) with its useful list prefix length - 1 above bit 18
   __shared__ uint32_t s_nb_tile[kNbr], s_nb_word[kNbr];
-  __shared__ uint16_t s_cmap[kCmap];            // chunk -> tile in reach
+  // chunk -> tile in reach; while the sub-bricks scan (and when the candidates
+  // are staged) it holds each wavefront's compacted candidates instead
+  __shared__ __attribute__((aligned(16))) uint16_t s_cmap[kCmap];
   __shared__ uint32_t s_off[kNbr];              // chunk offsets
-  __shared__ uint32_t s_spos[kStage], s_sr2[kStage];  // candidates in list order
+  // candidates in list order; when they are not staged, the wavefronts'
+  // compacted candidates
+  __shared__ __attribute__((aligned(16))) uint32_t s_spos[kStage];
+  __shared__ uint32_t s_sr2[kStage];
   __shared__ uint32_t s_nn, s_over, s_pending, s_nst, s_nchunks;
   __shared__ unsigned long long s_tgt[8 * TXN];
   uint32_t vb;
@@ -593,6 +598,12 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __syncthreads();
   const uint32_t nst = s_nst;
   const bool staged = nst <= (uint32_t)kStage;  // else the placement re-reads the lists
+  static_assert(sizeof(s_cmap) >= (size_t)NW * 64 * 8 && sizeof(s_spos) >= (size_t)NW * 64 * 8,
+                "per-wave candidate buffers");
+  // This wave's 64 (position, key) pairs of compacted candidates, in LDS that
+  // the scan does not otherwise use: the chunk map once the candidates are
+  // staged (the placement then reads the staging area), else the staging area.
+  uint2* const wbuf = (staged ? (uint2*)s_cmap : (uint2*)s_spos) + wv * 64;
   // Staged candidates, 64 per wave and trip (the trip count is uniform over a wave).
   auto staged_pass = [&](auto&& use) {
     for (uint32_t i0 = wv * 64; i0 < nst; i0 += blockDim.x) {
@@ -739,28 +750,30 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         unsigned long long rm = __ballot(rel);
         if (dotp) {
           if (rm) {
-            // Compact the chunk's relevant candidates into lanes 0 .. nrel-1
-            // with a forward permute (a bijection: the other lanes go after
-            // them and carry a key whose K = 0x7FFF no voxel reaches, since
-            // 2 v.c - |v|^2 < 2^15), then test them four per trip with
-            // uniform lane indices (at most 63: the last trip starts at or
-            // below 60).  The test is one dot product and a compare.
+            // Compact the chunk's relevant candidates into slots 0 .. nrel-1
+            // of this wave's LDS buffer (a bijection: the other lanes go
+            // after them and carry a key whose K = 0x7FFF no voxel reaches,
+            // since 2 v.c - |v|^2 < 2^15), then test them four per trip,
+            // each pair read by every lane from the same address (one
+            // broadcast 16-byte LDS read per two candidates; the last trip
+            // ends at slot 63 at most).  The test is one dot product and a
+            // compare.  The wave's LDS operations complete in order, so the
+            // reads see this chunk's writes; the previous chunk's reads are
+            // complete before the writes, as their values were used.
             const uint32_t nrel = (uint32_t)__popcll(rm);
             const uint32_t below = (uint32_t)__builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(rm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
             const uint32_t dst = rel ? below : nrel + (lane - below);
-            const uint32_t pk_c =
-                (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), rel ? (int)cp : 0);
-            const uint32_t key_c =
-                (uint32_t)__builtin_amdgcn_ds_permute((int)(dst << 2), rel ? (int)cr : 0x7FFF);
+            wbuf[dst] = make_uint2(rel ? cp : 0u, rel ? cr : 0x7FFFu);
+            const uint4* const wq = (const uint4*)wbuf;
             for (uint32_t i = 0; i < nrel; i += 4) {
+              const uint4 q0 = wq[i >> 1], q1 = wq[(i >> 1) + 1];
+              const uint32_t pk[4] = {q0.x, q0.z, q1.x, q1.z}, key[4] = {q0.y, q0.w, q1.y, q1.w};
 #pragma unroll
-              for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)pk_c, (int)(i + u));
-                const uint32_t key = (uint32_t)__builtin_amdgcn_readlane((int)key_c, (int)(i + u));
-                if ((int)__builtin_amdgcn_udot4(v2pk, pk, nega, false) >=
-                    (int)(int16_t)(key & 0xFFFFu))
-                  m = max(m, key);
+              for (int u = 0; u < 4; ++u) {
+                if ((int)__builtin_amdgcn_udot4(v2pk, pk[u], nega, false) >=
+                    (int)(int16_t)(key[u] & 0xFFFFu))
+                  m = max(m, key[u]);
               }
             }
             tests += nrel;
