@@ -358,10 +358,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // are staged) it holds each wavefront's compacted candidates instead
   __shared__ __attribute__((aligned(16))) uint16_t s_cmap[kCmap];
   __shared__ uint32_t s_off[kNbr];              // chunk offsets
-  // candidates in list order; when they are not staged, the wavefronts'
-  // compacted candidates
-  __shared__ __attribute__((aligned(16))) uint32_t s_spos[kStage];
-  __shared__ uint32_t s_sr2[kStage];
+  // candidates in list order (kStage position words, then kStage r^2 words;
+  // or 2 kStage one-word candidates); when they are not staged, the
+  // wavefronts' compacted candidates
+  __shared__ __attribute__((aligned(16))) uint32_t s_stage[2 * kStage];
   __shared__ uint32_t s_nn, s_over, s_pending, s_nst, s_nchunks;
   __shared__ unsigned long long s_tgt[8 * TXN];
   uint32_t vb;
@@ -576,6 +576,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     if (mapped) chunk_trips(std::true_type{}, use);
     else chunk_trips(std::false_type{}, use);
   };
+  // Staging format.  With R <= 45 a candidate is one word: r^2 (below 2048)
+  // above bit 21 and its block-relative centre c + R in three 7-bit fields
+  // (c lies in [-R, 8 * TXN - 1 + R], so c + R <= 16 + 90 - 1 < 128); the
+  // staging area then holds 2 kStage candidates.  Otherwise a position word
+  // and an r^2 word each.
+  const bool compact = R <= 45;
+  const uint32_t stage_cap = compact ? 2u * (uint32_t)kStage : (uint32_t)kStage;
   // P2: histogram by bucket, one LDS atomic per kept candidate (the LDS is
   // otherwise idle here, the vector units are not); the candidates are also
   // staged in LDS (one slot reservation per chunk) so that the placement
@@ -589,27 +596,41 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
       if (lane == 0) k0 = atomicAdd(&s_nst, (uint32_t)__popcll(km));
       k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)k0);
       const uint32_t k = k0 + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
-      if (keep && k < (uint32_t)kStage) {
-        s_spos[k] = pos;
-        s_sr2[k] = r2;
+      if (keep && k < stage_cap) {
+        if (compact) {
+          s_stage[k] = (r2 << 21) | (((pos >> 20) - 512u + (uint32_t)R) << 14) |
+                       ((((pos >> 10) & 1023u) - 512u + (uint32_t)R) << 7) |
+                       ((pos & 1023u) - 512u + (uint32_t)R);
+        } else {
+          s_stage[k] = pos;
+          s_stage[kStage + k] = r2;
+        }
       }
     }
   });
   __syncthreads();
   const uint32_t nst = s_nst;
-  const bool staged = nst <= (uint32_t)kStage;  // else the placement re-reads the lists
-  static_assert(sizeof(s_cmap) >= (size_t)NW * 64 * 8 && sizeof(s_spos) >= (size_t)NW * 64 * 8,
+  const bool staged = nst <= stage_cap;  // else the placement re-reads the lists
+  static_assert(sizeof(s_cmap) >= (size_t)NW * 64 * 8 && sizeof(s_stage) >= (size_t)NW * 64 * 8,
                 "per-wave candidate buffers");
   // This wave's 64 (position, key) pairs of compacted candidates, in LDS that
   // the scan does not otherwise use: the chunk map once the candidates are
   // staged (the placement then reads the staging area), else the staging area.
-  uint2* const wbuf = (staged ? (uint2*)s_cmap : (uint2*)s_spos) + wv * 64;
+  uint2* const wbuf = (staged ? (uint2*)s_cmap : (uint2*)s_stage) + wv * 64;
   // Staged candidates, 64 per wave and trip (the trip count is uniform over a wave).
   auto staged_pass = [&](auto&& use) {
     for (uint32_t i0 = wv * 64; i0 < nst; i0 += blockDim.x) {
       const uint32_t i = i0 + lane;
       const bool keep = i < nst;
-      use(keep, keep ? s_spos[i] : 0u, keep ? s_sr2[i] : 0u);
+      if (compact) {
+        const uint32_t w = keep ? s_stage[i] : 0u;
+        const uint32_t off = 512u - (uint32_t)R;
+        use(keep,
+            ((w & 127u) + off) | ((((w >> 7) & 127u) + off) << 10) | ((((w >> 14) & 127u) + off) << 20),
+            w >> 21);
+      } else {
+        use(keep, keep ? s_stage[i] : 0u, keep ? s_stage[kStage + i] : 0u);
+      }
     }
   };
   // bucket starts (and the placement cursors), scanned by one wavefront
