@@ -588,9 +588,9 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // staged in LDS (one slot reservation per chunk) so that the placement
   // passes need not read the lists again.
   chunk_pass([&](bool keep, uint32_t pos, uint32_t r2) {
+    const unsigned long long km = __builtin_amdgcn_ballot_w64(keep);
     const uint32_t b = keep ? gtop - (r2 >> gshift) : 0u;
     if (keep) atomicAdd(&s_cnt[b], 1u);
-    const unsigned long long km = __ballot(keep);
     if (km) {
       uint32_t k0 = 0;
       if (lane == 0) k0 = atomicAdd(&s_nst, (uint32_t)__popcll(km));
@@ -676,7 +676,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // Rounds over bucket ranges that fit in LDS, largest radii first.
   const bool target = (s_tgt[tgt_word] >> (li & 63)) & 1ull;
   uint32_t m = 0;
-  bool done = __ballot(target) == 0;
+  const unsigned long long tmask = __builtin_amdgcn_ballot_w64(target);
+  bool done = tmask == 0;
   uint32_t held = 0xFFFFFFFFu;  // bucket key that must be finished before leaving
   unsigned long long tests = 0, scanned = 0, rounds = 0;
   uint32_t b0 = 0;
@@ -722,7 +723,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     clk_p3 += c4 - c3;
     // P4: this wave's 4x4x4 sub-brick scans the round in order.
     if (!done) {
-      unsigned long long wait = __ballot(target && m == 0);
+      unsigned long long wait = (__builtin_amdgcn_uicmp(m, 0u, 32) & tmask);
       for (uint32_t c0 = 0; c0 < cnt_round; c0 += 64) {
         ++scanned;
         const uint32_t ci = c0 + lane;
@@ -742,7 +743,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         // (lane 0 holds entry c0: no LDS round trip)
         const uint32_t first_key = (uint32_t)__builtin_amdgcn_readfirstlane((int)cr) >> kshift;
         const uint32_t cmax = ((first_key + 1u) << gshift) - 1u;
-        const unsigned long long open = __ballot(target && (m >> 16) < cmax);
+        const unsigned long long open = (__builtin_amdgcn_uicmp(m >> 16, cmax, 36) & tmask);
         bool rel = false;
         if (open) {
           // z from the lowest and highest open lane, y and x from the (y, x)
@@ -768,7 +769,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           rel = live & (gap2((uint32_t)ex, (uint32_t)ey, (uint32_t)ez) <= (cr >> 16)) &
                 (held == 0xFFFFFFFFu || (cr >> kshift) == held);
         }
-        unsigned long long rm = __ballot(rel);
+        unsigned long long rm = __builtin_amdgcn_ballot_w64(rel);
         if (dotp) {
           if (rm) {
             // Compact the chunk's relevant candidates into slots 0 .. nrel-1
@@ -822,7 +823,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
             }
           }
         }
-        wait = __ballot(target && m == 0);
+        wait = (__builtin_amdgcn_uicmp(m, 0u, 32) & tmask);
         // Leave once every target is covered: candidates come in decreasing
         // r^2, so later ones cannot raise a maximum -- except that a bucket
         // with shift > 0 mixes radii and is finished first (rounds never split
