@@ -72,3 +72,41 @@ def test_never_covered_key():
     assert (res < 0x7FFF).all()
     # the largest 2 v.c - |v|^2 over the whole range stays below 2^15 as well
     assert (2 * 15 * 79 * 3) < 2**15
+
+
+# One-word staged candidates of the fill (radius R <= 45): r^2 << 21 plus the
+# block-relative centre c + R in three 7-bit fields, built as
+#   word = (r^2 << 21) + local7(entry) + tc,  tc = sum over axes of (corner + R) 2^(7 i)
+# where the tile corner (relative to the block origin) + R may be negative; the
+# placement decodes the fields back to the packed 10-bit position c + 512.
+def _local7(en):
+    return (en & 7) | ((en & 0x38) << 4) | ((en & 0x1C0) << 8)
+
+
+def test_compact_staging_word_round_trip():
+    rng = np.random.default_rng(1)
+    n = 300_000
+    for txn in (1, 2):
+        radius = rng.integers(1, 46, size=n)
+        k = (radius + 7) // 8
+        # tile of the reach box (ix, iy, iz) and the entry's local coordinates
+        tile = np.stack([rng.integers(0, 2 * k + txn), rng.integers(0, 2 * k + 1),
+                         rng.integers(0, 2 * k + 1)], axis=1)
+        loc = rng.integers(0, 8, size=(n, 3))
+        corner = (tile - k[:, None]) * 8  # relative to the block origin
+        c = corner + loc
+        hi = np.array([8 * txn - 1, 7, 7])
+        kept = ((c >= -radius[:, None]) & (c <= hi + radius[:, None])).all(1)
+        r2 = rng.integers(1, radius ** 2 + 1)
+        en = loc[:, 0] | (loc[:, 1] << 3) | (loc[:, 2] << 6) | (r2 << 9)
+        tc = ((corner[:, 0] + radius) + (corner[:, 1] + radius) * 128
+              + (corner[:, 2] + radius) * 16384) % 2**32
+        word = ((r2 << 21) + _local7(en) + tc) % 2**32
+        off = 512 - radius
+        pos = ((word & 127) + off) | ((((word >> 7) & 127) + off) << 10) | \
+              ((((word >> 14) & 127) + off) << 20)
+        want = (c[:, 0] + 512) | ((c[:, 1] + 512) << 10) | ((c[:, 2] + 512) << 20)
+        assert kept.sum() > n // 4
+        np.testing.assert_array_equal(pos[kept], want[kept])
+        np.testing.assert_array_equal((word >> 21)[kept], r2[kept])
+        assert (c[kept] + radius[kept, None]).max() <= 127 and (c[kept] + radius[kept, None]).min() >= 0
