@@ -1205,6 +1205,15 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
     uint32_t zxy_a = 0, zxy_b = 0;  // zxy(s-2), zxy(s-1) of this thread's word (t < 30)
     for (int sp = za - 2; sp <= zb + 1; ++sp) {
       const int rel = sp - (za - 2);
+      // selected voxels of this iteration's output plane sp - 2 (this row of
+      // the footprint: 8 tiles), read now so the load is complete by the
+      // output phase
+      const int pz = sp - 2;
+      const unsigned long long selw =
+          out_valid && pz >= za
+              ? masks[((size_t)((uint32_t)pz / 8 * TY + (uint32_t)gy / 8) * TX + (uint32_t)gx / 8) * 16 +
+                      ((uint32_t)pz & 7)]
+              : 0ull;
       // plane sp: M values and zero bits
       uint32_t* sm = s_m[rel & 3];
       sm[t] = pre0;
@@ -1257,13 +1266,7 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
         if (hh1) c1[t + 512] = contrib(t + 512);
       }
       __syncthreads();
-      const int pz = sp - 2;  // output plane
-      if (pz < za) continue;
-      // selected voxels of the plane (this row of the footprint: 8 tiles)
-      const unsigned long long selw =
-          out_valid ? masks[((size_t)((uint32_t)pz / 8 * TY + (uint32_t)gy / 8) * TX +
-                             (uint32_t)gx / 8) * 16 + ((uint32_t)pz & 7)]
-                    : 0ull;
+      if (pz < za) continue;  // no output plane yet
       const bool sel = (selw >> (((uint32_t)gx & 7) + 8 * ((uint32_t)gy & 7))) & 1ull;
       const uint32_t m = s_m[(rel - 2) & 3][(oy + 2) * kRegX + ox + 2];
       float tv = 0.0f;
