@@ -348,16 +348,22 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   constexpr int kStage = kStageCap * TXN;    // staged candidates
   constexpr int kNbr = kMaxNeighbours * TXN;  // tiles in reach
   constexpr int kCmap = kChunkMap * TXN;     // chunks mapped to their tile
-  __shared__ uint32_t s_pos[kCap], s_r2[kCap];  // one round, sorted by bucket
+  // One round sorted by bucket: position words s_sort[0 ..) and key words
+  // s_sort[koff ..).  The tiles in reach and their chunk offsets live in the
+  // same array above 2 kCap; once the candidates are staged they are dead and
+  // a round holds kCapX = kCap + 3 kNbr / 2 candidates (key words from kCapX).
+  constexpr int kCapX = kCap + 3 * kNbr / 2;
+  __shared__ uint32_t s_sort[2 * kCap + 3 * kNbr];
   __shared__ uint32_t s_cnt[kBuckets];          // histogram, then cursors
   __shared__ uint32_t s_start[kBuckets + 1];    // bucket starts over all rounds
   // a tile in reach: its index, and its offset (ix, iy, iz) in the reach box
   // (6 bits each) with its useful list prefix length - 1 above bit 18
-  __shared__ uint32_t s_nb_tile[kNbr], s_nb_word[kNbr];
+  uint32_t* const s_nb_tile = s_sort + 2 * kCap;
+  uint32_t* const s_nb_word = s_nb_tile + kNbr;
   // chunk -> tile in reach; while the sub-bricks scan (and when the candidates
   // are staged) it holds each wavefront's compacted candidates instead
   __shared__ __attribute__((aligned(16))) uint16_t s_cmap[kCmap];
-  __shared__ uint32_t s_off[kNbr];              // chunk offsets
+  uint32_t* const s_off = s_nb_word + kNbr;  // chunk offsets
   // candidates in list order (kStage position words, then kStage r^2 words;
   // or 2 kStage one-word candidates); when they are not staged, the
   // wavefronts' compacted candidates
@@ -373,7 +379,6 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   BM_GLOBAL PhaseStatus* __restrict__ st = gbl(I.status);
   BM_GLOBAL uint32_t* __restrict__ M = gbl(I.h);  // the Y-pass buffer is free once the Z pass has run
   BM_GLOBAL uint32_t* __restrict__ overflow = gbl(I.overflow);
-  const uint32_t cap = min(I.fill_cap * TXN, (uint32_t)kCap);
   constexpr bool diag = kDiag;
   BM_GLOBAL unsigned long long* __restrict__ diag_buf = gbl(I.diag);
   if (!st->has_site) return;
@@ -611,6 +616,11 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __syncthreads();
   const uint32_t nst = s_nst;
   const bool staged = nst <= stage_cap;  // else the placement re-reads the lists
+  // candidates per round (a smaller tuned or test capacity is kept as is)
+  const uint32_t cap = I.fill_cap >= (uint32_t)kFillCap ? (staged ? (uint32_t)kCapX : (uint32_t)kCap)
+                                                        : min(I.fill_cap * TXN, (uint32_t)kCap);
+  uint32_t* const s_pos = s_sort;
+  uint32_t* const s_r2 = s_sort + (staged ? kCapX : kCap);
   static_assert(sizeof(s_cmap) >= (size_t)NW * 64 * 8 && sizeof(s_stage) >= (size_t)NW * 64 * 8,
                 "per-wave candidate buffers");
   // This wave's 64 (position, key) pairs of compacted candidates, in LDS that
