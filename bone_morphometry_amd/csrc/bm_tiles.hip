@@ -362,7 +362,11 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   uint32_t* const s_nb_word = s_nb_tile + kNbr;
   // chunk -> tile in reach; while the sub-bricks scan (and when the candidates
   // are staged) it holds each wavefront's compacted candidates instead
-  __shared__ __attribute__((aligned(16))) uint16_t s_cmap[kCmap];
+  // (chunk -> its tile in reach: tile index | chunk of the list << 24 -- a
+  // volume of fewer than 2^32 voxels has fewer than 2^24 tiles -- and the
+  // tile's reach word, for the first kDesc chunks)
+  constexpr int kDesc = kCmap / 4;
+  __shared__ __attribute__((aligned(16))) uint2 s_cmap[kDesc];
   uint32_t* const s_off = s_nb_word + kNbr;  // chunk offsets
   // candidates in list order (kStage position words, then kStage r^2 words;
   // or 2 kStage one-word candidates); when they are not staged, the
@@ -502,8 +506,9 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __syncthreads();
   const uint32_t nchunks = s_nchunks;
   for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
-    const uint32_t c0 = s_off[j], c1 = min(c0 + (((s_nb_word[j] >> 18) + 64u) >> 6), (uint32_t)kCmap);
-    for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = (uint16_t)j;
+    const uint32_t c0 = s_off[j], c1 = min(c0 + (((s_nb_word[j] >> 18) + 64u) >> 6), (uint32_t)kDesc);
+    const uint32_t u = s_nb_tile[j], w = s_nb_word[j];
+    for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = make_uint2(u | ((c - c0) << 24), w);
   }
   __syncthreads();
   const unsigned long long tc = diag ? wall_clock64() : 0ull;
@@ -518,7 +523,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // block) a trip has no loop in it, so the LDS lookups of its chunks and then
   // their list loads are all issued together; otherwise a chunk past the map
   // finds its slot by a binary search over the offsets.
-  const bool mapped = nchunks <= (uint32_t)kCmap;
+  const bool mapped = nchunks <= (uint32_t)kDesc;
   // A list entry of a tile in reach is a candidate if its ball meets the
   // block box.  Positions relative to the block origin, offset by 512 (the
   // packed-position fields): p = (ix - K) * 8 + 512 + the entry's local
@@ -532,9 +537,16 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
 #pragma unroll
       for (int k = 0; k < kTrip; ++k) {
         const uint32_t q = q0 + (uint32_t)NW * k;
-        uint32_t cur;
-        if (kMapped || q < (uint32_t)kCmap) {
-          cur = s_cmap[min(q, nchunks - 1)];
+        // the chunk's values are uniform over the wave: scalar registers
+        uint32_t u, w1, e;
+        if (kMapped) {
+          // one broadcast read of the chunk's descriptor; a chunk q >=
+          // nchunks gets e beyond the list's last useful entry
+          const uint2 dsc = s_cmap[min(q, nchunks - 1)];
+          const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dsc.x);
+          w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dsc.y);
+          u = d0 & 0xFFFFFFu;
+          e = (q < nchunks ? (d0 >> 24) * 64u : 512u) + lane;
         } else {  // last slot with s_off <= q (offsets increase strictly)
           uint32_t lo = 0, hi = nn - 1;
           while (lo < hi) {
@@ -542,15 +554,13 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
             if (s_off[mid] <= q) lo = mid;
             else hi = mid - 1;
           }
-          cur = lo;
+          const uint32_t cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+          const uint32_t off = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_off[cur]);
+          u = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_tile[cur]);
+          w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_word[cur]);
+          e = (q - off) * 64 + lane;
         }
-        // the chunk's values are uniform over the wave: scalar registers
-        cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)cur);
-        const uint32_t off = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_off[cur]);
-        const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_tile[cur]);
-        const uint32_t w1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_nb_word[cur]);
         const uint32_t last = w1 >> 18;  // useful prefix length - 1
-        const uint32_t e = (q - off) * 64 + lane;
         nbw[k] = w1;
         // Every read here is unconditional (the list index clamped to the
         // list's last useful entry), so no branch separates the trip's
