@@ -1125,6 +1125,15 @@ constexpr int kRegX = 68, kRegY = 12, kReg = kRegX * kRegY;  // footprint + 2-vo
 constexpr int kHalX = 66, kHalY = 10, kHal = kHalX * kHalY;  // footprint + 1-voxel halo
 constexpr int kZWords = 30;  // 10 rows x 3 words of one-halo bits
 
+// 1 for a positive contribution, 0 for +0.0f (contributions are never
+// negative): min(bits, 1), written out so that it stays an integer minimum
+// (the compiler turns a comparison into a compare and a carry through VCC).
+__device__ __forceinline__ uint32_t nonzero1(float c) {
+  uint32_t r;
+  asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(__float_as_uint(c)));
+  return r;
+}
+
 // kLdsDiam: every diameter of the item is in the LDS table (the item's
 // largest r^2 is below kDiamLds), so a lookup needs no bounds test.
 struct CleanupLds {
@@ -1314,7 +1323,9 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
 #pragma unroll
             for (int k = 0; k < 13; ++k) {
               sum += c[k];
-              count += c[k] > 0.0f ? 1u : 0u;
+              // contributions are +0.0f or at least 2: a positive one has
+              // nonzero bits (an integer count, no compare)
+              count += nonzero1(c[k]);
             }
             __builtin_amdgcn_sched_barrier(0);
           }
