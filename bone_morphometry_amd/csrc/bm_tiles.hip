@@ -334,6 +334,15 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t n, uint32_t lan
   return (uint32_t)__shfl((int)x, 63);
 }
 
+// The same scan with the fewest values per lane that hold n (<= 64 kMax).
+template <int kMax, typename Val, typename Out>
+__device__ __forceinline__ uint32_t wave_scan_sized(uint32_t n, uint32_t lane, Val val, Out out) {
+  if (n <= 64) return wave_exclusive_scan<1>(n, lane, val, out);
+  if (kMax >= 2 && n <= 128) return wave_exclusive_scan<(kMax >= 2 ? 2 : 1)>(n, lane, val, out);
+  if (kMax >= 4 && n <= 256) return wave_exclusive_scan<(kMax >= 4 ? 4 : 1)>(n, lane, val, out);
+  return wave_exclusive_scan<kMax>(n, lane, val, out);
+}
+
 // One block per TXN x-adjacent tiles (a 8*TXN x 8 x 8 target box, 512*TXN
 // threads, one wavefront per 4x4x4 sub-brick): the tiles share the gathering
 // and sorting of their candidates.  kDiag: work counters and phase clocks into
@@ -425,7 +434,10 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     s_over = 0;
     s_nst = 0;
   }
-  for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) s_cnt[i] = 0;
+  // buckets 0 .. gtop hold every candidate (keys r^2 >> gshift of r^2 <= max_r2)
+  const uint32_t gshift = bucket_shift(max_r2);
+  const uint32_t gtop = max_r2 >> gshift;
+  for (uint32_t i = threadIdx.x; i <= gtop; i += blockDim.x) s_cnt[i] = 0;
   __syncthreads();
   const unsigned long long ta = diag ? wall_clock64() : 0ull;
   bool any_target = false;
@@ -498,7 +510,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // chunk offsets of the tiles in reach (64-entry chunks of their prefixes),
   // scanned by one wavefront
   if (wv == 0) {
-    const uint32_t tot = wave_exclusive_scan<kNbr / 64>(
+    const uint32_t tot = wave_scan_sized<kNbr / 64>(
         nn, lane, [&](uint32_t j) { return ((s_nb_word[j] >> 18) + 64) >> 6; },
         [&](uint32_t j, uint32_t p) { s_off[j] = p; });
     if (lane == 0) s_nchunks = tot;
@@ -513,8 +525,6 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __syncthreads();
   const unsigned long long tc = diag ? wall_clock64() : 0ull;
   unsigned long long td = tc;
-  const uint32_t gshift = bucket_shift(max_r2);
-  const uint32_t gtop = max_r2 >> gshift;
   const uint32_t kshift = gshift + 16;  // bucket of a key word
   // Chunks q = wv, wv + NW, ... of the flattened (neighbour, 64 entries)
   // range; one chunk is a sorted run of one list.  Read kTrip chunks per trip.
@@ -655,16 +665,16 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   };
   // bucket starts (and the placement cursors), scanned by one wavefront
   if (wv == 0) {
-    const uint32_t tot = wave_exclusive_scan<kBuckets / 64>(
-        (uint32_t)kBuckets, lane, [&](uint32_t j) { return s_cnt[j]; },
+    const uint32_t tot = wave_scan_sized<kBuckets / 64>(
+        gtop + 1, lane, [&](uint32_t j) { return s_cnt[j]; },
         [&](uint32_t j, uint32_t p) {
           s_cnt[j] = p;
           s_start[j] = p;
         });
-    if (lane == 0) s_start[kBuckets] = tot;
+    if (lane == 0) s_start[gtop + 1] = tot;
   }
   __syncthreads();
-  const uint32_t n = s_start[kBuckets];
+  const uint32_t n = s_start[gtop + 1];
   const unsigned long long clk1 = diag ? wall_clock64() : 0ull;
   // The sorted candidates are kept as a position word and a key word
   // key = r^2 << 16 | (K & 0xFFFF), and a voxel's running maximum m is the
