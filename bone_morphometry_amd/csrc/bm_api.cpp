@@ -109,7 +109,9 @@ Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
 // Kernel tunables for a volume: the autotuner's choice for its shape class,
 // with test overrides from the environment (BM_FILL_CAP: fill candidates per
 // round; BM_FILL_TILES: tiles per fill block; BM_TILE_ORDER: tile order bits;
-// BM_FILL_DOT: 0 for the packed-position candidate test at every radius).
+// BM_FILL_DOT: 0 for the packed-position candidate test at every radius;
+// BM_FILL_CHUNK_MAP, BM_FILL_STAGE: 0 to turn off the fill's chunk descriptors or
+// its staging area, so that the tests reach the paths large radii take).
 static Tuning tuning_for(const Dims& dims) {
   Tuning t = tuned_for(dims);
   if (const char* env = std::getenv("BM_FILL_CAP")) {
@@ -123,6 +125,14 @@ static Tuning tuning_for(const Dims& dims) {
   if (const char* env = std::getenv("BM_FILL_DOT")) {
     const long v = std::strtol(env, nullptr, 10);
     if (v == 0 || v == 1) t.fill_dot = (uint32_t)v;
+  }
+  if (const char* env = std::getenv("BM_FILL_CHUNK_MAP")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v == 0 || v == 1) t.fill_chunk_map = (uint32_t)v;
+  }
+  if (const char* env = std::getenv("BM_FILL_STAGE")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v == 0 || v == 1) t.fill_stage = (uint32_t)v;
   }
   if (const char* env = std::getenv("BM_TILE_ORDER")) {  // tile order bits (see Tuning)
     const long v = std::strtol(env, nullptr, 10);

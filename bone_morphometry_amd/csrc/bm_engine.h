@@ -124,6 +124,12 @@ struct Tuning {
   // item's radii are at most 64; 0: the packed-position test at every radius
   // (both forms give identical maps; 0 is a test override, not tuned).
   uint32_t fill_dot = 1;
+  // Test overrides of the fill's fast paths (1: as the kernel chooses; 0:
+  // never): list chunks looked up in the LDS chunk descriptors (else a
+  // binary search over the chunk offsets), candidates staged in LDS (else
+  // every round re-reads the lists).  Results never depend on them.
+  uint32_t fill_chunk_map = 1;
+  uint32_t fill_stage = 1;
 };
 
 constexpr int kStatsBlocks = 1024;  // most clean-up blocks of one item
@@ -142,6 +148,8 @@ constexpr uint32_t kItemQuantiles = 2u;  // run the quantile selection (needs th
 constexpr uint32_t kItemDiag = 4u;       // fill work counters
 constexpr uint32_t kItemMap = 8u;        // the clean-up writes the thickness map
 constexpr uint32_t kItemPackedFill = 16u;  // fill: packed-position test at every radius
+constexpr uint32_t kItemNoChunkMap = 32u;  // fill: chunks found by binary search (test override)
+constexpr uint32_t kItemNoStage = 64u;     // fill: candidates never staged (test override)
 
 struct TileMeta;
 

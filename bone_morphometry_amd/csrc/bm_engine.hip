@@ -121,6 +121,8 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
             (lt && s.diag ? kItemDiag : 0u) | (mapped ? kItemMap : 0u);
   const Tuning& t = s.tune;
   if (lt && !t.fill_dot) d.flags |= kItemPackedFill;
+  if (lt && !t.fill_chunk_map) d.flags |= kItemNoChunkMap;
+  if (lt && !t.fill_stage) d.flags |= kItemNoStage;
   d.xblocks = grid_for((uint64_t)dims.h * dims.d * 64, 256, kEdtXBlocks);
   {
     const uint32_t H = dims.h;

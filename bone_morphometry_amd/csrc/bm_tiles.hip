@@ -533,7 +533,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // block) a trip has no loop in it, so the LDS lookups of its chunks and then
   // their list loads are all issued together; otherwise a chunk past the map
   // finds its slot by a binary search over the offsets.
-  const bool mapped = nchunks <= (uint32_t)kDesc;
+  const bool mapped = nchunks <= (uint32_t)kDesc && !(I.flags & kItemNoChunkMap);
   // A list entry of a tile in reach is a candidate if its ball meets the
   // block box.  Positions relative to the block origin, offset by 512 (the
   // packed-position fields): p = (ix - K) * 8 + 512 + the entry's local
@@ -607,7 +607,8 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // staging area then holds 2 kStage candidates.  Otherwise a position word
   // and an r^2 word each.
   const bool compact = R <= 45;
-  const uint32_t stage_cap = compact ? 2u * (uint32_t)kStage : (uint32_t)kStage;
+  const uint32_t stage_cap = (I.flags & kItemNoStage) ? 0u
+                             : compact ? 2u * (uint32_t)kStage : (uint32_t)kStage;
   // P2: histogram by bucket, one LDS atomic per kept candidate (the LDS is
   // otherwise idle here, the vector units are not); the candidates are also
   // staged in LDS (one slot reservation per chunk) so that the placement

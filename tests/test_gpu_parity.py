@@ -201,6 +201,34 @@ def test_fill_packed_position_form_matches_oracle(case, tiles, monkeypatch):
     np.testing.assert_array_equal(got["thickness"], ref["thickness"])
 
 
+@pytest.mark.parametrize("tiles", ["1", "2"])
+@pytest.mark.parametrize("overrides", [{"BM_FILL_CHUNK_MAP": "0"}, {"BM_FILL_STAGE": "0"},
+                                       {"BM_FILL_CHUNK_MAP": "0", "BM_FILL_STAGE": "0",
+                                        "BM_FILL_CAP": "96"}],
+                         ids=["binary-search chunks", "unstaged", "both, small rounds"])
+def test_fill_slow_paths_match_oracle(overrides, tiles, monkeypatch):
+    """The fill's paths for tiles with more list chunks than its descriptors
+    hold (a binary search over the chunk offsets) and more candidates than its
+    staging area holds (every round re-reads the lists), forced on an ordinary
+    volume and a volume of a few sites: the oracle's map either way."""
+    for k, v in overrides.items():
+        monkeypatch.setenv(k, v)
+    monkeypatch.setenv("BM_FILL_TILES", tiles)
+    for case in ("structure", "sparse_sites"):
+        if case == "structure":
+            w, h, d = 40, 36, 44
+            a = _structure(5, (d, h, w), 0.75)
+        else:
+            rng = np.random.default_rng(29)
+            w, h, d = 72, 64, 56
+            a = np.ones(w * h * d, np.uint8)
+            a[rng.choice(a.size, 30, replace=False)] = 0
+        ref = orc.local_thickness_stages(a, w, h, d, 0)
+        got = bm.local_thickness_stages(vol(a, w, h, d), bm.Phase.Foreground)
+        np.testing.assert_array_equal(got["propagated"], ref["propagated"], err_msg=case)
+        np.testing.assert_array_equal(got["thickness"], ref["thickness"], err_msg=case)
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_fill_axis_beyond_1024_tiles_matches_oracle(axis):
     """An axis of 8216 voxels (1027 tiles): tile coordinates past 10 bits in
