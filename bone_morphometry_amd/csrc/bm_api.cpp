@@ -111,7 +111,8 @@ Dims device_dims(uint64_t w, uint64_t h, uint64_t d) {
 // round; BM_FILL_TILES: tiles per fill block; BM_TILE_ORDER: tile order bits;
 // BM_FILL_DOT: 0 for the packed-position candidate test at every radius;
 // BM_FILL_CHUNK_MAP, BM_FILL_STAGE: 0 to turn off the fill's chunk descriptors or
-// its staging area, so that the tests reach the paths large radii take).
+// its staging area, so that the tests reach the paths large radii take;
+// BM_EDT_TILES "y_rows,y_halo,z_rows,z_halo": the Y and Z passes' tiles).
 static Tuning tuning_for(const Dims& dims) {
   Tuning t = tuned_for(dims);
   if (const char* env = std::getenv("BM_FILL_CAP")) {
@@ -137,6 +138,16 @@ static Tuning tuning_for(const Dims& dims) {
   if (const char* env = std::getenv("BM_TILE_ORDER")) {  // tile order bits (see Tuning)
     const long v = std::strtol(env, nullptr, 10);
     if (v >= 0 && v <= 7) t.tile_order = (uint32_t)v;
+  }
+  if (const char* env = std::getenv("BM_EDT_TILES")) {
+    unsigned v[4];
+    if (std::sscanf(env, "%u,%u,%u,%u", &v[0], &v[1], &v[2], &v[3]) == 4 && v[0] >= 2 &&
+        v[0] <= 512 && v[2] >= 2 && v[2] <= 512) {
+      t.y_rows = v[0];
+      t.y_halo = v[1];
+      t.z_rows = v[2];
+      t.z_halo = v[3];
+    }
   }
   return t;
 }

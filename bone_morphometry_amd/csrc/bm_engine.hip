@@ -82,6 +82,14 @@ static uint32_t grid_for(uint64_t items, unsigned per_block, unsigned cap) {
 
 static uint32_t round8(uint64_t v) { return (uint32_t)((v + kXcds - 1) / kXcds * kXcds); }
 
+// Rows an EDT Y/Z tile stages: [lo & ~1, hi) rounded up to whole pairs, for a
+// line of n rows cut in chunks (even when split) with halos of the given size.
+static uint32_t pair_rows(uint32_t n, uint32_t chunk, uint32_t halo) {
+  const uint32_t span = std::min(n, chunk + 2 * halo);  // hi - lo
+  // an odd halo puts lo on an odd row whenever it is not clipped at 0
+  return ((halo & 1u) ? span + 2 : span + 1) & ~1u;
+}
+
 // Largest ridge r^2 the tile lists hold, plus one (kListR2Limit).  Tests lower
 // it with BM_LIST_R2_LIMIT to reach the out-of-range error on small volumes.
 static uint32_t list_r2_limit() {
@@ -126,15 +134,18 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
   d.xblocks = grid_for((uint64_t)dims.h * dims.d * 64, 256, kEdtXBlocks);
   {
     const uint32_t H = dims.h;
-    d.ychunk = H <= t.y_rows ? H : t.y_rows - 2 * t.y_halo;
+    // split columns take even chunks: the Y pass stages and searches pairs of
+    // rows from an even row
+    d.ychunk = H <= t.y_rows ? H : t.y_rows > 2 * t.y_halo + 1 ? (t.y_rows - 2 * t.y_halo) & ~1u : 0u;
     d.yhalo = H <= t.y_rows ? 0 : t.y_halo;
+    if (d.ychunk == 0) throw std::invalid_argument("y tile chunk below 2 rows");
     d.ygx = (dims.w + 63) / 64;
     d.ygy = dims.d;
     d.ygz = (H + d.ychunk - 1) / d.ychunk;
   }
   {
     const uint32_t D = dims.d;
-    d.zchunk = D <= t.z_rows ? D : t.z_rows - 2 * t.z_halo;
+    d.zchunk = D <= t.z_rows ? D : t.z_rows > 2 * t.z_halo ? (t.z_rows - 2 * t.z_halo) & ~1u : 0u;
     d.zhalo = D <= t.z_rows ? 0 : t.z_halo;
     if (d.zchunk < kMinZChunk && d.zchunk < D)
       throw std::invalid_argument("z tile chunk below kMinZChunk");
@@ -253,8 +264,9 @@ void Plan::build(const std::vector<ItemSpec>& specs, hipStream_t s, void* ext, s
     offs[i] = total;
     total += lay_out(specs[i], desc_[i], nullptr, &blocks[i * kNumStages], fill_tiles_);
     any_ |= desc_[i].flags;
-    y_lds_ = std::max(y_lds_, std::min(desc_[i].H, desc_[i].ychunk + 2 * desc_[i].yhalo) * 64u * 2u);
-    z_lds_ = std::max(z_lds_, std::min(desc_[i].D, desc_[i].zchunk + 2 * desc_[i].zhalo) * 32u * 4u);
+    // 16-bit values in pairs of rows from an even row
+    y_lds_ = std::max(y_lds_, pair_rows(desc_[i].H, desc_[i].ychunk, desc_[i].yhalo) * 64u * 2u);
+    z_lds_ = std::max(z_lds_, pair_rows(desc_[i].D, desc_[i].zchunk, desc_[i].zhalo) * 32u * 2u);
   }
   if (y_lds_ > 65536 || z_lds_ > 65536) throw std::invalid_argument("EDT tile exceeds 64 KiB of LDS");
   char* base;

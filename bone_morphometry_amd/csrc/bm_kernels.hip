@@ -227,8 +227,7 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
 
 // ---------------------------------------------------------------------------
 // The 1-D step of the Y and Z passes for one lane: d(q) = min_p (q-p)^2 + f(p)
-// for ROWS consecutive q per group, groups q0 = first, first + step, ... below
-// end.  At offset o the candidates of row k are q0+k-o and q0+k+o, a window
+// for the ROWS consecutive rows q0 .. q0 + ROWS - 1 below end.  At offset o the candidates of row k are q0+k-o and q0+k+o, a window
 // that slides by one row per offset.  Offsets go in blocks of ROWS: a block
 // reads the ROWS new values on each side once, and every candidate of the
 // block is then a compile-time element of the previous or the new values, so
@@ -243,83 +242,213 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
 // when a whole block's positions lie in [in_lo, in_hi).
 constexpr int kEdtRows = 8;
 template <int ROWS, class F, class Fin, class Out>
-__device__ __forceinline__ void window_search(uint32_t first, uint32_t end, uint32_t step,
-                                              uint32_t n, int in_lo, int in_hi, F&& f, Fin&& fin,
-                                              Out&& out) {
-  for (uint32_t q0 = first; q0 < end; q0 += step) {
-    const int iq = (int)q0;
-    uint32_t best[ROWS], lp[ROWS], up[ROWS];
-    if (__ballot(!(iq >= in_lo && iq + ROWS <= in_hi)) == 0) {
+__device__ __forceinline__ void window_group(uint32_t q0, uint32_t end, uint32_t n, int in_lo,
+                                             int in_hi, F&& f, Fin&& fin, Out&& out) {
+  const int iq = (int)q0;
+  uint32_t best[ROWS], lp[ROWS], up[ROWS];
+  if (__ballot(!(iq >= in_lo && iq + ROWS <= in_hi)) == 0) {
 #pragma unroll
-      for (int k = 0; k < ROWS; ++k) best[k] = fin(iq + k);
-    } else {
+    for (int k = 0; k < ROWS; ++k) best[k] = fin(iq + k);
+  } else {
 #pragma unroll
-      for (int k = 0; k < ROWS; ++k) best[k] = f(iq + k);
-    }
-    uint32_t bmax = 0;
+    for (int k = 0; k < ROWS; ++k) best[k] = f(iq + k);
+  }
+  uint32_t bmax = 0;
 #pragma unroll
-    for (int k = 0; k < ROWS; ++k) {
-      lp[k] = up[k] = best[k];  // f at q0 + k: the low side at o0 = 0, and the high side
-      if (q0 + k < end) bmax = max(bmax, best[k]);
-    }
-    for (uint32_t o0 = 0;; o0 += ROWS) {  // offsets o0 + 1 .. o0 + ROWS
-      if (!__ballot((o0 + 1) * (o0 + 1) < bmax)) break;
-      // nothing left on either side of any row of any lane
-      if (!__ballot(!(iq + ROWS - 1 < (int)(o0 + 1) && q0 + o0 + 1 >= n))) break;
-      // ln[m] = f(q0 - o0 - ROWS + m), un[m] = f(q0 + o0 + ROWS + m); lp and up
-      // hold f(q0 - o0 + m) and f(q0 + o0 + m)
-      const int plo = iq - (int)o0 - ROWS, phi = iq + (int)o0 + ROWS;
-      uint32_t ln[ROWS], un[ROWS];
-      if (__ballot(!(plo >= in_lo && phi + ROWS <= in_hi)) == 0) {
-#pragma unroll
-        for (int m = 0; m < ROWS; ++m) {
-          ln[m] = fin(plo + m);
-          un[m] = fin(phi + m);
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < ROWS; ++m) {
-          ln[m] = f(plo + m);
-          un[m] = f(phi + m);
-        }
-      }
-      bool stop = false;
-#pragma unroll
-      for (int j = 1; j <= ROWS; ++j) {
-        const uint32_t o = o0 + j, oo = o * o;
-        bmax = 0;
-#pragma unroll
-        for (int k = 0; k < ROWS; ++k) {
-          const int ci = ROWS + k - j;  // q0 + k - o = plo + ci
-          const int di = k + j;         // q0 + k + o = q0 + o0 + di
-          const uint32_t lo = ci < ROWS ? ln[ci] : lp[ci - ROWS];
-          const uint32_t hi = di < ROWS ? up[di] : un[di - ROWS];
-          best[k] = min(best[k], min(lo, hi) + oo);
-          if (q0 + k < end) bmax = max(bmax, best[k]);
-        }
-        if (j < ROWS && !__ballot((o + 1) * (o + 1) < bmax)) {
-          stop = true;
-          break;
-        }
-      }
-      if (stop) break;
+  for (int k = 0; k < ROWS; ++k) {
+    lp[k] = up[k] = best[k];  // f at q0 + k: the low side at o0 = 0, and the high side
+    if (q0 + k < end) bmax = max(bmax, best[k]);
+  }
+  for (uint32_t o0 = 0;; o0 += ROWS) {  // offsets o0 + 1 .. o0 + ROWS
+    if (!__ballot((o0 + 1) * (o0 + 1) < bmax)) break;
+    // nothing left on either side of any row of any lane
+    if (!__ballot(!(iq + ROWS - 1 < (int)(o0 + 1) && q0 + o0 + 1 >= n))) break;
+    // ln[m] = f(q0 - o0 - ROWS + m), un[m] = f(q0 + o0 + ROWS + m); lp and up
+    // hold f(q0 - o0 + m) and f(q0 + o0 + m)
+    const int plo = iq - (int)o0 - ROWS, phi = iq + (int)o0 + ROWS;
+    uint32_t ln[ROWS], un[ROWS];
+    if (__ballot(!(plo >= in_lo && phi + ROWS <= in_hi)) == 0) {
 #pragma unroll
       for (int m = 0; m < ROWS; ++m) {
-        lp[m] = ln[m];
-        up[m] = un[m];
+        ln[m] = fin(plo + m);
+        un[m] = fin(phi + m);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < ROWS; ++m) {
+        ln[m] = f(plo + m);
+        un[m] = f(phi + m);
+      }
+    }
+    bool stop = false;
+#pragma unroll
+    for (int j = 1; j <= ROWS; ++j) {
+      const uint32_t o = o0 + j, oo = o * o;
+      bmax = 0;
+#pragma unroll
+      for (int k = 0; k < ROWS; ++k) {
+        const int ci = ROWS + k - j;  // q0 + k - o = plo + ci
+        const int di = k + j;         // q0 + k + o = q0 + o0 + di
+        const uint32_t lo = ci < ROWS ? ln[ci] : lp[ci - ROWS];
+        const uint32_t hi = di < ROWS ? up[di] : un[di - ROWS];
+        best[k] = min(best[k], min(lo, hi) + oo);
+        if (q0 + k < end) bmax = max(bmax, best[k]);
+      }
+      if (j < ROWS && !__ballot((o + 1) * (o + 1) < bmax)) {
+        stop = true;
+        break;
+      }
+    }
+    if (stop) break;
+#pragma unroll
+    for (int m = 0; m < ROWS; ++m) {
+      lp[m] = ln[m];
+      up[m] = un[m];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ROWS; ++k)
+    if (q0 + k < end) out(q0 + k, best[k]);
+}
+
+// The same search on saturated 16-bit values, two rows per 32-bit register:
+// pair (p, p + 1), p even, holds min(f(p), 0xFFFF) in the low half and
+// min(f(p + 1), 0xFFFF) in the high half, and one packed minimum or
+// saturating add serves two rows.  Saturation is monotone, so a row's packed
+// result is min(d(q), 0xFFFF) and is exact below 0xFFFF; a saturated row
+// makes the call return false without writing, and the caller runs the
+// 32-bit search for the group.  Offsets stop by o = 255 at the latest, where
+// o^2 alone saturates.  Rows at or past end start at 0, so they never hold
+// the search open and are never written.  The candidates of two rows at an
+// odd offset straddle two pairs: those pairs come from one byte alignment
+// per pair and block of offsets.  q0 must be even; f2 and fin2 take an even
+// position and return its pair (fin2 from the staged tile when a whole
+// block lies in [in_lo, in_hi)).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_min16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_max16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_add_sat16(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, a),
+                                                                    __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t sat16(uint32_t v) { return min(v, 0xFFFFu); }
+__device__ __forceinline__ uint32_t pair16(uint32_t lo, uint32_t hi) {
+  return sat16(lo) | (sat16(hi) << 16);
+}
+
+template <int ROWS, class F2, class Fin2, class Out>
+__device__ __forceinline__ bool window_group16(uint32_t q0, uint32_t end, uint32_t n, int in_lo,
+                                               int in_hi, F2&& f2, Fin2&& fin2, Out&& out) {
+  constexpr int P = ROWS / 2;
+  static_assert(ROWS % 2 == 0, "rows go in pairs");
+  const int iq = (int)q0;
+  uint32_t best[P], lp[P], up[P];
+  if (__ballot(!(iq >= in_lo && iq + ROWS <= in_hi)) == 0) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) best[i] = fin2(iq + 2 * i);
+  } else {
+#pragma unroll
+    for (int i = 0; i < P; ++i) best[i] = f2(iq + 2 * i);
+  }
+  uint32_t bm = 0;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    lp[i] = up[i] = best[i];  // pairs at q0 + 2i: the low side at o0 = 0, and the high side
+    const uint32_t r = q0 + 2 * i;
+    best[i] &= (r < end ? 0xFFFFu : 0u) | (r + 1 < end ? 0xFFFF0000u : 0u);
+    bm = pk_max16(bm, best[i]);
+  }
+  uint32_t bmax = max(bm & 0xFFFFu, bm >> 16);
+  for (uint32_t o0 = 0;; o0 += ROWS) {  // offsets o0 + 1 .. o0 + ROWS
+    if (!__ballot((o0 + 1) * (o0 + 1) < bmax)) break;
+    // nothing left on either side of any row of any lane
+    if (!__ballot(!(iq + ROWS - 1 < (int)(o0 + 1) && q0 + o0 + 1 >= n))) break;
+    // lw = pairs at q0 - o0 - ROWS + 2m (ln) then q0 - o0 + 2m (lp); uw = pairs
+    // at q0 + o0 + 2m (up) then q0 + o0 + ROWS + 2m (un)
+    const int plo = iq - (int)o0 - ROWS, phi = iq + (int)o0 + ROWS;
+    uint32_t lw[2 * P], uw[2 * P];
+    if (__ballot(!(plo >= in_lo && phi + ROWS <= in_hi)) == 0) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        lw[m] = fin2(plo + 2 * m);
+        uw[P + m] = fin2(phi + 2 * m);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        lw[m] = f2(plo + 2 * m);
+        uw[P + m] = f2(phi + 2 * m);
       }
     }
 #pragma unroll
-    for (int k = 0; k < ROWS; ++k)
-      if (q0 + k < end) out(q0 + k, best[k]);
+    for (int m = 0; m < P; ++m) {
+      lw[P + m] = lp[m];
+      uw[m] = up[m];
+    }
+    // the pairs that straddle: (high half of pair m, low half of pair m + 1)
+    uint32_t ls[2 * P - 1], us[2 * P - 1];
+#pragma unroll
+    for (int m = 0; m < 2 * P - 1; ++m) {
+      ls[m] = __builtin_amdgcn_alignbyte(lw[m + 1], lw[m], 2);
+      us[m] = __builtin_amdgcn_alignbyte(uw[m + 1], uw[m], 2);
+    }
+    bool stop = false;
+#pragma unroll
+    for (int j = 1; j <= ROWS; ++j) {
+      const uint32_t o = o0 + j, oo2 = sat16(o * o) * 0x10001u;
+      bm = 0;
+#pragma unroll
+      for (int i = 0; i < P; ++i) {
+        // rows 2i and 2i + 1: low candidates at ROWS + 2i - j of lw's rows,
+        // high candidates at 2i + j of uw's rows
+        const int c = ROWS + 2 * i - j, d = 2 * i + j;
+        const uint32_t lo = (j & 1) ? ls[(c - 1) / 2] : lw[c / 2];
+        const uint32_t hi = (j & 1) ? us[(d - 1) / 2] : uw[d / 2];
+        best[i] = pk_min16(best[i], pk_add_sat16(pk_min16(lo, hi), oo2));
+        bm = pk_max16(bm, best[i]);
+      }
+      bmax = max(bm & 0xFFFFu, bm >> 16);
+      if (j < ROWS && !__ballot((o + 1) * (o + 1) < bmax)) {
+        stop = true;
+        break;
+      }
+    }
+    if (stop) break;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      lp[m] = lw[m];
+      up[m] = uw[P + m];
+    }
   }
+  bool sat = false;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const uint32_t r = q0 + 2 * i;
+    sat |= (r < end && (best[i] & 0xFFFFu) == 0xFFFFu) || (r + 1 < end && (best[i] >> 16) == 0xFFFFu);
+  }
+  if (__ballot(sat)) return false;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const uint32_t r = q0 + 2 * i;
+    if (r < end) out(r, best[i] & 0xFFFFu);
+    if (r + 1 < end) out(r + 1, best[i] >> 16);
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------
-// Y pass: tile of 64 x-columns x rows [ya, yb) of one z-slice in LDS (u16).
+// Y pass: tile of 64 x-columns x the rows [ya, ya + 2 np) of one z-slice in
+// LDS as saturated 16-bit squares, one word per column and pair of rows (the
+// tile starts on an even row: chunks are even, so y0 is).
 __global__ void __launch_bounds__(256)
 k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
-  extern __shared__ uint16_t ytile[];
+  extern __shared__ uint32_t ytile[];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const uint16_t* __restrict__ in = gbl(I.dx);
@@ -328,44 +457,63 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t bx = vb % I.ygx, bz = vb / I.ygx / I.ygy;
   const uint32_t x0 = bx * 64, z = (vb / I.ygx) % I.ygy;
   const uint32_t y0 = bz * chunk, y1 = min(H, y0 + chunk);
-  const uint32_t ya = y0 > halo ? y0 - halo : 0, yb = min(H, y1 + halo);
+  const uint32_t ya = (y0 > halo ? y0 - halo : 0) & ~1u, yb = min(H, y1 + halo);
+  const uint32_t np = (yb - ya + 1) / 2;
   const size_t sbase = (size_t)z * W * H;
-  const uint32_t nrows = yb - ya;
   const bool vec = (x0 + 64 <= W) && ((W & 7) == 0);
-  for (uint32_t i = threadIdx.x; i < nrows * 8; i += blockDim.x) {
-    const uint32_t rr = i >> 3, c = (i & 7) * 8;
-    auto p = in + sbase + (size_t)(ya + rr) * W + x0 + c;
-    uint4 v;
+  for (uint32_t i = threadIdx.x; i < np * 8; i += blockDim.x) {
+    const uint32_t pr = i >> 3, c = (i & 7) * 8, r0 = ya + 2 * pr;  // r0 < H
+    auto p = in + sbase + (size_t)r0 * W + x0 + c;
+    const bool has1 = r0 + 1 < H;
+    uint32_t a[8], b[8];
     if (vec) {
-      const v4u32 w = *(BM_GLOBAL const v4u32*)(p);
-      v = make_uint4(w.x, w.y, w.z, w.w);
+      const v4u32 w0 = *(BM_GLOBAL const v4u32*)(p);
+      const v4u32 w1 = has1 ? *(BM_GLOBAL const v4u32*)(p + W) : v4u32{~0u, ~0u, ~0u, ~0u};
+      for (int k = 0; k < 4; ++k) {
+        a[2 * k] = w0[k] & 0xFFFFu;
+        a[2 * k + 1] = w0[k] >> 16;
+        b[2 * k] = w1[k] & 0xFFFFu;
+        b[2 * k + 1] = w1[k] >> 16;
+      }
     } else {
-      uint32_t t[8];
-      for (int k = 0; k < 8; ++k) t[k] = (x0 + c + k < W) ? p[k] : kInf16;
-      v = make_uint4(t[0] | (t[1] << 16), t[2] | (t[3] << 16), t[4] | (t[5] << 16),
-                     t[6] | (t[7] << 16));
+      for (int k = 0; k < 8; ++k) {
+        const bool in_row = x0 + c + k < W;
+        a[k] = in_row ? p[k] : kInf16;
+        b[k] = in_row && has1 ? p[W + k] : kInf16;
+      }
     }
-    *reinterpret_cast<uint4*>(&ytile[rr * 64 + c]) = v;
+    uint32_t w[8];
+    for (int k = 0; k < 8; ++k) w[k] = pair16(sq16(a[k]), sq16(b[k]));
+    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c]) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c + 4]) = make_uint4(w[4], w[5], w[6], w[7]);
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const uint32_t x = x0 + lane;
   if (x >= W) return;
   auto col = in + sbase + x;
-  // f(p) = dx(p)^2 of this column, kInf32 outside it; global beyond the halo.
+  // f(p) = dx(p)^2 of this column, kInf32 outside it, from global memory
   auto fy = [&](int p) -> uint32_t {
-    const uint32_t r = (uint32_t)(p - (int)ya);
-    if (r < nrows) return sq16(ytile[r * 64 + lane]);
     if (p < 0 || p >= (int)H) return kInf32;
     return sq16(col[(size_t)p * W]);
   };
-  auto fy_in = [&](int p) -> uint32_t { return sq16(ytile[(uint32_t)(p - (int)ya) * 64 + lane]); };
-  window_search<kEdtRows>(y0 + kEdtRows * wv, y1, kEdtRows * nwv, H, (int)ya, (int)yb, fy, fy_in,
-                          [&](uint32_t q, uint32_t v) { out[sbase + (size_t)q * W + x] = v; });
+  // the pair at even p: from the tile, else from global memory
+  const int tlo = (int)ya, thi = (int)(ya + 2 * np);
+  auto fin2 = [&](int p) -> uint32_t { return ytile[(uint32_t)(p - tlo) / 2 * 64 + lane]; };
+  auto f2 = [&](int p) -> uint32_t {
+    const uint32_t r = (uint32_t)(p - tlo);
+    if (r < 2 * np) return ytile[r / 2 * 64 + lane];
+    return pair16(fy(p), fy(p + 1));
+  };
+  auto outy = [&](uint32_t q, uint32_t v) { out[sbase + (size_t)q * W + x] = v; };
+  for (uint32_t q0 = y0 + kEdtRows * wv; q0 < y1; q0 += kEdtRows * nwv)
+    if (!window_group16<kEdtRows>(q0, y1, H, tlo, thi, f2, fin2, outy))
+      window_group<kEdtRows>(q0, y1, H, 0, 0, fy, fy, outy);
 }
 
 // ---------------------------------------------------------------------------
-// Z pass: tile of 32 x-columns x z-range [za, zb) at one y in LDS (u32).
+// Z pass: tile of 32 x-columns x the planes [za, za + 2 np) at one y in LDS as
+// saturated 16-bit values, one word per column and pair of planes (za even).
 __global__ void __launch_bounds__(256)
 k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts, uint32_t nitems) {
   extern __shared__ uint32_t ztile[];
@@ -376,25 +524,33 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t W = I.W, H = I.H, D = I.D, chunk = I.zchunk, halo = I.zhalo;
   const uint32_t x0 = (vb % I.zgx) * 32, y = (vb / I.zgx) % I.zgy;
   const uint32_t z0 = (vb / I.zgx / I.zgy) * chunk, z1 = min(D, z0 + chunk);
-  const uint32_t za = z0 > halo ? z0 - halo : 0, zb = min(D, z1 + halo);
+  const uint32_t za = (z0 > halo ? z0 - halo : 0) & ~1u, zb = min(D, z1 + halo);
+  const uint32_t np = (zb - za + 1) / 2;
   const size_t slice = (size_t)W * H;
   const size_t base = (size_t)y * W + x0;
-  const uint32_t nrows = zb - za;
   const bool vec = (x0 + 32 <= W) && ((W & 3) == 0);
-  for (uint32_t i = threadIdx.x; i < nrows * 8; i += blockDim.x) {
-    const uint32_t rr = i >> 3, c = (i & 7) * 4;
-    auto p = in + (size_t)(za + rr) * slice + base + c;
-    uint4 v;
+  for (uint32_t i = threadIdx.x; i < np * 8; i += blockDim.x) {
+    const uint32_t pr = i >> 3, c = (i & 7) * 4, r0 = za + 2 * pr;  // r0 < D
+    auto p = in + (size_t)r0 * slice + base + c;
+    const bool has1 = r0 + 1 < D;
+    uint32_t a[4], b[4];
     if (vec) {
-      const v4u32 w = *(BM_GLOBAL const v4u32*)(p);
-      v = make_uint4(w.x, w.y, w.z, w.w);
+      const v4u32 w0 = *(BM_GLOBAL const v4u32*)(p);
+      const v4u32 w1 =
+          has1 ? *(BM_GLOBAL const v4u32*)(p + slice) : v4u32{kInf32, kInf32, kInf32, kInf32};
+      for (int k = 0; k < 4; ++k) {
+        a[k] = w0[k];
+        b[k] = w1[k];
+      }
     } else {
-      v.x = (x0 + c + 0 < W) ? p[0] : kInf32;
-      v.y = (x0 + c + 1 < W) ? p[1] : kInf32;
-      v.z = (x0 + c + 2 < W) ? p[2] : kInf32;
-      v.w = (x0 + c + 3 < W) ? p[3] : kInf32;
+      for (int k = 0; k < 4; ++k) {
+        const bool in_row = x0 + c + k < W;
+        a[k] = in_row ? p[k] : kInf32;
+        b[k] = in_row && has1 ? p[slice + k] : kInf32;
+      }
     }
-    *reinterpret_cast<uint4*>(&ztile[rr * 32 + c]) = v;
+    *reinterpret_cast<uint4*>(&ztile[pr * 32 + c]) =
+        make_uint4(pair16(a[0], b[0]), pair16(a[1], b[1]), pair16(a[2], b[2]), pair16(a[3], b[3]));
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -403,19 +559,25 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   uint32_t mx = 0;
   if (x < W) {
     auto col = in + base + xl;
-    auto fz = [&](int p) -> uint32_t {  // kInf32 outside the line; global beyond the halo
-      const uint32_t r = (uint32_t)(p - (int)za);
-      if (r < nrows) return ztile[r * 32 + xl];
+    auto fz = [&](int p) -> uint32_t {  // kInf32 outside the line, from global memory
       if (p < 0 || p >= (int)D) return kInf32;
       return col[(size_t)p * slice];
     };
+    const int tlo = (int)za;
+    auto fin2 = [&](int p) -> uint32_t { return ztile[(uint32_t)(p - tlo) / 2 * 32 + xl]; };
+    auto f2 = [&](int p) -> uint32_t {
+      const uint32_t r = (uint32_t)(p - tlo);
+      if (r < 2 * np) return ztile[r / 2 * 32 + xl];
+      return pair16(fz(p), fz(p + 1));
+    };
+    auto outz = [&](uint32_t q, uint32_t v) {
+      out[(size_t)q * slice + base + xl] = v;
+      mx = max(mx, v);
+    };
     // the two half-waves take adjacent groups of rows
-    auto fz_in = [&](int p) -> uint32_t { return ztile[(uint32_t)(p - (int)za) * 32 + xl]; };
-    window_search<kEdtRows>(z0 + 2 * kEdtRows * wv + kEdtRows * zs, z1, 2 * kEdtRows * nwv, D,
-                            (int)za, (int)zb, fz, fz_in, [&](uint32_t q, uint32_t v) {
-                              out[(size_t)q * slice + base + xl] = v;
-                              mx = max(mx, v);
-                            });
+    for (uint32_t q0 = z0 + 2 * kEdtRows * wv + kEdtRows * zs; q0 < z1; q0 += 2 * kEdtRows * nwv)
+      if (!window_group16<kEdtRows>(q0, z1, D, tlo, (int)(za + 2 * np), f2, fin2, outz))
+        window_group<kEdtRows>(q0, z1, D, 0, 0, fz, fz, outz);
   }
   mx = wave_max_u32(mx);
   __shared__ uint32_t s_mx[4];

@@ -108,6 +108,46 @@ def test_edt_sparse_sites_long_searches():
     np.testing.assert_array_equal(got, orc.edt_squared(a, w, h, d, 0))
 
 
+@pytest.mark.parametrize("shape,sites", [
+    ((600, 5, 4), [(0, 0, 0)]),
+    ((5, 700, 3), [(4, 0, 2)]),
+    ((4, 3, 1300), [(0, 2, 0)]),
+    ((70, 620, 9), [(0, 0, 0), (69, 300, 8)]),
+    ((9, 40, 900), [(4, 20, 450), (0, 0, 0)]),
+])
+def test_edt_distances_beyond_16_bits(shape, sites):
+    """Squared distances of 2^16 and more next to smaller ones in the same
+    wavefront: the Y and Z passes search in saturated 16-bit pairs and search
+    a group again in 32 bits when one of its rows saturates."""
+    w, h, d = shape
+    a = np.ones((d, h, w), np.uint8)
+    for x, y, z in sites:
+        a[z, y, x] = 0
+    ref = orc.edt_squared(a, w, h, d, 0)
+    assert ref.max() >= 1 << 16 and (ref < 1 << 16).sum() > 0
+    got = bm.euclidean_distance_transform_squared(vol(a, w, h, d), bm.Phase.Foreground)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("tiles", ["40,7,41,7", "41,8,36,9", "16,0,18,1", "24,2,512,64"])
+@pytest.mark.parametrize("phase", [bm.Phase.Foreground, bm.Phase.Background])
+def test_edt_tile_shapes_match_oracle(tiles, phase, monkeypatch):
+    """Y and Z tiles of odd halos and odd row counts (chunks rounded down to
+    even rows, tiles starting one row early) against the oracle."""
+    monkeypatch.setenv("BM_EDT_TILES", tiles)
+    w, h, d = 37, 150, 140
+    a = _structure(11 + int(phase), (d, h, w), 0.5)
+    got = bm.euclidean_distance_transform_squared(vol(a, w, h, d), phase)
+    np.testing.assert_array_equal(got, orc.edt_squared(a, w, h, d, int(phase)))
+
+
+def test_edt_tile_without_chunk_raises(monkeypatch):
+    monkeypatch.setenv("BM_EDT_TILES", "21,10,128,16")
+    a = _structure(3, (40, 60, 20), 0.5)
+    with pytest.raises(bm.BmError, match="y tile chunk below 2 rows"):
+        bm.euclidean_distance_transform_squared(vol(a, 20, 60, 40), bm.Phase.Foreground)
+
+
 # --- local thickness --------------------------------------------------------------
 @pytest.mark.parametrize("seed,shape,frac", [(0, (20, 21, 22), 0.3), (1, (33, 17, 25), 0.7),
                                              (2, (40, 40, 40), 0.85), (3, (7, 60, 9), 0.5)])
