@@ -101,10 +101,11 @@ struct SelectState {
 };
 
 // Tunables of the kernels (chosen per shape class by the autotuner).
-// A Y-pass tile holds up to y_rows rows of 64 columns (u16) in LDS: the whole
-// column when H <= y_rows, otherwise a chunk of y_rows - 2*y_halo rows plus a
-// halo on each side (candidates beyond the halo are read from HBM).  The Z
-// pass is the same with z_rows lines of 32 columns (u32).
+// A Y-pass tile holds up to y_rows rows of 64 columns in LDS (saturated 16-bit
+// values, two rows per word): the whole column when H <= y_rows, otherwise a
+// chunk of y_rows - 2*y_halo rows (rounded down to even) plus a halo on each
+// side (candidates beyond the halo are read from HBM).  The Z pass is the
+// same with z_rows lines of 32 columns.
 // fill_cap bounds the candidates the fill holds in LDS per round (at most
 // kFillCap); a tile with more candidates runs several rounds.
 struct Tuning {
@@ -178,6 +179,12 @@ struct ItemDesc {
   float* map;              // thickness map (kItemMap)
   SelectState* sel;
   unsigned long long* qhist;  // kMaxRanks x 256
+  // the first digit pass's candidates: block b of the pass keeps those of its
+  // range [b qper, (b + 1) qper) of the map at the same offsets of cand, and
+  // their count in qcnt[b] (cand is the propagated map's buffer, dead by then)
+  uint32_t* cand;
+  uint32_t* qcnt;
+  uint64_t qper;
   unsigned long long* diag;   // 8 x 16 words per fill block (kItemDiag)
   uint64_t n;
   uint32_t W, H, D, TX, TY, TZ, ntiles, bound;

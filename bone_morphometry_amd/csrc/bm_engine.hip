@@ -172,6 +172,8 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
                        kStatsBlocks)
                  : 0;
   d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 32768), 8), 1024) : 0;
+  // contiguous ranges of whole 4096-value trips
+  d.qper = quant ? ((d.n + d.qblocks - 1) / d.qblocks + 4095) / 4096 * 4096 : 0;
   if (blocks) {
     blocks[kStEdtX] = d.xblocks;
     blocks[kStEdtY] = d.ygx * d.ygy * d.ygz;
@@ -220,6 +222,10 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
       d.top16 = reinterpret_cast<uint32_t*>(take(kTopBins * sizeof(uint32_t)));
       d.sel = reinterpret_cast<SelectState*>(take(sizeof(SelectState)));
       d.qhist = reinterpret_cast<unsigned long long*>(take(kMaxRanks * 256 * 8));
+      d.qcnt = reinterpret_cast<uint32_t*>(take((size_t)d.qblocks * sizeof(uint32_t)));
+      // the propagated map is dead once the clean-up has run (items with
+      // quantiles have no stage read-back)
+      d.cand = d.h;
     }
     if (d.flags & kItemDiag)
       d.diag = reinterpret_cast<unsigned long long*>(

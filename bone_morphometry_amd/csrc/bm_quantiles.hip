@@ -110,22 +110,33 @@ k_top16_pick(const ItemDesc* __restrict__ items) {
 // of a trip's 16 values have no branch between them.  A bin holding two
 // groups' prefixes (they then differ below bit 16) is flagged and its values
 // search the groups.
+//
+// Block b of the first pass (shift 8) reads the map's range [b qper, (b + 1)
+// qper) and keeps the values that share a prefix (a few per cent of the map)
+// at the same offsets of cand, with their count in qcnt[b]; block b of the
+// second pass reads only those.  Every value sharing a prefix of the second
+// pass shares the first pass's prefix of its rank, so the second pass sees
+// all of them.
 constexpr uint32_t kNoGroup = 0xFFu, kMultiGroup = 0x100u;
 __global__ void __launch_bounds__(256)
 k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
              uint32_t nitems, int shift) {
   __shared__ uint32_t h[kMaxRanks][256];
-  __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng;
+  __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng, s_kept;
   // per top-16 bin: group (kNoGroup: none) | kMultiGroup | prefix bits 0..15 << 16
   __shared__ uint32_t s_btab[kTopBins];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const SelectState* __restrict__ st = gbl(I.sel);
-  BM_GLOBAL const float* __restrict__ map = gbl(I.map);
-  const uint64_t n = I.n;
+  const bool first = shift == 8;
+  BM_GLOBAL uint32_t* __restrict__ cand = gbl(I.cand);
   BM_GLOBAL unsigned long long* __restrict__ hist = gbl(I.qhist);
   const uint32_t nr = st->nranks;
   if (nr == 0) return;
+  const uint64_t lo = (uint64_t)vb * I.qper;
+  const uint64_t hi = first ? min(I.n, lo + I.qper) : lo + gbl(I.qcnt)[vb];
+  BM_GLOBAL const uint32_t* __restrict__ src =
+      first ? (BM_GLOBAL const uint32_t*)gbl(I.map) : (BM_GLOBAL const uint32_t*)cand;
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) (&h[0][0])[i] = 0;
   for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_btab[i] = kNoGroup;
   const uint32_t hi_mask = shift >= 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
@@ -147,24 +158,24 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
       s_btab[bin] = (e & 0xFFu) == kNoGroup ? (g | (pre << 16)) : (e | kMultiGroup);
     }
     s_ng = ng;
+    s_kept = 0;
   }
   __syncthreads();
   const uint32_t ng = s_ng;
   const uint32_t lo_mask = hi_mask & 0xFFFFu;  // prefix bits below the top 16
-  // four consecutive values per 16-byte load (the map is 256-byte aligned),
-  // and kLoads such loads per thread and trip, each coalesced across the
-  // wave and all issued before the first is used
-  constexpr int kLoads = 4;
-  const uint64_t n4 = (n + 3) / 4;
-  const uint64_t stride = (uint64_t)I.qblocks * blockDim.x;
-  const uint64_t iters = (n4 + stride * kLoads - 1) / (stride * kLoads);  // uniform trip count
-  for (uint64_t it = 0; it < iters; ++it) {
+  const uint32_t lane = threadIdx.x & 63;
+  // four consecutive values per 16-byte load (the map and cand are 256-byte
+  // aligned and qper is a multiple of 4096), and kLoads such loads per thread
+  // and trip, each coalesced across the wave and all issued before the first
+  // is used
+  constexpr int kLoads = 4, kTrip = kLoads * 256 * 4;
+  for (uint64_t base = lo; base < hi; base += kTrip) {
     uint32_t b[kLoads][4];
-    const uint64_t trip0 = it * kLoads * stride + (uint64_t)vb * blockDim.x + threadIdx.x;
-    if (4 * (((it + 1) * kLoads) * stride) <= n) {  // the whole trip in bounds (uniform)
+    const uint64_t p0 = base + 4 * threadIdx.x;
+    if (base + kTrip <= hi) {  // the whole trip in range (uniform)
 #pragma unroll
       for (int u = 0; u < kLoads; ++u) {
-        const v4u32 w = *(BM_GLOBAL const v4u32*)(map + 4 * (trip0 + u * stride));
+        const v4u32 w = *(BM_GLOBAL const v4u32*)(src + p0 + 1024 * u);
         b[u][0] = w.x;
         b[u][1] = w.y;
         b[u][2] = w.z;
@@ -172,11 +183,11 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
       }
     } else {
 #pragma unroll
-      for (int u = 0; u < kLoads; ++u) {
-        const uint64_t i4 = trip0 + u * stride;
-        for (int k = 0; k < 4; ++k)
-          b[u][k] = 4 * i4 + k < n ? __float_as_uint(map[4 * i4 + k]) : 0u;
-      }
+      for (int u = 0; u < kLoads; ++u)
+        for (int k = 0; k < 4; ++k) {
+          const uint64_t i = p0 + 1024 * u + k;
+          b[u][k] = i < hi ? src[i] : 0u;
+        }
     }
     uint32_t t[kLoads][4];
 #pragma unroll
@@ -184,6 +195,7 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         t[u][k] = s_btab[min((b[u][k] >> 16) - kTopBase, (uint32_t)kTopBins - 1)];
+    uint32_t keep = 0;  // bit 4u + k: value (u, k) shares a prefix
 #pragma unroll
     for (int u = 0; u < kLoads; ++u)
 #pragma unroll
@@ -202,10 +214,40 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
           }
         }
         // few lanes hold a candidate, so a plain LDS atomic per candidate
-        if (g < ng) atomicAdd(&h[g][(v >> shift) & 0xFFu], 1u);
+        if (g < ng) {
+          atomicAdd(&h[g][(v >> shift) & 0xFFu], 1u);
+          keep |= 1u << (4 * u + k);
+        }
       }
+    if (first) {
+      // this wave's kept values go to consecutive offsets from one LDS
+      // reservation: the lane's offset is the sum of the kept counts of the
+      // lanes below it, bit by bit of the count (at most 16)
+      const uint32_t c = __popc(keep);
+      uint32_t below = 0, total = 0;
+#pragma unroll
+      for (int bit = 0; bit < 5; ++bit) {
+        const unsigned long long m = __ballot((c >> bit) & 1u);
+        const uint32_t lower =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        below += lower << bit;
+        total += (uint32_t)__popcll(m) << bit;
+      }
+      if (total) {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&s_kept, total);
+        at = __shfl(at, 0) + below;
+        BM_GLOBAL uint32_t* __restrict__ dst = cand + lo;
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u)
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (keep & (1u << (4 * u + k))) dst[at++] = b[u][k];
+      }
+    }
   }
   __syncthreads();
+  if (first && threadIdx.x == 0) gbl(I.qcnt)[vb] = s_kept;
   for (uint32_t i = threadIdx.x; i < ng * 256; i += blockDim.x) {
     const uint32_t v = (&h[0][0])[i];
     if (!v) continue;
