@@ -241,6 +241,7 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
 // candidate exists.  fin(p) reads the staged tile without checks and is used
 // when a whole block's positions lie in [in_lo, in_hi).
 constexpr int kEdtRows = 8;
+constexpr int kStageLoads = 4;  // Y/Z tile staging: items per thread with loads in flight
 template <int ROWS, class F, class Fin, class Out>
 __device__ __forceinline__ void window_group(uint32_t q0, uint32_t end, uint32_t n, int in_lo,
                                              int in_hi, F&& f, Fin&& fin, Out&& out) {
@@ -461,31 +462,54 @@ k_edt_y(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const uint32_t np = (yb - ya + 1) / 2;
   const size_t sbase = (size_t)z * W * H;
   const bool vec = (x0 + 64 <= W) && ((W & 7) == 0);
-  for (uint32_t i = threadIdx.x; i < np * 8; i += blockDim.x) {
-    const uint32_t pr = i >> 3, c = (i & 7) * 8, r0 = ya + 2 * pr;  // r0 < H
-    auto p = in + sbase + (size_t)r0 * W + x0 + c;
-    const bool has1 = r0 + 1 < H;
-    uint32_t a[8], b[8];
-    if (vec) {
-      const v4u32 w0 = *(BM_GLOBAL const v4u32*)(p);
-      const v4u32 w1 = has1 ? *(BM_GLOBAL const v4u32*)(p + W) : v4u32{~0u, ~0u, ~0u, ~0u};
-      for (int k = 0; k < 4; ++k) {
-        a[2 * k] = w0[k] & 0xFFFFu;
-        a[2 * k + 1] = w0[k] >> 16;
-        b[2 * k] = w1[k] & 0xFFFFu;
-        b[2 * k + 1] = w1[k] >> 16;
+  auto put = [&](uint32_t pr, uint32_t c, const uint32_t* a, const uint32_t* b) {
+    uint32_t w[8];
+    for (int k = 0; k < 8; ++k) w[k] = pair16(sq16(a[k]), sq16(b[k]));
+    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c]) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c + 4]) = make_uint4(w[4], w[5], w[6], w[7]);
+  };
+  const uint32_t nitem = np * 8;  // (pair of rows, 8 columns)
+  if (vec) {
+    // kStageLoads items' loads in flight per thread before the first is used
+    for (uint32_t i0 = threadIdx.x; i0 < nitem; i0 += kStageLoads * blockDim.x) {
+      v4u32 w0[kStageLoads], w1[kStageLoads];
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const uint32_t i = i0 + u * blockDim.x, r0 = ya + 2 * (i >> 3);  // r0 < H
+        auto p = in + sbase + (size_t)r0 * W + x0 + (i & 7) * 8;
+        w0[u] = w1[u] = v4u32{~0u, ~0u, ~0u, ~0u};
+        if (i < nitem) {
+          w0[u] = *(BM_GLOBAL const v4u32*)(p);
+          if (r0 + 1 < H) w1[u] = *(BM_GLOBAL const v4u32*)(p + W);
+        }
       }
-    } else {
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const uint32_t i = i0 + u * blockDim.x;
+        if (i >= nitem) break;
+        uint32_t a[8], b[8];
+        for (int k = 0; k < 4; ++k) {
+          a[2 * k] = w0[u][k] & 0xFFFFu;
+          a[2 * k + 1] = w0[u][k] >> 16;
+          b[2 * k] = w1[u][k] & 0xFFFFu;
+          b[2 * k + 1] = w1[u][k] >> 16;
+        }
+        put(i >> 3, (i & 7) * 8, a, b);
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < nitem; i += blockDim.x) {
+      const uint32_t pr = i >> 3, c = (i & 7) * 8, r0 = ya + 2 * pr;  // r0 < H
+      auto p = in + sbase + (size_t)r0 * W + x0 + c;
+      const bool has1 = r0 + 1 < H;
+      uint32_t a[8], b[8];
       for (int k = 0; k < 8; ++k) {
         const bool in_row = x0 + c + k < W;
         a[k] = in_row ? p[k] : kInf16;
         b[k] = in_row && has1 ? p[W + k] : kInf16;
       }
+      put(pr, c, a, b);
     }
-    uint32_t w[8];
-    for (int k = 0; k < 8; ++k) w[k] = pair16(sq16(a[k]), sq16(b[k]));
-    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c]) = make_uint4(w[0], w[1], w[2], w[3]);
-    *reinterpret_cast<uint4*>(&ytile[pr * 64 + c + 4]) = make_uint4(w[4], w[5], w[6], w[7]);
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -529,28 +553,47 @@ k_edt_z(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
   const size_t slice = (size_t)W * H;
   const size_t base = (size_t)y * W + x0;
   const bool vec = (x0 + 32 <= W) && ((W & 3) == 0);
-  for (uint32_t i = threadIdx.x; i < np * 8; i += blockDim.x) {
-    const uint32_t pr = i >> 3, c = (i & 7) * 4, r0 = za + 2 * pr;  // r0 < D
-    auto p = in + (size_t)r0 * slice + base + c;
-    const bool has1 = r0 + 1 < D;
-    uint32_t a[4], b[4];
-    if (vec) {
-      const v4u32 w0 = *(BM_GLOBAL const v4u32*)(p);
-      const v4u32 w1 =
-          has1 ? *(BM_GLOBAL const v4u32*)(p + slice) : v4u32{kInf32, kInf32, kInf32, kInf32};
-      for (int k = 0; k < 4; ++k) {
-        a[k] = w0[k];
-        b[k] = w1[k];
+  const uint32_t nitem = np * 8;  // (pair of planes, 4 columns)
+  auto put = [&](uint32_t i, const uint32_t* a, const uint32_t* b) {
+    *reinterpret_cast<uint4*>(&ztile[(i >> 3) * 32 + (i & 7) * 4]) =
+        make_uint4(pair16(a[0], b[0]), pair16(a[1], b[1]), pair16(a[2], b[2]), pair16(a[3], b[3]));
+  };
+  if (vec) {
+    // kStageLoads items' loads in flight per thread before the first is used
+    for (uint32_t i0 = threadIdx.x; i0 < nitem; i0 += kStageLoads * blockDim.x) {
+      v4u32 w0[kStageLoads], w1[kStageLoads];
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const uint32_t i = i0 + u * blockDim.x, r0 = za + 2 * (i >> 3);  // r0 < D
+        auto p = in + (size_t)r0 * slice + base + (i & 7) * 4;
+        w0[u] = w1[u] = v4u32{kInf32, kInf32, kInf32, kInf32};
+        if (i < nitem) {
+          w0[u] = *(BM_GLOBAL const v4u32*)(p);
+          if (r0 + 1 < D) w1[u] = *(BM_GLOBAL const v4u32*)(p + slice);
+        }
       }
-    } else {
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const uint32_t i = i0 + u * blockDim.x;
+        if (i >= nitem) break;
+        const uint32_t a[4] = {w0[u].x, w0[u].y, w0[u].z, w0[u].w};
+        const uint32_t b[4] = {w1[u].x, w1[u].y, w1[u].z, w1[u].w};
+        put(i, a, b);
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < nitem; i += blockDim.x) {
+      const uint32_t c = (i & 7) * 4, r0 = za + 2 * (i >> 3);  // r0 < D
+      auto p = in + (size_t)r0 * slice + base + c;
+      const bool has1 = r0 + 1 < D;
+      uint32_t a[4], b[4];
       for (int k = 0; k < 4; ++k) {
         const bool in_row = x0 + c + k < W;
         a[k] = in_row ? p[k] : kInf32;
         b[k] = in_row && has1 ? p[slice + k] : kInf32;
       }
+      put(i, a, b);
     }
-    *reinterpret_cast<uint4*>(&ztile[pr * 32 + c]) =
-        make_uint4(pair16(a[0], b[0]), pair16(a[1], b[1]), pair16(a[2], b[2]), pair16(a[3], b[3]));
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
