@@ -73,6 +73,7 @@ __device__ __forceinline__ uint32_t bucket_shift(uint32_t max_r2) {
 // histogram; when a tile layer is complete, wavefront k sorts tile k's
 // entries by bucket into the tile's list slot and writes its header, and the
 // tile masks of each plane are written once the plane is done.
+constexpr int kTmplLds = 1024;
 __global__ void __launch_bounds__(512)
 k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
                 uint32_t nitems) {
@@ -83,6 +84,8 @@ k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   __shared__ unsigned long long s_msk[2][8][2];  // plane parity, tile, (selected, target)
   __shared__ uint32_t s_n[8], s_mx[8];
   __shared__ uint16_t s_pref[8][16];
+  // the three templates of r^2 < kTmplLds as 16-bit fields (t1 | t2 << 16, t3)
+  __shared__ uint2 s_tm[kTmplLds];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const uint32_t* __restrict__ d2 = gbl(I.d2);
@@ -105,6 +108,12 @@ k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   const bool valid = x0 + (int)lane < (int)W && y0 + (int)wv < (int)H;
   const uint32_t shift = bucket_shift(st->max_d2);  // one key for every tile
   const uint32_t top = st->max_d2 >> shift;
+  // templates of the phase's radii in LDS when they fit (every template is
+  // then below 2^16: T(m) <= m + 2 sqrt(3 m) + 3 for m < kTmplLds)
+  const bool lds_tmpl = st->max_d2 < (uint32_t)kTmplLds;
+  if (lds_tmpl)
+    for (uint32_t i = threadIdx.x; i <= st->max_d2; i += blockDim.x)
+      s_tm[i] = make_uint2(tmpl[i] | (tmpl[(bound + 1) + i] << 16), tmpl[2 * (bound + 1) + i]);
   for (uint32_t i = t; i < 8u * kBuckets; i += blockDim.x) (&s_cnt[0][0])[i] = 0;
   if (t < 8) {
     s_n[t] = 0;
@@ -150,7 +159,17 @@ k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
     const bool sel = r2 != 0;
     // the neighbours and templates are read by every lane, selected or not
     // (r2 = 0 reads the tables' first entries), so no branch separates reads
-    const uint32_t t1 = tmpl[r2], t2 = tmpl[(bound + 1) + r2], t3 = tmpl[2 * (bound + 1) + r2];
+    uint32_t t1, t2, t3;
+    if (lds_tmpl) {
+      const uint2 tt = s_tm[r2];
+      t1 = tt.x & 0xFFFFu;
+      t2 = tt.x >> 16;
+      t3 = tt.y;
+    } else {
+      t1 = tmpl[r2];
+      t2 = tmpl[(bound + 1) + r2];
+      t3 = tmpl[2 * (bound + 1) + r2];
+    }
     uint32_t nb[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
