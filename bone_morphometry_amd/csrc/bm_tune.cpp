@@ -259,7 +259,8 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
   std::lock_guard<std::mutex> g(e.mutex());
   const std::string arch = device_arch();
   // (rows, halo) of the EDT tiles; a split line needs a chunk of at least kMinZChunk
-  const uint32_t tile_c[][2] = {{64, 8}, {128, 16}, {128, 24}, {128, 32}, {256, 32}, {256, 48}, {512, 64}};
+  const uint32_t tile_c[][2] = {{64, 8},   {128, 16}, {128, 24}, {128, 32}, {192, 16}, {192, 24},
+                                {256, 32}, {256, 48}, {384, 16}, {384, 32}, {512, 64}};
   const uint32_t cap_c[] = {512, 1024};
   std::map<std::string, TuneEntry> found;
   for (const auto& smp : samples) {
