@@ -1155,14 +1155,10 @@ constexpr int kRegX = 68, kRegY = 12, kReg = kRegX * kRegY;  // footprint + 2-vo
 constexpr int kHalX = 66, kHalY = 10, kHal = kHalX * kHalY;  // footprint + 1-voxel halo
 constexpr int kZWords = 30;  // 10 rows x 3 words of one-halo bits
 
-// 1 for a positive contribution, 0 for +0.0f (contributions are never
-// negative): min(bits, 1), written out so that it stays an integer minimum
-// (the compiler turns a comparison into a compare and a carry through VCC).
-__device__ __forceinline__ uint32_t nonzero1(float c) {
-  uint32_t r;
-  asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(__float_as_uint(c)));
-  return r;
-}
+// 1 for a positive contribution, 0 for +0.0f: contributions are +0.0f or
+// finite and at least 2 (bits in [2^30, 2^31)), so bit 30 is the answer (a
+// shift, where a comparison would become a compare and a carry through VCC).
+__device__ __forceinline__ uint32_t nonzero1(float c) { return __float_as_uint(c) >> 30; }
 
 // kLdsDiam: every diameter of the item is in the LDS table (the item's
 // largest r^2 is below kDiamLds), so a lookup needs no bounds test.
@@ -1172,6 +1168,7 @@ struct CleanupLds {
   uint32_t z3[2][kZWords];   // Z3 ring
   float c[3][kHal];          // contribution ring
   float diam[kDiamLds];      // diameters 2*sqrt(m) for small m
+  double rcp[27];            // 1/n rounded to f64, n = 1..26 (rcp[0] unused)
   uint32_t top[kTopBins];    // top-16-bit histogram of the values (quantiles)
 };
 
@@ -1184,6 +1181,7 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
   auto& s_c = L.c;
   auto& s_diam = L.diam;
   auto& s_top = L.top;
+  auto& s_rcp = L.rcp;
   BM_GLOBAL const uint32_t* __restrict__ M = gbl(I.h);
   BM_GLOBAL const unsigned long long* __restrict__ masks = gbl(I.masks);
   BM_GLOBAL const float* __restrict__ diam = gbl(I.diam);
@@ -1196,6 +1194,7 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
   if (top16)
     for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_top[i] = 0;
   if (threadIdx.x < 14) s_zb[threadIdx.x] = 0;  // (the padding word stays 0)
+  if (threadIdx.x < 27) s_rcp[threadIdx.x] = threadIdx.x ? 1.0 / (double)threadIdx.x : 0.0;
   unsigned long long cnt = 0;
   u128 s = 0, s2 = 0;
   uint32_t mxb = 0, flags = 0;
@@ -1359,7 +1358,13 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
             }
             __builtin_amdgcn_sched_barrier(0);
           }
-          tv = count > 0 ? sum / (float)count : dia(m);
+          // sum / count rounded to f32: the f64 product with 1/count rounded to
+          // f64 is within 2^-51.9 of the quotient relative, and the quotient
+          // lies at least 2^-25/26 from every f32 rounding midpoint (an f32
+          // sum over count <= 26 is never one: count times a 25-bit odd
+          // significand needs 25 bits; s - count * m is a nonzero multiple of
+          // half an ulp of the quotient), so both round alike
+          tv = count > 0 ? (float)((double)sum * s_rcp[count]) : dia(m);
         }
         if (tv > 0.0f && tv < INFINITY) acc_value(tv * scale_f32, lsb_exp, cnt, s, s2, mxb, flags);
       }
