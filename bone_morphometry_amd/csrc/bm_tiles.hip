@@ -216,6 +216,7 @@ k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
         const uint32_t tile = ((uint32_t)s / 8 * TY + fy) * TX + tx;
         uint32_t* cnt = s_cnt[kk];
         // exclusive scan of the counts, 8 buckets per lane
+        static_assert(kBuckets == 64 * 8, "the tile-layer scan holds 8 buckets per lane");
         uint32_t local[8], sum = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -381,6 +382,7 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   // same array above 2 kCap; once the candidates are staged they are dead and
   // a round holds kCapX = kCap + 3 kNbr / 2 candidates (key words from kCapX).
   constexpr int kCapX = kCap + 3 * kNbr / 2;
+  static_assert(kNbr % 64 == 0 && kBuckets % 64 == 0, "one-wavefront scans cover whole lanes");
   __shared__ uint32_t s_sort[2 * kCap + 3 * kNbr];
   __shared__ uint32_t s_cnt[kBuckets];          // histogram, then cursors
   __shared__ uint32_t s_start[kBuckets + 1];    // bucket starts over all rounds
