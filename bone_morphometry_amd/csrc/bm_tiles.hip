@@ -1262,18 +1262,29 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
     };
     uint32_t pre0, pre1;
     load2(za - 2, pre0, pre1);
+    // this voxel's tile mask words: its tile column's (x, y) part, plus the
+    // plane's tile layer and index (uniform) per plane
+    BM_GLOBAL const unsigned long long* const mcol =
+        masks + ((size_t)((uint32_t)gy / 8) * TX + (uint32_t)gx / 8) * 16;
+    const size_t layer = (size_t)TY * TX * 16;  // words per tile layer
     uint32_t zxy_a = 0, zxy_b = 0;  // zxy(s-2), zxy(s-1) of this thread's word (t < 30)
+    // contribution ring slots of planes sp - 3, sp - 2 and sp - 1 (plane p in
+    // slot (p - za + 2) mod 3), rotated once per plane
+    uint32_t ca = 0, cb = 1, cc = 2;
     for (int sp = za - 2; sp <= zb + 1; ++sp) {
       const int rel = sp - (za - 2);
+      if (rel > 0) {
+        const uint32_t t0 = ca;
+        ca = cb;
+        cb = cc;
+        cc = t0;
+      }
       // selected voxels of this iteration's output plane sp - 2 (this row of
       // the footprint: 8 tiles), read now so the load is complete by the
       // output phase
       const int pz = sp - 2;
       const unsigned long long selw =
-          out_valid && pz >= za
-              ? masks[((size_t)((uint32_t)pz / 8 * TY + (uint32_t)gy / 8) * TX + (uint32_t)gx / 8) * 16 +
-                      ((uint32_t)pz & 7)]
-              : 0ull;
+          out_valid && pz >= za ? mcol[(uint32_t)pz / 8 * layer + ((uint32_t)pz & 7)] : 0ull;
       // plane sp: M values and zero bits
       uint32_t* sm = s_m[rel & 3];
       sm[t] = pre0;
@@ -1313,7 +1324,7 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
         // c(sp - 1) over the one-halo region
         const uint32_t* m1 = s_m[(rel - 1) & 3];
         const uint32_t* z3 = s_z3[(rel + 1) & 1];
-        float* c1 = s_c[(rel + 2) % 3];  // slot of plane sp - 1
+        float* c1 = s_c[cc];  // slot of plane sp - 1
         auto contrib = [&](uint32_t e) {
           asm volatile("" : "+v"(e));
           const int hx = (int)(e % kHalX), hy = (int)(e / kHalX);
@@ -1341,7 +1352,7 @@ cleanup_stream(const ItemDesc& I, uint32_t vb, CleanupLds& L) {
           // non-surface neighbours in NEIGHBORS_26 order (two groups of 13
           // reads in flight; non-contributors hold +0.0f, which leaves the f32
           // sum bit for bit as skipping them would)
-          const float* cz[3] = {s_c[(rel + 0) % 3], s_c[(rel + 1) % 3], s_c[(rel + 2) % 3]};
+          const float* cz[3] = {s_c[ca], s_c[cb], s_c[cc]};
           const int c0 = hy * kHalX + hx;
           float sum = 0.0f;
           uint32_t count = 0;
