@@ -141,6 +141,22 @@ __device__ inline void block_reduce_store(unsigned long long cnt, u128 s, u128 s
   }
 }
 
+// Inclusive sum over lanes 0..lane of a converged wavefront, without LDS:
+// Hillis-Steele steps of 1, 2, 4 and 8 lanes inside each row of 16 (DPP row
+// shifts; a lane with no source lane adds 0), then each row adds the totals of
+// the rows below it (lanes 15, 31 and 47 read as scalars).
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 15);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 47);
+  const uint32_t lane = threadIdx.x & 63;
+  return x + (lane >= 16 ? r0 : 0u) + (lane >= 32 ? r1 : 0u) + (lane >= 48 ? r2 : 0u);
+}
+
 // Adds 1 to hist[bin] for every active lane, one LDS atomic per distinct bin of
 // the wavefront (thickness maps repeat values, so lanes often share a bin).
 __device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint32_t bin) {
@@ -162,8 +178,9 @@ __device__ __forceinline__ void wave_hist_add(uint32_t* hist, bool active, uint3
 __device__ __forceinline__ void wave_hist_add_runs(uint32_t* hist, bool active, uint32_t bin) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t key = active ? bin : 0xFFFFFFFFu;
-  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
-  const bool head = active && (lane == 0 || prev != key);
+  // the key of the lane below (DPP wave shift; lane 0 gets 0xFFFFFFFF, no key)
+  const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x138, 0xF, 0xF, false);
+  const bool head = active && prev != key;
   // a run ends before the next head or the next inactive lane
   const unsigned long long stop = __ballot(head) | ~__ballot(active);
   if (head) {

@@ -223,18 +223,14 @@ k_ridges_stream(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
           local[i] = cnt[lane * 8 + i];
           sum += local[i];
         }
-        uint32_t x = sum;
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t y = __shfl_up(x, off);
-          if (lane >= (uint32_t)off) x += y;
-        }
+        const uint32_t x = wave_inclusive_sum(sum);
         uint32_t run = x - sum;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           cnt[lane * 8 + i] = run;
           run += local[i];
         }
-        const uint32_t count = (uint32_t)__shfl((int)x, 63);
+        const uint32_t count = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -339,11 +335,7 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t n, uint32_t lan
     local[i] = j < n ? val(j) : 0u;
     sum += local[i];
   }
-  uint32_t x = sum;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= (uint32_t)off) x += y;
-  }
+  const uint32_t x = wave_inclusive_sum(sum);
   uint32_t run = x - sum;
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
@@ -351,7 +343,7 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t n, uint32_t lan
     if (j < n) out(j, run);
     run += local[i];
   }
-  return (uint32_t)__shfl((int)x, 63);
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
 // The same scan with the fewest values per lane that hold n (<= 64 kMax).
