@@ -106,8 +106,10 @@ k_top16_pick(const ItemDesc* __restrict__ items) {
 // A value can share a prefix only if its top 16 bits name the top-16 bin of
 // one (a bin of the clean-up's histogram).  A table over those bins gives the
 // bin's group and the group's prefix bits below the top 16, so a value is
-// classified with one LDS read at an index clamped into the table; the reads
-// of a trip's 16 values have no branch between them.  A bin holding two
+// classified with one LDS read at an index clamped into the table, whose
+// extra last entry (no group) takes every value outside the histogram's range
+// (zero, beyond it, infinite: their top 16 bits minus kTopBase wrap or exceed
+// kTopBins); the reads of a trip's 16 values have no branch between them.  A bin holding two
 // groups' prefixes (they then differ below bit 16) is flagged and its values
 // search the groups.
 //
@@ -122,9 +124,9 @@ __global__ void __launch_bounds__(256)
 k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
              uint32_t nitems, int shift) {
   __shared__ uint32_t h[kMaxRanks][256];
-  __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng, s_kept;
+  __shared__ uint32_t gpre[kMaxRanks], grow[kMaxRanks], s_ng, s_kept, s_multi;
   // per top-16 bin: group (kNoGroup: none) | kMultiGroup | prefix bits 0..15 << 16
-  __shared__ uint32_t s_btab[kTopBins];
+  __shared__ uint32_t s_btab[kTopBins + 1];
   uint32_t vb;
   const ItemDesc& I = items[item_of(starts, nitems, blockIdx.x, vb)];
   BM_GLOBAL const SelectState* __restrict__ st = gbl(I.sel);
@@ -138,11 +140,11 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
   BM_GLOBAL const uint32_t* __restrict__ src =
       first ? (BM_GLOBAL const uint32_t*)gbl(I.map) : (BM_GLOBAL const uint32_t*)cand;
   for (uint32_t i = threadIdx.x; i < kMaxRanks * 256; i += blockDim.x) (&h[0][0])[i] = 0;
-  for (uint32_t i = threadIdx.x; i < (uint32_t)kTopBins; i += blockDim.x) s_btab[i] = kNoGroup;
+  for (uint32_t i = threadIdx.x; i <= (uint32_t)kTopBins; i += blockDim.x) s_btab[i] = kNoGroup;
   const uint32_t hi_mask = shift >= 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
   __syncthreads();
   if (threadIdx.x == 0) {  // distinct prefixes, and the table of their top-16 bins
-    uint32_t ng = 0;
+    uint32_t ng = 0, multi = 0;
     for (uint32_t r = 0; r < nr; ++r) {
       const uint32_t pre = st->prefix[r];
       uint32_t g = 0;
@@ -156,12 +158,15 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
       if (bin >= (uint32_t)kTopBins) continue;
       const uint32_t e = s_btab[bin];
       s_btab[bin] = (e & 0xFFu) == kNoGroup ? (g | (pre << 16)) : (e | kMultiGroup);
+      multi |= (e & 0xFFu) != kNoGroup;
     }
     s_ng = ng;
+    s_multi = multi;
     s_kept = 0;
   }
   __syncthreads();
   const uint32_t ng = s_ng;
+  const bool any_multi = s_multi != 0;  // a bin holds two groups' prefixes (rare)
   const uint32_t lo_mask = hi_mask & 0xFFFFu;  // prefix bits below the top 16
   const uint32_t lane = threadIdx.x & 63;
   // four consecutive values per 16-byte load (the map and cand are 256-byte
@@ -194,19 +199,18 @@ k_digit_hist(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ st
     for (int u = 0; u < kLoads; ++u)
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        t[u][k] = s_btab[min((b[u][k] >> 16) - kTopBase, (uint32_t)kTopBins - 1)];
+        t[u][k] = s_btab[min((b[u][k] >> 16) - kTopBase, (uint32_t)kTopBins)];
     uint32_t keep = 0;  // bit 4u + k: value (u, k) shares a prefix
 #pragma unroll
     for (int u = 0; u < kLoads; ++u)
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t v = b[u][k];
-        // positive finite values in the table's range only
-        const bool in = v != 0 && v < 0x7F800000u && (v >> 16) - kTopBase < (uint32_t)kTopBins;
+        // (values outside the table's range read its last entry: no group)
         const uint32_t e = t[u][k];
-        uint32_t g = in && (v & lo_mask) == (e >> 16) ? (e & 0xFFu) : kNoGroup;
-        const bool multi = in && (e & kMultiGroup);
-        if (__ballot(multi)) {
+        uint32_t g = (v & lo_mask) == (e >> 16) ? (e & 0xFFu) : kNoGroup;
+        const bool multi = e & kMultiGroup;
+        if (any_multi && __ballot(multi)) {
           if (multi) {
             g = kNoGroup;
             for (uint32_t q = 0; q < ng; ++q)
