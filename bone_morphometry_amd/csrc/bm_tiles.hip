@@ -728,23 +728,25 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   while (b0 <= gtop) {
     ++rounds;
     uint32_t b1 = b0 + 1;
-    if (s_start[b1] - s_start[b0] > cap) {
+    const uint32_t start0 = s_start[b0];
+    if (n - start0 <= cap) {
+      b1 = gtop + 1;  // the rest fits: the last round (the usual case)
+    } else if (s_start[b1] - start0 > cap) {
       if (threadIdx.x < ntx) {  // one bucket alone exceeds LDS: exact slow path
         overflow[1 + atomicAdd((uint32_t*)&overflow[0], 1u)] = t + threadIdx.x;
         atomicAdd((uint32_t*)&st->overflow_tiles, 1u);
       }
       return;
-    }
-    {  // largest b1 <= gtop + 1 with s_start[b1] - s_start[b0] <= cap (starts are monotone)
+    } else {  // largest b1 <= gtop + 1 with s_start[b1] - s_start[b0] <= cap (starts are monotone)
       uint32_t lo = b1, hi = gtop + 1;
       while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] - s_start[b0] <= cap) lo = mid;
+        if (s_start[mid] - start0 <= cap) lo = mid;
         else hi = mid - 1;
       }
       b1 = lo;
     }
-    const uint32_t base = s_start[b0], cnt_round = s_start[b1] - base;
+    const uint32_t base = start0, cnt_round = s_start[b1] - base;
     const unsigned long long c3 = diag ? wall_clock64() : 0ull;
     // P3: place this round's candidates at their sorted positions (any order
     // inside a bucket: the scan below finishes a bucket before it leaves, and
