@@ -521,21 +521,20 @@ k_fill_gather_t(const ItemDesc* __restrict__ items, const uint32_t* __restrict__
   }
   const uint32_t nn = s_nn;
   // chunk offsets of the tiles in reach (64-entry chunks of their prefixes),
-  // scanned by one wavefront
+  // scanned by one wavefront, which also writes each tile's chunk descriptors
   if (wv == 0) {
     const uint32_t tot = wave_scan_sized<kNbr / 64>(
         nn, lane, [&](uint32_t j) { return ((s_nb_word[j] >> 18) + 64) >> 6; },
-        [&](uint32_t j, uint32_t p) { s_off[j] = p; });
+        [&](uint32_t j, uint32_t p) {
+          s_off[j] = p;
+          const uint32_t u = s_nb_tile[j], w = s_nb_word[j];
+          const uint32_t c1 = min(p + (((w >> 18) + 64u) >> 6), (uint32_t)kDesc);
+          for (uint32_t c = p; c < c1; ++c) s_cmap[c] = make_uint2(u | ((c - p) << 24), w);
+        });
     if (lane == 0) s_nchunks = tot;
   }
   __syncthreads();
   const uint32_t nchunks = s_nchunks;
-  for (uint32_t j = threadIdx.x; j < nn; j += blockDim.x) {
-    const uint32_t c0 = s_off[j], c1 = min(c0 + (((s_nb_word[j] >> 18) + 64u) >> 6), (uint32_t)kDesc);
-    const uint32_t u = s_nb_tile[j], w = s_nb_word[j];
-    for (uint32_t c = c0; c < c1; ++c) s_cmap[c] = make_uint2(u | ((c - c0) << 24), w);
-  }
-  __syncthreads();
   const unsigned long long tc = diag ? wall_clock64() : 0ull;
   unsigned long long td = tc;
   const uint32_t kshift = gshift + 16;  // bucket of a key word
