@@ -28,6 +28,15 @@ namespace bm {
 // ---------------------------------------------------------------------------
 // X pass.  Sites bits of 8 consecutive voxels x..x+7 of a row; bit j set when
 // voxel x+j exists and is not selected.  Counts selected voxels.
+// 0x80 in each zero byte of x, 0 elsewhere (no carry crosses a byte: the low
+// seven bits of a byte plus 0x7F stay below 0x100).
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+  return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+}
+// Bits 0..3 from bits 7, 15, 23 and 31 of m (m holds no other bits).
+__device__ __forceinline__ uint32_t byte_msbs(uint32_t m) {
+  return ((m >> 7) | (m >> 14) | (m >> 21) | (m >> 28)) & 0xFu;
+}
 __device__ __forceinline__ uint32_t site_bits8(BM_GLOBAL const uint8_t* __restrict__ data,
                                                BM_GLOBAL const uint8_t* __restrict__ roi, uint32_t want,
                                                size_t rb, uint32_t x, uint32_t W,
@@ -43,13 +52,12 @@ __device__ __forceinline__ uint32_t site_bits8(BM_GLOBAL const uint8_t* __restri
       const v2u32 ra = *(BM_GLOBAL const v2u32*)(roi + rb + x);
       r = make_uint2(ra.x, ra.y);
     }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      uint8_t vb = (uint8_t)((j < 4 ? v.x : v.y) >> (8 * (j & 3)));
-      uint8_t rbt = (uint8_t)((j < 4 ? r.x : r.y) >> (8 * (j & 3)));
-      if (selected_voxel(vb, rbt, want)) ++selcnt;
-      else bits |= 1u << j;
-    }
+    // per byte (bit 7): selected = ROI byte nonzero and data byte nonzero as wanted
+    const uint32_t zv0 = zero_bytes(v.x), zv1 = zero_bytes(v.y);
+    const uint32_t sel0 = ~zero_bytes(r.x) & (want ? ~zv0 : zv0) & 0x80808080u;
+    const uint32_t sel1 = ~zero_bytes(r.y) & (want ? ~zv1 : zv1) & 0x80808080u;
+    selcnt += __popc(sel0) + __popc(sel1);
+    bits = byte_msbs(sel0 ^ 0x80808080u) | (byte_msbs(sel1 ^ 0x80808080u) << 4);
   } else {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -113,15 +121,21 @@ k_edt_x(const ItemDesc* __restrict__ items, const uint32_t* __restrict__ starts,
     const int Rc = higher ? fromR : carryR;
     if (has) carryL = __shfl(last, 63 - __clzll(has));
     if (x0 < W) {
-      uint16_t o[8];
+      // nearest site at or left of x0 + j, and at or right of it, by one
+      // step per voxel from Lc and from Rc
+      int Ls[8];
+      int Lrun = Lc, Rrun = Rc;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
+        Lrun = (bits >> j) & 1u ? (int)x0 + j : Lrun;
+        Ls[j] = Lrun;
+      }
+      uint16_t o[8];
+#pragma unroll
+      for (int j = 7; j >= 0; --j) {
+        Rrun = (bits >> j) & 1u ? (int)x0 + j : Rrun;
         const int x = (int)x0 + j;
-        const uint32_t mL = bits & ((2u << j) - 1u);
-        const int L = mL ? (int)(x0 + 31 - __clz(mL)) : Lc;
-        const uint32_t mR = bits & (~0u << j);
-        const int R = mR ? (int)(x0 + __ffs(mR) - 1) : Rc;
-        const int dist = min(x - L, R - x);
+        const int dist = min(x - Ls[j], Rrun - x);
         o[j] = dist > 65534 ? (uint16_t)kInf16 : (uint16_t)dist;
       }
       auto p = out + rb + x0;
