@@ -28,6 +28,7 @@ EXPORTS = (
     "bm_local_thickness_device", "bm_analyze_thickness_spacing_device",
     "bm_analyze_thickness_spacing_device_noalloc",
     "bm_local_thickness_stages", "bm_set_stats_order", "bm_profile_enable", "bm_profile_read",
+    "bm_profile_only",
     "bm_batch_create", "bm_batch_run", "bm_batch_device_bytes", "bm_batch_destroy",
     "bm_autotune", "bm_tuning_load",
     "bm_server_create", "bm_server_submit", "bm_server_poll", "bm_server_cancel",
@@ -197,6 +198,7 @@ def lib():
         L.bm_read_isotropic_spacing_um.argtypes = [CS, ctypes.POINTER(D)]
         L.bm_profile_enable.argtypes = [I]
         L.bm_profile_read.argtypes = [ctypes.POINTER(BmKernelTime), I, ctypes.POINTER(I)]
+        L.bm_profile_only.argtypes = [ctypes.c_char_p]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("bm_last_error", "bm_version"):
@@ -234,6 +236,11 @@ def device_count():
 
 def profile_enable(on=True):
     lib().bm_profile_enable(1 if on else 0)
+
+
+def profile_only(names=None):
+    """Times only the kernels named (an iterable of names); None: every kernel."""
+    lib().bm_profile_only(",".join(names).encode() if names else None)
 
 
 def profile_read(max_kernels=64):

@@ -650,6 +650,10 @@ int bm_profile_enable(int on) {
   return guarded([&] { profile_enable(on != 0); });
 }
 
+int bm_profile_only(const char* names) {
+  return guarded([&] { profile_only(names); });
+}
+
 int bm_profile_read(bm_kernel_time* out, int max, int* count) {
   return guarded([&] {
     static_assert(sizeof(bm_kernel_time) == sizeof(KernelTime), "layout");

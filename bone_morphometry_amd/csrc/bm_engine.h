@@ -343,6 +343,10 @@ struct KernelTime {
 };
 void profile_enable(bool on);
 bool profile_enabled();
+// Whether launches of the kernel named name are timed: profiling on and the
+// name in the set given to profile_only (every name when the set is empty).
+bool profile_enabled_for(const char* name);
+void profile_only(const char* names);  // comma-separated names; null or "" = all
 hipEvent_t profile_begin();
 void profile_end(const char* name, hipEvent_t start, hipStream_t s);
 int profile_read(KernelTime* out, int max);  // synchronises, aggregates, resets
@@ -352,7 +356,7 @@ void profile_suspend(bool on);
 // Launch a kernel, bracketed by profiling events when profiling is on.
 #define BM_LAUNCH(name, kernel, grid, block, lds, stream, ...)           \
   do {                                                                   \
-    hipEvent_t bm_ev0_ = ::bm::profile_enabled() ? ::bm::profile_begin() \
+    hipEvent_t bm_ev0_ = ::bm::profile_enabled_for(name) ? ::bm::profile_begin() \
                                                  : nullptr;              \
     if (bm_ev0_) BM_HIP(hipEventRecord(bm_ev0_, stream));                \
     kernel<<<grid, block, lds, stream>>>(__VA_ARGS__);                   \

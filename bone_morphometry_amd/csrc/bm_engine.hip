@@ -490,6 +490,24 @@ void profile_suspend(bool on) {
 
 bool profile_enabled() { return g_prof_on && !g_prof_suspended; }
 
+// ",name1,name2," (empty: every kernel)
+static std::string& prof_only() {
+  static std::string* s = new std::string();
+  return *s;
+}
+
+void profile_only(const char* names) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  prof_only() = names && *names ? std::string(",") + names + "," : std::string();
+}
+
+bool profile_enabled_for(const char* name) {
+  if (!profile_enabled()) return false;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  const std::string& only = prof_only();
+  return only.empty() || only.find(std::string(",") + name + ",") != std::string::npos;
+}
+
 hipEvent_t profile_begin() {
   std::lock_guard<std::mutex> g(g_prof_mu);
   return take_event();

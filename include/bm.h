@@ -402,6 +402,11 @@ typedef struct bm_kernel_time {
 } bm_kernel_time;
 int bm_profile_enable(int on);
 int bm_profile_read(bm_kernel_time *out, int max, int *count);
+/* Restricts the timed launches to the kernels named in names (comma-separated,
+ * the names bm_profile_read reports); null or "" times every kernel.  Fewer
+ * events keep the timed region closer to an unprofiled run (no counterpart in
+ * the reference). */
+int bm_profile_only(const char *names);
 
 #ifdef __cplusplus
 }
