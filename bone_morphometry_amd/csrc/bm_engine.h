@@ -121,6 +121,13 @@ struct Tuning {
   // threads per 16x8x8 pair sharing one candidate gathering).  One value per
   // plan: the first item's.
   uint32_t fill_tiles = 1;
+  // Most clean-up blocks per item (a multiple of 8 up to kStatsBlocks; 0:
+  // kStatsBlocks).  384 makes the two phases of a volume one resident wave of
+  // blocks (256 CUs x 3 blocks of 512 threads), so no block starts after
+  // others have finished: for 512^3 volumes (4096 footprints per item) that
+  // beats 1024 blocks in 2.67 rounds; for 1024x1024x1536 (49152 per item)
+  // 1024 blocks are faster.  Set in the table by measurement, not swept.
+  uint32_t cleanup_blocks = 0;
   // 1: the fill tests a candidate with one 4 x u8 dot product when the
   // item's radii are at most 64; 0: the packed-position test at every radius
   // (both forms give identical maps; 0 is a test override, not tuned).

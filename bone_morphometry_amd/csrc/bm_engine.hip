@@ -167,10 +167,13 @@ static size_t lay_out(const ItemSpec& s, ItemDesc& d, char* base, uint32_t* bloc
       d.divs[i][3] = f.s2;
     }
   }
-  d.cblocks = lt ? std::min<uint32_t>(
-                       std::max<uint32_t>(round8(cleanup_work_items(d.W, d.H, d.D)), 8),
-                       kStatsBlocks)
-                 : 0;
+  const uint32_t cmax = t.cleanup_blocks
+                            ? std::min<uint32_t>(std::max<uint32_t>(t.cleanup_blocks & ~7u, 8),
+                                                 kStatsBlocks)
+                            : (uint32_t)kStatsBlocks;
+  d.cblocks =
+      lt ? std::min<uint32_t>(std::max<uint32_t>(round8(cleanup_work_items(d.W, d.H, d.D)), 8), cmax)
+         : 0;
   d.qblocks = quant ? std::min<uint32_t>(std::max<uint32_t>((uint32_t)(d.n / 32768), 8), 1024) : 0;
   // contiguous ranges of whole 4096-value trips
   d.qper = quant ? ((d.n + d.qblocks - 1) / d.qblocks + 4095) / 4096 * 4096 : 0;

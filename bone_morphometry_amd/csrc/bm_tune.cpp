@@ -148,6 +148,7 @@ void parse_cache(const std::string& text) {
         get("fill_cap", e.t.fill_cap);
         get("tile_order", e.t.tile_order);
         get("fill_tiles", e.t.fill_tiles);
+        get("cleanup_blocks", e.t.cleanup_blocks);
         for (const auto& kv : m)
           if (kv.first.find("_ms") != std::string::npos) e.extra[kv.first] = kv.second;
         classes[cls] = e;
@@ -171,6 +172,7 @@ std::string dump_cache() {
         << ", \"y_halo\": " << t.y_halo << ", \"z_rows\": " << t.z_rows
         << ", \"z_halo\": " << t.z_halo << ", \"fill_cap\": " << t.fill_cap
         << ", \"tile_order\": " << t.tile_order << ", \"fill_tiles\": " << t.fill_tiles;
+      if (t.cleanup_blocks) o << ", \"cleanup_blocks\": " << t.cleanup_blocks;
       for (const auto& kv : c.second.extra) {
         char buf[64];
         snprintf(buf, sizeof buf, "%.4f", kv.second);
