@@ -338,7 +338,13 @@ void autotune(const std::vector<TuneSample>& samples, const std::string& path, b
   } catch (const std::exception&) {
     g_tune.clear();
   }
-  for (auto& kv : found) g_tune[arch][kv.first] = kv.second;
+  // (cleanup_blocks is not swept: a class keeps the value its entry had)
+  for (auto& kv : found) {
+    TuneEntry& slot = g_tune[arch][kv.first];
+    const uint32_t cleanup_blocks = slot.t.cleanup_blocks;
+    slot = kv.second;
+    slot.t.cleanup_blocks = cleanup_blocks;
+  }
   std::ofstream f(path);
   if (!f) throw Error("cannot write the tuning cache " + path);
   f << dump_cache();
